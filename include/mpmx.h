@@ -1,0 +1,345 @@
+/*
+ * mpmx.h — C-ABI of the MI355X-native extended Matter MPM step.
+ *
+ * The reference has no FFI layer: its boundary is the C++ class `Simulation`
+ * (matter/src/simulation/simulation.hpp:25-281), driven by the renderer adapter
+ * `MPMIntegrationSim` (raytracing/src/mpm/MPMIntegrationSim.cpp:16-116) and by
+ * tests through public members + `simulate()`.  This header is the thin C layer
+ * that a `Simulation` host facade (pigment_mixing_amd/csrc/host/simulation.hpp)
+ * calls; every entry point cites the reference member it replaces.
+ *
+ * Conventions
+ *   - Every function returns an mpmx_status (0 = OK).  No C++ exception crosses
+ *     the ABI.  On error, mpmx_last_error(h) returns a message.
+ *   - Plain pointers and sizes only.  Host arrays are borrowed for the call.
+ *     Arrays are structure-of-arrays, row-major per particle:
+ *       x, v      : n*3 floats   (x0,y0,z0, x1,y1,z1, ...)
+ *       F, C      : n*9 floats   row-major 3x3 per particle (F[r*3+c])
+ *       pig       : n*7 floats   Mixbox latent per particle
+ *       scalars   : n floats
+ *       active    : n bytes (0/1)
+ *   - Device memory is owned by the handle.  One handle per host thread; calls on
+ *     one handle are serialised on the handle's HIP stream.  Not reentrant, like
+ *     the reference's single GL-thread use (raytracing/src/main.cpp:270-285).
+ *   - FP32 throughout (matter/src/tools.hpp:19, `typedef float T`), 3-D, quadratic
+ *     B-spline (tools.hpp:20-21).
+ */
+#ifndef MPMX_H
+#define MPMX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPMX_ABI_VERSION 1
+
+typedef enum {
+  MPMX_OK = 0,
+  MPMX_ERR_ARG = 1,       /* invalid argument / size                              */
+  MPMX_ERR_HIP = 2,       /* HIP runtime error (no device, launch failure, ...)   */
+  MPMX_ERR_STATE = 3,     /* call out of order (e.g. step before prepare)         */
+  MPMX_ERR_DEVICE = 4,    /* device status word set (reference `exit = 1`)        */
+  MPMX_ERR_CAPACITY = 5,  /* exceeds a fixed capacity                              */
+  MPMX_ERR_IO = 6         /* file could not be read / written                      */
+} mpmx_status;
+
+/* Boundary conditions — reference enum class BC { NoSlip, SlipStick, SlipFree }
+ * (matter/src/tools.hpp:59). */
+typedef enum { MPMX_BC_NOSLIP = 0, MPMX_BC_SLIPSTICK = 1, MPMX_BC_SLIPFREE = 2 } mpmx_bc;
+
+/* Plate types — reference enum class PlateType (matter/src/tools.hpp:55). */
+typedef enum {
+  MPMX_PLATE_TOP = 0, MPMX_PLATE_BOTTOM = 1, MPMX_PLATE_LEFT = 2,
+  MPMX_PLATE_RIGHT = 3, MPMX_PLATE_FRONT = 4, MPMX_PLATE_BACK = 5
+} mpmx_plate_type;
+
+/* Plastic models on the path (reference enum class PlasticModel, tools.hpp:57). */
+typedef enum { MPMX_PLASTIC_NONE = 0, MPMX_PLASTIC_VMVISC = 5 } mpmx_plastic_model;
+
+/* Return-map closure used inside the step. */
+typedef enum {
+  MPMX_CLOSURE_EXACT = 0,    /* von Mises/Perzyna return map (plasticity.cpp:163-238) */
+  MPMX_CLOSURE_LEARNED = 1   /* BF16 MLP on MFMA, exact map as per-particle fallback */
+} mpmx_closure;
+
+/* Knife (palette knife / spatula) shapes. */
+typedef enum {
+  MPMX_KNIFE_SLAB = 0,       /* exact box SDF, half extents hx,hy,hz                 */
+  MPMX_KNIFE_SPATULA = 1     /* reference trapezoid SDF (object_spatula.hpp:40-82)   */
+} mpmx_knife_shape;
+
+/* Knife kinematics (replaces ObjectSpatula::move keyframes, object_spatula.hpp:164-233). */
+typedef enum {
+  MPMX_MOTION_STATIC = 0,
+  MPMX_MOTION_LINEAR = 1,    /* centre = p0 + velocity * t                           */
+  MPMX_MOTION_CIRCULAR = 2,  /* centre on a circle in the xz plane, heading tangent */
+  MPMX_MOTION_LINEAR_THEN_CIRCULAR = 3, /* linear for t < t_switch, then circular    */
+  MPMX_MOTION_KEYFRAMES = 4  /* 60 fps pos+quat frames, lerp/slerp (spatula_motion.bin) */
+} mpmx_motion;
+
+/* Scene/material configuration.  Field names follow Simulation's public members
+ * (simulation.hpp:38-126) and the keys of sim_config.json / pigment_config.json
+ * (loaded at simulation_units.cpp:32-80). */
+typedef struct {
+  /* material — sim_config.json: E, nu, rho, flip_ratio, perzyna_exp, perzyna_visc */
+  float E, nu, rho;
+  float flip_ratio;          /* [-1,0): AFLIP blend, [0,1]: PIC/FLIP (simulation.hpp:64) */
+  int   plastic_model;       /* mpmx_plastic_model                                    */
+  float q_max, q_min;        /* von Mises yield (Pa), simulation.hpp:94-95            */
+  float xi;                  /* softening (simulation.hpp:91)                         */
+  float p_min;               /* capped von Mises (simulation.hpp:96)                  */
+  float perzyna_exp, perzyna_visc; /* simulation.hpp:103-104; visc is a time (s)      */
+  float q_prefac;            /* 1/sqrt(2) (simulation.hpp:118)                        */
+  float cfl;                 /* sim_config.json "cfl" (used by adaptive dt only)      */
+  float M;                   /* tan(friction_angle) (simulation_units.cpp:60-61)      */
+  float q_cohesion;          /* sim_config.json "q_cohesion"                          */
+  float gravity[3];
+  /* pigment_config.json */
+  float D_max, D_edge0, D_edge1, start_boost_time, end_boost_time, boost_factor;
+  /* grid / time */
+  float dx;
+  float ppc;                 /* particles per cell: particle_volume = dx^3/ppc        */
+  int   grid_dims[3];        /* node counts; 0 = auto from bbox (remesh.cpp:131-153)  */
+  int   origin_mode;         /* 0: bbox_min - 2dx (remesh.cpp:114-137); 1: explicit   */
+  float grid_origin[3];      /* used when origin_mode == 1                            */
+  float grid_reference_point[3]; /* >= 1e10 disables (simulation.hpp:52)              */
+  float dt;                  /* fixed time step                                       */
+  float fps;                 /* frame rate for frameFinished (simulation_units.cpp:235) */
+  int   use_musl;            /* simulation.hpp:50                                     */
+  int   use_particle_boundaries;           /* simulation.hpp:55-57                    */
+  float particle_boundary_min[3], particle_boundary_max[3];
+  int   closure;             /* mpmx_closure                                          */
+  float closure_tol;         /* relative yield-residual tolerance for the fallback    */
+  int   n_threads;           /* CPU threads (oracle/baseline only; ignored on GPU)    */
+} mpmx_config;
+
+/* Axis-aligned plate — ObjectPlate (matter/src/objects/object_plate.hpp:13-78). */
+typedef struct {
+  float pos_object;
+  int   plate_type;          /* mpmx_plate_type */
+  int   bc;                  /* mpmx_bc         */
+  float friction;
+  float pos_lower, pos_upper;
+  float v[3];                /* vx_object, vy_object, vz_object */
+} mpmx_plate;
+
+/* Knife description + motion. */
+typedef struct {
+  int   shape;               /* mpmx_knife_shape */
+  float half[3];             /* slab half extents (m); spatula: halfWidth, halfThickness, halfLength */
+  float offset;              /* spatula offset (object_spatula.hpp:23)                */
+  int   bc;                  /* mpmx_bc         */
+  float friction;
+  int   motion;              /* mpmx_motion     */
+  float p0[3];               /* centre at t = 0 (linear), or circle centre (circular) */
+  float tilt_axis[3];        /* body rotation applied before the heading rotation     */
+  float tilt_angle;          /* radians                                               */
+  float velocity[3];         /* linear motion velocity (m/s)                          */
+  float radius;              /* circular motion radius (m)                            */
+  float speed;               /* circular motion speed (m/s)                           */
+  float phase0;              /* circular start angle (rad)                            */
+  float t_switch;            /* LINEAR_THEN_CIRCULAR: switch time (s)                 */
+} mpmx_knife;
+
+/* Per-step knife state, as held by ObjectSpatula (transform, invTransform,
+ * velocity, angularVelocity; object_spatula.hpp:245-270).  Matrices are 3x4
+ * row-major (rotation | translation). */
+typedef struct {
+  float T[12];               /* local -> world                                        */
+  float Tinv[12];            /* world -> local                                        */
+  float Nmat[9];             /* (T.linear^-1)^T, row-major (object_spatula.hpp:114)   */
+  float v[3];                /* vx_object, vy_object, vz_object                       */
+  float w[3];                /* angularVelocity                                       */
+} mpmx_knife_state;
+
+/* Counters surfaced by mpmx_stats — reference "Proj: plastic_count / Np"
+ * (deformation_update.cpp:49-50), the `exit` flag (simulation.hpp:30), and timings
+ * (simulation.cpp:210-215). */
+typedef struct {
+  int64_t step;              /* completed steps                                       */
+  double  time;              /* simulated time (s)                                    */
+  int64_t n_particles;
+  int64_t n_active;          /* active particles                                      */
+  int64_t plastic_count;     /* particles projected in the last step                  */
+  int64_t fallback_count;    /* learned closure: particles sent to the exact map      */
+  int64_t active_nodes;      /* nodes with mass > 0 in the last step                  */
+  int64_t active_blocks;     /* 4x4x4 node blocks touched in the last step            */
+  int64_t stray_count;       /* particles that left their bin tile (slow path)        */
+  uint32_t device_error;     /* bit mask; nonzero <=> reference exit = 1              */
+  float   ms_last_step;      /* wall ms of the last mpmx_step call / its steps        */
+} mpmx_stats;
+
+typedef struct mpmx_handle mpmx_handle;
+
+/* ---------------- lifecycle ---------------- */
+
+/* Defaults = Simulation's constructor values (simulation.hpp:38-126) with
+ * the reference files' JSON defaults; replaces Simulation::Simulation(). */
+void mpmx_config_default(mpmx_config* cfg);
+
+/* Load sim_config.json / pigment_config.json into cfg as
+ * Simulation::setupScene does (simulation_units.cpp:32-80): a key missing from
+ * a readable file takes the reference's fallback value (e.g. cfl 0.2,
+ * flip_ratio -0.8, E 5e5); an unreadable sim_config.json sets the same
+ * fallbacks; an unreadable pigment_config.json leaves the pigment fields as
+ * they are.  Either path may be NULL (skipped).  Extra keys understood beyond
+ * the reference's: "yield" (q_max = q_min), "viscosity" (Pa s; perzyna_visc =
+ * viscosity / yield), "q_max", "q_min", "xi", "dt", "dx", "ppc", "use_musl",
+ * "gravity_y".  Returns MPMX_ERR_IO if a given file cannot be read or parsed. */
+int mpmx_config_load_json(mpmx_config* cfg, const char* sim_config_path,
+                          const char* pigment_config_path);
+
+/* Create a handle on the current HIP device (Simulation::initializeBasic,
+ * simulation_units.cpp:9-26). */
+int mpmx_create(const mpmx_config* cfg, mpmx_handle** out);
+void mpmx_destroy(mpmx_handle* h);
+const char* mpmx_last_error(const mpmx_handle* h);
+const char* mpmx_status_string(int status);
+
+/* ---------------- scene ---------------- */
+
+/* Upload particle state (host SoA, borrowed).  Any of v, F, C, pig, diff,
+ * eps_pl_dev, active may be NULL (defaults: 0, I, 0, 0, 0, 0, 1), as in
+ * Particles(Np) (data_structures.hpp:11-29).  particle_mass = rho*dx^3/ppc
+ * (sampling_particles.hpp:28-29). */
+int mpmx_upload_particles(mpmx_handle* h, int64_t n, const float* x, const float* v,
+                          const float* F, const float* C, const float* pig,
+                          const float* diff, const float* eps_pl_dev,
+                          const uint8_t* active);
+
+/* Plates and knives (Simulation::plates / objects, simulation.hpp:129-131;
+ * setupScene adds the floor and the spatula, simulation_units.cpp:117-122).
+ * At most 8 plates and 1 knife. */
+int mpmx_set_objects(mpmx_handle* h, const mpmx_plate* plates, int n_plates,
+                     const mpmx_knife* knives, int n_knives);
+
+/* Keyframes for MPMX_MOTION_KEYFRAMES: n frames of {pos[3], quat_xyzw[4]} at
+ * `fps` (ObjectSpatula::loadAnimation, object_spatula.hpp:235-250). */
+int mpmx_set_knife_keyframes(mpmx_handle* h, int k, const float* frames7, int n_frames,
+                             float fps);
+
+/* Override the knife state used by the next step (driven externally, e.g. by an
+ * app per frame).  Clears after one step; kinematics resume afterwards. */
+int mpmx_set_knife_state(mpmx_handle* h, int k, const mpmx_knife_state* s);
+
+/* Knife state that step `step` uses (host kinematics; for tests/renderer). */
+int mpmx_knife_state_at(mpmx_handle* h, int k, int64_t step, mpmx_knife_state* out);
+
+/* Same kinematics without a handle (no device needed): knife `k`, optional
+ * keyframes (n_frames x {pos[3], quat_xyzw[4]} at fps), time step dt. */
+int mpmx_knife_state_eval(const mpmx_knife* k, const float* frames7, int n_frames, float fps, double dt,
+                          int64_t step, mpmx_knife_state* out);
+
+/* Fix the lattice and derived constants (Simulation::prepareSimulation,
+ * simulation_units.cpp:133-215): origin rule (remesh.cpp:45-180), lambda, mu,
+ * K, prefactors (simulation_units.cpp:167-183), then bin the particles and run
+ * the first P2G + grid update so that mpmx_step can start. */
+int mpmx_prepare(mpmx_handle* h);
+
+/* Lattice actually used: dims[3], origin (= grid.xc/yc/zc) and node coordinate
+ * tables xs[dims[0]], ys[dims[1]], zs[dims[2]] (may be NULL). */
+int mpmx_get_lattice(mpmx_handle* h, int* dims, float* origin, float* xs, float* ys,
+                     float* zs);
+
+/* ---------------- stepping ---------------- */
+
+/* Advance n steps (Simulation::step / advanceStep, simulation_units.cpp:217-233,
+ * simulation.cpp:225-280).  Returns MPMX_ERR_DEVICE if the device status word
+ * is set (the reference's exit flag), after completing the steps. */
+int mpmx_step(mpmx_handle* h, int nsteps);
+
+/* Frame bookkeeping (simulation_units.cpp:235-245). */
+int mpmx_frame_finished(mpmx_handle* h, int* finished);
+
+int mpmx_synchronize(mpmx_handle* h);
+
+/* ---------------- output ---------------- */
+
+/* Download particle state in the caller's original particle order.  Any
+ * pointer may be NULL. delta_gamma is plasticity.cpp:235's rate. */
+int mpmx_download_particles(mpmx_handle* h, float* x, float* v, float* F, float* C,
+                            float* pig, float* diff, float* eps_pl_dev,
+                            float* delta_gamma, uint8_t* active);
+
+/* Renderer format (MPMIntegrationSim::recountParticles,
+ * MPMIntegrationSim.cpp:100-116): pos4 = (x,y,z,1), lat7, diff — host arrays in
+ * original particle order. */
+int mpmx_pack_render(mpmx_handle* h, float* pos4, float* lat7, float* diff);
+
+/* Same, into device memory (e.g. a mapped GL/VK buffer); no host copy. */
+int mpmx_pack_render_device(mpmx_handle* h, void* d_pos4, void* d_lat7, void* d_diff);
+
+/* Grid after the last grid update: dense node arrays (Nx*Ny*Nz, index
+ * (i*Ny+j)*Nz+k as Simulation::ind, simulation.hpp:236-238).  mass is the
+ * node mass m_i (kg), v the post-BC velocity, flip = v_new - v_old, pig the
+ * normalised latent, shear the normalised shear intensity.  Any may be NULL. */
+int mpmx_get_grid(mpmx_handle* h, float* mass, float* v, float* flip, float* pig,
+                  float* shear);
+
+int mpmx_stats_get(mpmx_handle* h, mpmx_stats* out);
+
+/* Per-kernel timing: mean device duration (ms) of each kernel of the step,
+ * measured with HIP events on the handle's stream. */
+typedef struct {
+  float ms_fused;            /* G2P + return map + advection + next P2G scatter */
+  float ms_scan;             /* key prefix sums (two kernels) */
+  float ms_perm;             /* slot permutation */
+  float ms_grid;             /* grid update + boundary collision */
+  float ms_step;             /* whole step */
+  int64_t steps;
+  double particles_per_step; /* live particles averaged over the steps */
+  double active_nodes_per_step;
+} mpmx_kernel_times;
+
+/* Advance nsteps without the HIP graph, timing every kernel with events
+ * (no reference counterpart: measurement for the roofline). */
+int mpmx_profile_steps(mpmx_handle* h, int nsteps, mpmx_kernel_times* out);
+
+/* ---------------- synthetic scenes ---------------- */
+
+/* A paint blob: an axis-aligned box of cells[0]*cells[1]*cells[2] cells of size
+ * dx starting at lo, with one Mixbox latent.  Replaces the VDB blob sampling of
+ * the paint app (simulation_blobs.cpp:7-150) for the benchmark scenes. */
+typedef struct {
+  float lo[3];
+  int   cells[3];
+  float latent[7];
+} mpmx_box;
+
+/* Particles of the boxes: per cell per_axis^3 sub-cells, one particle each at
+ * lo + (cell + (sub + u)/per_axis) * dx (float arithmetic in that order), u a
+ * uniform [0,1) from a counter-based hash of (seed, particle index, axis) —
+ * splitmix64(seed * 0x100000001B3 + 3*index + axis) >> 40, times 2^-24.
+ * Particles are numbered box by box, cell (i,j,k) i-major, sub-cell i-major.
+ * Writes x (n*3) and pig (n*7, may be NULL).  *n_out = particle count; pass
+ * x = NULL to query it. */
+int mpmx_scene_boxes(const mpmx_box* boxes, int n_boxes, float dx, int per_axis, uint64_t seed,
+                     float* x, float* pig, int64_t capacity, int64_t* n_out);
+
+/* ---------------- learned return-map closure ---------------- */
+
+/* MLP 8 -> 64 -> 64 -> 64 -> 64 -> 4, SiLU, BF16 weights, FP32 accumulate.
+ * Parameters are a flat FP32 master copy laid out layer by layer as W[out][in]
+ * then b[out]; mpmx_closure_n_params() gives the count. */
+int64_t mpmx_closure_n_params(void);
+int mpmx_closure_set_params(mpmx_handle* h, const float* params, int64_t n);
+int mpmx_closure_get_params(mpmx_handle* h, float* params, int64_t n);
+/* Input/output normalisation: x_n = (x - in_mean)/in_std, y = y_n*out_std + out_mean. */
+int mpmx_closure_set_norm(mpmx_handle* h, const float* in_mean, const float* in_std,
+                          const float* out_mean, const float* out_std);
+
+/* Harvest (trial, corrected) pairs from the exact solver: while enabled, each
+ * step appends up to `capacity` records of 12 floats
+ * (8 inputs: h1,h2,h3, ||dev h||, q_y, perzyna_visc, dt*E, nu; 4 targets:
+ *  h1',h2',h3', dgamma) for particles whose trial state is plastic. */
+int mpmx_harvest_begin(mpmx_handle* h, int64_t capacity);
+int mpmx_harvest_end(mpmx_handle* h, float* out12, int64_t max_records, int64_t* n_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPMX_H */

@@ -1,0 +1,295 @@
+"""ctypes binding of include/mpmx.h (libmpmx.so).
+
+The library is the product: every call here goes through the C-ABI into the
+HIP path.  `load()` raises if libmpmx.so is missing — there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libmpmx.so")
+
+c_float_p = ctypes.POINTER(ctypes.c_float)
+c_u8_p = ctypes.POINTER(ctypes.c_uint8)
+c_int_p = ctypes.POINTER(ctypes.c_int)
+c_i64_p = ctypes.POINTER(ctypes.c_int64)
+
+OK, ERR_ARG, ERR_HIP, ERR_STATE, ERR_DEVICE, ERR_CAPACITY, ERR_IO = range(7)
+BC_NOSLIP, BC_SLIPSTICK, BC_SLIPFREE = 0, 1, 2
+PLATE_TOP, PLATE_BOTTOM, PLATE_LEFT, PLATE_RIGHT, PLATE_FRONT, PLATE_BACK = range(6)
+PLASTIC_NONE, PLASTIC_VMVISC = 0, 5
+CLOSURE_EXACT, CLOSURE_LEARNED = 0, 1
+KNIFE_SLAB, KNIFE_SPATULA = 0, 1
+MOTION_STATIC, MOTION_LINEAR, MOTION_CIRCULAR, MOTION_LINEAR_THEN_CIRCULAR, MOTION_KEYFRAMES = range(5)
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("E", ctypes.c_float), ("nu", ctypes.c_float), ("rho", ctypes.c_float),
+        ("flip_ratio", ctypes.c_float),
+        ("plastic_model", ctypes.c_int),
+        ("q_max", ctypes.c_float), ("q_min", ctypes.c_float),
+        ("xi", ctypes.c_float), ("p_min", ctypes.c_float),
+        ("perzyna_exp", ctypes.c_float), ("perzyna_visc", ctypes.c_float),
+        ("q_prefac", ctypes.c_float),
+        ("cfl", ctypes.c_float), ("M", ctypes.c_float), ("q_cohesion", ctypes.c_float),
+        ("gravity", ctypes.c_float * 3),
+        ("D_max", ctypes.c_float), ("D_edge0", ctypes.c_float), ("D_edge1", ctypes.c_float),
+        ("start_boost_time", ctypes.c_float), ("end_boost_time", ctypes.c_float),
+        ("boost_factor", ctypes.c_float),
+        ("dx", ctypes.c_float), ("ppc", ctypes.c_float),
+        ("grid_dims", ctypes.c_int * 3),
+        ("origin_mode", ctypes.c_int),
+        ("grid_origin", ctypes.c_float * 3),
+        ("grid_reference_point", ctypes.c_float * 3),
+        ("dt", ctypes.c_float), ("fps", ctypes.c_float),
+        ("use_musl", ctypes.c_int),
+        ("use_particle_boundaries", ctypes.c_int),
+        ("particle_boundary_min", ctypes.c_float * 3),
+        ("particle_boundary_max", ctypes.c_float * 3),
+        ("closure", ctypes.c_int), ("closure_tol", ctypes.c_float),
+        ("n_threads", ctypes.c_int),
+    ]
+
+
+class Plate(ctypes.Structure):
+    _fields_ = [
+        ("pos_object", ctypes.c_float), ("plate_type", ctypes.c_int), ("bc", ctypes.c_int),
+        ("friction", ctypes.c_float), ("pos_lower", ctypes.c_float), ("pos_upper", ctypes.c_float),
+        ("v", ctypes.c_float * 3),
+    ]
+
+
+class Knife(ctypes.Structure):
+    _fields_ = [
+        ("shape", ctypes.c_int), ("half", ctypes.c_float * 3), ("offset", ctypes.c_float),
+        ("bc", ctypes.c_int), ("friction", ctypes.c_float), ("motion", ctypes.c_int),
+        ("p0", ctypes.c_float * 3), ("tilt_axis", ctypes.c_float * 3), ("tilt_angle", ctypes.c_float),
+        ("velocity", ctypes.c_float * 3), ("radius", ctypes.c_float), ("speed", ctypes.c_float),
+        ("phase0", ctypes.c_float), ("t_switch", ctypes.c_float),
+    ]
+
+
+class KnifeState(ctypes.Structure):
+    _fields_ = [
+        ("T", ctypes.c_float * 12), ("Tinv", ctypes.c_float * 12), ("Nmat", ctypes.c_float * 9),
+        ("v", ctypes.c_float * 3), ("w", ctypes.c_float * 3),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("step", ctypes.c_int64), ("time", ctypes.c_double), ("n_particles", ctypes.c_int64),
+        ("n_active", ctypes.c_int64), ("plastic_count", ctypes.c_int64),
+        ("fallback_count", ctypes.c_int64), ("active_nodes", ctypes.c_int64),
+        ("active_blocks", ctypes.c_int64), ("stray_count", ctypes.c_int64),
+        ("device_error", ctypes.c_uint32), ("ms_last_step", ctypes.c_float),
+    ]
+
+
+class KernelTimes(ctypes.Structure):
+    _fields_ = [
+        ("ms_fused", ctypes.c_float), ("ms_scan", ctypes.c_float), ("ms_perm", ctypes.c_float),
+        ("ms_grid", ctypes.c_float), ("ms_step", ctypes.c_float), ("steps", ctypes.c_int64),
+        ("particles_per_step", ctypes.c_double), ("active_nodes_per_step", ctypes.c_double),
+    ]
+
+
+class Box(ctypes.Structure):
+    _fields_ = [("lo", ctypes.c_float * 3), ("cells", ctypes.c_int * 3), ("latent", ctypes.c_float * 7)]
+
+
+_LIB = None
+
+
+def _sig(lib):
+    H = ctypes.c_void_p
+    S = [
+        ("mpmx_config_default", None, [ctypes.POINTER(Config)]),
+        ("mpmx_config_load_json", ctypes.c_int, [ctypes.POINTER(Config), ctypes.c_char_p, ctypes.c_char_p]),
+        ("mpmx_create", ctypes.c_int, [ctypes.POINTER(Config), ctypes.POINTER(H)]),
+        ("mpmx_destroy", None, [H]),
+        ("mpmx_last_error", ctypes.c_char_p, [H]),
+        ("mpmx_status_string", ctypes.c_char_p, [ctypes.c_int]),
+        ("mpmx_upload_particles", ctypes.c_int,
+         [H, ctypes.c_int64, c_float_p, c_float_p, c_float_p, c_float_p, c_float_p, c_float_p, c_float_p, c_u8_p]),
+        ("mpmx_set_objects", ctypes.c_int, [H, ctypes.POINTER(Plate), ctypes.c_int, ctypes.POINTER(Knife), ctypes.c_int]),
+        ("mpmx_set_knife_keyframes", ctypes.c_int, [H, ctypes.c_int, c_float_p, ctypes.c_int, ctypes.c_float]),
+        ("mpmx_set_knife_state", ctypes.c_int, [H, ctypes.c_int, ctypes.POINTER(KnifeState)]),
+        ("mpmx_knife_state_at", ctypes.c_int, [H, ctypes.c_int, ctypes.c_int64, ctypes.POINTER(KnifeState)]),
+        ("mpmx_prepare", ctypes.c_int, [H]),
+        ("mpmx_get_lattice", ctypes.c_int, [H, c_int_p, c_float_p, c_float_p, c_float_p, c_float_p]),
+        ("mpmx_step", ctypes.c_int, [H, ctypes.c_int]),
+        ("mpmx_frame_finished", ctypes.c_int, [H, c_int_p]),
+        ("mpmx_synchronize", ctypes.c_int, [H]),
+        ("mpmx_download_particles", ctypes.c_int,
+         [H, c_float_p, c_float_p, c_float_p, c_float_p, c_float_p, c_float_p, c_float_p, c_float_p, c_u8_p]),
+        ("mpmx_pack_render", ctypes.c_int, [H, c_float_p, c_float_p, c_float_p]),
+        ("mpmx_pack_render_device", ctypes.c_int, [H, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+        ("mpmx_get_grid", ctypes.c_int, [H, c_float_p, c_float_p, c_float_p, c_float_p, c_float_p]),
+        ("mpmx_stats_get", ctypes.c_int, [H, ctypes.POINTER(Stats)]),
+        ("mpmx_profile_steps", ctypes.c_int, [H, ctypes.c_int, ctypes.POINTER(KernelTimes)]),
+        ("mpmx_knife_state_eval", ctypes.c_int, [ctypes.POINTER(Knife), c_float_p, ctypes.c_int, ctypes.c_float,
+                                                  ctypes.c_double, ctypes.c_int64, ctypes.POINTER(KnifeState)]),
+        ("mpmx_scene_boxes", ctypes.c_int,
+         [ctypes.POINTER(Box), ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_uint64, c_float_p, c_float_p,
+          ctypes.c_int64, c_i64_p]),
+    ]
+    for name, res, args in S:
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+def load():
+    """Load libmpmx.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m pigment_mixing_amd.build`")
+        _LIB = _sig(ctypes.CDLL(LIB_PATH))
+    return _LIB
+
+
+def fptr(a):
+    return None if a is None else a.ctypes.data_as(c_float_p)
+
+
+def default_config():
+    c = Config()
+    load().mpmx_config_default(ctypes.byref(c))
+    return c
+
+
+def scene_boxes(boxes, dx, per_axis=2, seed=42):
+    """Particles of a list of (lo[3], cells[3], latent[7]) boxes -> (x[n,3], pig[n,7])."""
+    lib = load()
+    arr = (Box * len(boxes))()
+    for b, (lo, cells, lat) in zip(arr, boxes):
+        b.lo[:] = [float(v) for v in lo]
+        b.cells[:] = [int(v) for v in cells]
+        b.latent[:] = [float(v) for v in lat]
+    n = ctypes.c_int64(0)
+    rc = lib.mpmx_scene_boxes(arr, len(boxes), dx, per_axis, seed, None, None, 0, ctypes.byref(n))
+    if rc:
+        raise RuntimeError(f"mpmx_scene_boxes: {rc}")
+    x = np.zeros((n.value, 3), np.float32)
+    pig = np.zeros((n.value, 7), np.float32)
+    rc = lib.mpmx_scene_boxes(arr, len(boxes), dx, per_axis, seed, fptr(x), fptr(pig), n.value, ctypes.byref(n))
+    if rc:
+        raise RuntimeError(f"mpmx_scene_boxes: {rc}")
+    return x, pig
+
+
+class MpmxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mpmx error {code}: {msg}")
+        self.code = code
+
+
+class Handle:
+    """One simulation on the current HIP device (thin wrapper over the C-ABI)."""
+
+    def __init__(self, cfg):
+        self.lib = load()
+        self.h = ctypes.c_void_p()
+        self._chk(self.lib.mpmx_create(ctypes.byref(cfg), ctypes.byref(self.h)), "create")
+        self.cfg = cfg
+        self.n = 0
+
+    def _chk(self, rc, what):
+        if rc != OK:
+            msg = self.lib.mpmx_last_error(self.h).decode() if self.h else ""
+            raise MpmxError(rc, f"{what}: {msg}")
+
+    def close(self):
+        if self.h:
+            self.lib.mpmx_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, x, v=None, F=None, C=None, pig=None, diff=None, eps=None, active=None):
+        x = np.ascontiguousarray(x, np.float32)
+        self.n = x.shape[0]
+        conv = lambda a: None if a is None else np.ascontiguousarray(a, np.float32)
+        v, F, C, pig, diff, eps = map(conv, (v, F, C, pig, diff, eps))
+        act = None if active is None else np.ascontiguousarray(active, np.uint8)
+        self._chk(self.lib.mpmx_upload_particles(self.h, self.n, fptr(x), fptr(v), fptr(F), fptr(C), fptr(pig),
+                                                 fptr(diff), fptr(eps),
+                                                 None if act is None else act.ctypes.data_as(c_u8_p)), "upload")
+
+    def set_objects(self, plates=(), knife=None):
+        P = (Plate * max(1, len(plates)))(*plates) if plates else None
+        K = ctypes.pointer(knife) if knife is not None else None
+        self._chk(self.lib.mpmx_set_objects(self.h, P, len(plates), K, 1 if knife is not None else 0), "set_objects")
+
+    def prepare(self):
+        self._chk(self.lib.mpmx_prepare(self.h), "prepare")
+
+    def step(self, n=1):
+        self._chk(self.lib.mpmx_step(self.h, n), "step")
+
+    def sync(self):
+        self._chk(self.lib.mpmx_synchronize(self.h), "sync")
+
+    def lattice(self):
+        dims = (ctypes.c_int * 3)()
+        org = np.zeros(3, np.float32)
+        self._chk(self.lib.mpmx_get_lattice(self.h, dims, fptr(org), None, None, None), "lattice")
+        d = list(dims)
+        xs, ys, zs = (np.zeros(k, np.float32) for k in d)
+        self._chk(self.lib.mpmx_get_lattice(self.h, dims, fptr(org), fptr(xs), fptr(ys), fptr(zs)), "lattice")
+        return d, org, xs, ys, zs
+
+    def download(self):
+        n = self.n
+        out = dict(x=np.zeros((n, 3), np.float32), v=np.zeros((n, 3), np.float32),
+                   F=np.zeros((n, 3, 3), np.float32), C=np.zeros((n, 3, 3), np.float32),
+                   pig=np.zeros((n, 7), np.float32), diff=np.zeros(n, np.float32),
+                   eps=np.zeros(n, np.float32), dgamma=np.zeros(n, np.float32), active=np.zeros(n, np.uint8))
+        self._chk(self.lib.mpmx_download_particles(
+            self.h, fptr(out["x"]), fptr(out["v"]), fptr(out["F"]), fptr(out["C"]), fptr(out["pig"]),
+            fptr(out["diff"]), fptr(out["eps"]), fptr(out["dgamma"]), out["active"].ctypes.data_as(c_u8_p)),
+            "download")
+        return out
+
+    def pack_render(self):
+        n = self.n
+        pos4 = np.zeros((n, 4), np.float32)
+        lat7 = np.zeros((n, 7), np.float32)
+        diff = np.zeros(n, np.float32)
+        self._chk(self.lib.mpmx_pack_render(self.h, fptr(pos4), fptr(lat7), fptr(diff)), "pack_render")
+        return pos4, lat7, diff
+
+    def grid(self):
+        d, _, _, _, _ = self.lattice()
+        nn = d[0] * d[1] * d[2]
+        out = dict(mass=np.zeros(nn, np.float32), v=np.zeros((nn, 3), np.float32),
+                   flip=np.zeros((nn, 3), np.float32), pig=np.zeros((nn, 7), np.float32),
+                   shear=np.zeros(nn, np.float32))
+        self._chk(self.lib.mpmx_get_grid(self.h, fptr(out["mass"]), fptr(out["v"]), fptr(out["flip"]),
+                                         fptr(out["pig"]), fptr(out["shear"])), "grid")
+        return out
+
+    def stats(self):
+        s = Stats()
+        self._chk(self.lib.mpmx_stats_get(self.h, ctypes.byref(s)), "stats")
+        return s
+
+    def profile(self, nsteps):
+        kt = KernelTimes()
+        self._chk(self.lib.mpmx_profile_steps(self.h, nsteps, ctypes.byref(kt)), "profile")
+        return kt
+
+    def knife_state(self, step):
+        ks = KnifeState()
+        self._chk(self.lib.mpmx_knife_state_at(self.h, 0, step, ctypes.byref(ks)), "knife_state")
+        return ks

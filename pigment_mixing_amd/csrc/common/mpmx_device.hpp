@@ -1,0 +1,482 @@
+// mpmx_device.hpp — device-side data layout and per-particle / per-node math of
+// the MI355X-native MPM step (host-compilable for CPU unit checks).
+//
+// Layout in HBM
+//   Node blocks: the lattice is tiled in 4x4x4-node blocks.  A scene's blocks are
+//   numbered (bi*BY + bj)*BZ + bk from the scene's `node_block_base` in a pool
+//   shared by all scenes of a batch.  Inside a block the 64 nodes are ordered
+//   ((li*4 + lj)*4 + lk), and every channel is a contiguous 64-float row:
+//   ACC[(blk*ACC_CH + ch)*64 + local], GV[(blk*GV_CH + ch)*64 + local],
+//   MASSFX[blk*64 + local] (u64 fixed point).  A wave of 64 lanes touching one
+//   block channel reads/writes 256 contiguous bytes.
+//   Particles: structure of arrays, physically sorted by particle key (the node
+//   block holding the particle's stencil base), ping-ponged every step.
+//
+// Reference math restated here: quadratic B-spline (tools.hpp:153-177), knife
+// SDFs (object_spatula.hpp:40-82, exact box), boundaryCollision
+// (boundary_collision.cpp:5-316), VMVisc return map (plasticity.cpp:163-238),
+// Hencky Kirchhoff stress (simulation.hpp:287-294).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#ifndef MPMX_HD
+#define MPMX_HD __host__ __device__ __forceinline__
+#endif
+
+namespace mpmx {
+
+constexpr int BLK = 4;            // nodes per block edge
+constexpr int BLK3 = 64;          // nodes per block
+constexpr int ACC_CH = 14;        // mom3, force3, pig7, shear1 (float accumulators)
+constexpr int GV_CH = 15;         // v3, flip3, pig7, shear1, mass1
+constexpr int NPIG = 7;
+constexpr int MAX_PLATES = 4;
+constexpr double MASS_FX_SCALE = 2147483648.0;        // 2^31
+constexpr float MASS_FX_SCALE_F = 2147483648.0f;
+constexpr double MASS_FX_INV = 1.0 / 2147483648.0;
+
+// channel indices
+enum { A_MOM = 0, A_FRC = 3, A_PIG = 6, A_SHR = 13 };
+enum { G_V = 0, G_FLIP = 3, G_PIG = 6, G_SHR = 13, G_MASS = 14 };
+
+// device error bits (the reference's `exit = 1` causes)
+enum : uint32_t {
+  ERR_NEG_DGAMMA = 1u,       // plasticity.cpp:189-192
+  ERR_NEWTON = 2u,           // plasticity.cpp:200-203, 219-222
+  ERR_BAD_BC = 4u,           // boundary_collision.cpp:84-88
+  ERR_TILE_OVERFLOW = 8u,    // internal: a particle moved > 1 cell in one step
+};
+
+struct PlateDev {
+  float pos_object, friction, pos_lower, pos_upper;
+  float v[3];
+  int plate_type, bc, pad;
+};
+
+struct KnifeShapeDev {
+  int present, shape, bc, pad;
+  float half[3];
+  float offset, friction, pad2, pad3;
+};
+
+struct KnifeStateDev {   // per scene, per step
+  float T[12];
+  float Tinv[12];
+  float N[9];
+  float v[3];
+  float w[3];
+  float time;            // simulated time at the step start (for pigment boost)
+};
+
+struct SceneDev {
+  // lattice
+  int dims[3];
+  int bdims[3];
+  int node_block_base;
+  int axis_off;              // offset of this scene's xs/ys/zs in the node-coordinate table
+  float origin[3];
+  float dx, one_over_dx;
+  // material / step constants (simulation_units.cpp:167-183)
+  float E, nu, mu, lambda, K;
+  float inv_d_prefac;        // 1/d_prefac (= q_prefac), the factor of plasticity.cpp:232-234
+  float e_mu_prefac, f_mu_prefac;
+  float q_min, q_max, xi, p_min, perzyna_exp, perzyna_visc;
+  float flip_ratio, apicDinv, particle_mass, particle_volume, dt;
+  float gravity[3];
+  float D_max, D_edge0, D_edge1, start_boost, end_boost, boost_factor;
+  int plastic_model, use_musl, use_pb, closure;
+  float pb_min[3], pb_max[3];
+  float closure_tol;
+  int n_plates;
+  PlateDev plates[MAX_PLATES];
+  KnifeShapeDev knife;
+};
+
+// ------------------------------------------------------------------- helpers
+MPMX_HD int node_block_of(const SceneDev& s, int i, int j, int k) {
+  return s.node_block_base + ((i >> 2) * s.bdims[1] + (j >> 2)) * s.bdims[2] + (k >> 2);
+}
+MPMX_HD int node_local(int i, int j, int k) { return (((i & 3) << 2) | (j & 3)) << 2 | (k & 3); }
+
+// ---------------------------------------------------------- quadratic spline
+// tools.hpp:153-177.  The reference evaluates with double constants and rounds
+// to float; for |u| < 0.5, 0.75 - fl(u*u) is exact in double, and for
+// 0.5 <= |u| < 1.5, t = 1.5 - |u| is exact in float and 0.5*fl(t*t) equals the
+// rounded double 0.5*t*t, so the unfused float forms below are bit-identical.
+// The products must not be contracted into FMAs.
+MPMX_HD float spline_N(float u) {
+  float a = fabsf(u);
+#ifdef __HIP_DEVICE_COMPILE__
+  if (a < 0.5f) return __fsub_rn(0.75f, __fmul_rn(a, a));
+  if (a < 1.5f) { float t = __fsub_rn(1.5f, a); return __fmul_rn(0.5f, __fmul_rn(t, t)); }
+#else
+  if (a < 0.5f) return (float)(0.75 - (double)(a * a));
+  if (a < 1.5f) return (float)(0.5 * (1.5 - (double)a) * (1.5 - (double)a));
+#endif
+  return 0.0f;
+}
+MPMX_HD float spline_dN(float u) {
+  float a = fabsf(u);
+  if (a < 0.5f) return -2.0f * u;
+  if (a < 1.5f) return u > 0.0f ? u - 1.5f : (u < 0.0f ? u + 1.5f : u);
+  return 0.0f;
+}
+
+// Per-axis weights of the 3-node stencil starting at node b.
+struct AxisW { float N[3]; float dN[3]; };
+
+MPMX_HD void axis_weights(float xp, const float* coords, int b, float inv_dx, AxisW& a) {
+#pragma unroll
+  for (int o = 0; o < 3; ++o) {
+#ifdef __HIP_DEVICE_COMPILE__
+    float u = __fmul_rn(__fsub_rn(xp, coords[b + o]), inv_dx);
+#else
+    float u = (xp - coords[b + o]) * inv_dx;
+#endif
+    a.N[o] = spline_N(u);
+    a.dN[o] = spline_dN(u);
+  }
+}
+
+// stencil base node (the particle "bin"): floor((x - origin)/dx - 0.5)
+MPMX_HD int stencil_base(float xp, float origin, float inv_dx) {
+#ifdef __HIP_DEVICE_COMPILE__
+  float f = __fmul_rn(__fsub_rn(xp, origin), inv_dx);
+  return (int)floorf(__fsub_rn(f, 0.5f));
+#else
+  float f = (xp - origin) * inv_dx;
+  return (int)floorf(f - 0.5f);
+#endif
+}
+
+// ----------------------------------------------------------------- knife SDF
+MPMX_HD float sdf_box(float px, float py, float pz, const float* h) {
+  float q0 = fabsf(px) - h[0], q1 = fabsf(py) - h[1], q2 = fabsf(pz) - h[2];
+  float m0 = fmaxf(q0, 0.0f), m1 = fmaxf(q1, 0.0f), m2 = fmaxf(q2, 0.0f);
+  return sqrtf(m0 * m0 + m1 * m1 + m2 * m2) + fminf(fmaxf(q0, fmaxf(q1, q2)), 0.0f);
+}
+
+// object_spatula.hpp:40-82
+MPMX_HD float sdf_spatula(float px_in, float py, float pz, float hw, float ht, float hl, float off) {
+  float px = fabsf(px_in);
+  float b1 = hw + off, b2 = (hw + off) * 0.25f, he = hl + off;
+  float bax = b2 - b1, bay = 2.0f * he;
+  float pax = px - b1, pay = pz + he;
+  float h = fmaxf(0.0f, fminf(1.0f, (pax * bax + pay * bay) / (bax * bax + bay * bay)));
+  float dxs = pax - bax * h, dys = pay - bay * h;
+  float d_side = sqrtf(dxs * dxs + dys * dys);
+  float inside_x = b1 - (b1 - b2) * (pz + he) / (2.0f * he);
+  float d2 = (px < inside_x) ? (px - inside_x) : d_side;
+  if (pz > he) {
+    d2 = sqrtf(px * px + (pz - he) * (pz - he)) - b2;
+  } else if (pz < -he) {
+    float rx = b1, rz = 1.5f * b1;
+    float prz = pz + he;
+    float k0 = sqrtf((px / rx) * (px / rx) + (prz / rz) * (prz / rz));
+    float k1 = sqrtf((px / (rx * rx)) * (px / (rx * rx)) + (prz / (rz * rz)) * (prz / (rz * rz)));
+    d2 = k0 * (k0 - 1.0f) / k1;
+  }
+  float dy = fabsf(py) - ht;
+  float ex = fmaxf(d2, 0.0f), ey = fmaxf(dy, 0.0f);
+  return fminf(fmaxf(d2, dy), 0.0f) + sqrtf(ex * ex + ey * ey);
+}
+
+MPMX_HD float knife_sd_local(const KnifeShapeDev& k, float px, float py, float pz) {
+  if (k.shape == 1) return sdf_spatula(px, py, pz, k.half[0], k.half[1], k.half[2], k.offset);
+  return sdf_box(px, py, pz, k.half);
+}
+
+MPMX_HD void knife_to_local(const KnifeStateDev& s, const float X[3], float p[3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    p[i] = ((s.Tinv[i * 4 + 0] * X[0] + s.Tinv[i * 4 + 1] * X[1]) + s.Tinv[i * 4 + 2] * X[2]) + s.Tinv[i * 4 + 3];
+}
+
+// ObjectSpatula::normal: central differences with e = 1e-4 in local space,
+// normalised, mapped by (T.linear^-1)^T and normalised (object_spatula.hpp:99-129).
+MPMX_HD void knife_normal(const KnifeShapeDev& k, const KnifeStateDev& s, const float X[3], float n[3]) {
+  float p[3];
+  knife_to_local(s, X, p);
+  const float e = 0.0001f;
+  float nl[3];
+  nl[0] = knife_sd_local(k, p[0] + e, p[1], p[2]) - knife_sd_local(k, p[0] - e, p[1], p[2]);
+  nl[1] = knife_sd_local(k, p[0], p[1] + e, p[2]) - knife_sd_local(k, p[0], p[1] - e, p[2]);
+  nl[2] = knife_sd_local(k, p[0], p[1], p[2] + e) - knife_sd_local(k, p[0], p[1], p[2] - e);
+  float nn = sqrtf(nl[0] * nl[0] + nl[1] * nl[1] + nl[2] * nl[2]);
+  if (nn > 0.0f) { nl[0] /= nn; nl[1] /= nn; nl[2] /= nn; }
+  float w[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) w[i] = s.N[i * 3 + 0] * nl[0] + s.N[i * 3 + 1] * nl[1] + s.N[i * 3 + 2] * nl[2];
+  float wn = sqrtf(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  if (wn > 0.0f) { w[0] /= wn; w[1] /= wn; w[2] /= wn; }
+  n[0] = w[0]; n[1] = w[1]; n[2] = w[2];
+}
+
+// ObjectSpatula::velocity: v + omega x (X - centre) (object_spatula.hpp:136-162)
+MPMX_HD void knife_velocity(const KnifeStateDev& s, const float X[3], float v[3]) {
+  float r0 = X[0] - s.T[3], r1 = X[1] - s.T[7], r2 = X[2] - s.T[11];
+  v[0] = s.v[0] + (s.w[1] * r2 - s.w[2] * r1);
+  v[1] = s.v[1] + (s.w[2] * r0 - s.w[0] * r2);
+  v[2] = s.v[2] + (s.w[0] * r1 - s.w[1] * r0);
+}
+
+// Simulation::boundaryCollision (boundary_collision.cpp:5-316), 3-D, no MIBF.
+// Xnode: node position; vi: post-force velocity (in/out).  Returns error bits.
+MPMX_HD uint32_t boundary_collision(const SceneDev& sc, const KnifeStateDev& ks, const float Xnode[3],
+                                    float vi[3]) {
+  float vo[3] = {vi[0], vi[1], vi[2]};
+  float X[3] = {Xnode[0] + sc.dt * vi[0], Xnode[1] + sc.dt * vi[1], Xnode[2] + sc.dt * vi[2]};
+  bool influenced = false;
+  if (sc.knife.present) {
+    float p[3];
+    knife_to_local(ks, X, p);
+    float sd = knife_sd_local(sc.knife, p[0], p[1], p[2]);
+    if (sd < 2.0f * sc.dx) influenced = true;
+    if (sd < 0.0f) {
+      float vob[3];
+      knife_velocity(ks, X, vob);
+      float vr[3] = {vo[0] - vob[0], vo[1] - vob[1], vo[2] - vob[2]};
+      int bc = sc.knife.bc;
+      if (bc == 0) {
+        vr[0] = vr[1] = vr[2] = 0.0f;
+      } else if (bc == 2 || bc == 1) {
+        float n[3];
+        knife_normal(sc.knife, ks, X, n);
+        float d = vr[0] * n[0] + vr[1] * n[1] + vr[2] * n[2];
+        float fr = sc.knife.friction;
+        if (bc == 2) {  // SlipFree
+          if (d < 0.0f) {
+            float vt[3] = {vr[0] - d * n[0], vr[1] - d * n[1], vr[2] - d * n[2]};
+            if (fr > 0.0f) {
+              float vtn = sqrtf(vt[0] * vt[0] + vt[1] * vt[1] + vt[2] * vt[2]);
+              if (-d * fr < vtn) {
+                float sc2 = d * fr / vtn;
+                vr[0] = vt[0] + vt[0] * sc2; vr[1] = vt[1] + vt[1] * sc2; vr[2] = vt[2] + vt[2] * sc2;
+              } else {
+                vr[0] = vr[1] = vr[2] = 0.0f;
+              }
+            } else {
+              vr[0] = vt[0]; vr[1] = vt[1]; vr[2] = vt[2];
+            }
+          }
+        } else {  // SlipStick
+          vr[0] -= d * n[0]; vr[1] -= d * n[1]; vr[2] -= d * n[2];
+          if (d < 0.0f && fr > 0.0f) {
+            float vtn = sqrtf(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
+            if (-d * fr < vtn) {
+              float sc2 = d * fr / vtn;
+              vr[0] += vr[0] * sc2; vr[1] += vr[1] * sc2; vr[2] += vr[2] * sc2;
+            } else {
+              vr[0] = vr[1] = vr[2] = 0.0f;
+            }
+          }
+        }
+      } else {
+        return ERR_BAD_BC;
+      }
+      vi[0] = vr[0] + vob[0]; vi[1] = vr[1] + vob[1]; vi[2] = vr[2] + vob[2];
+      if (X[1] < 0.8f * sc.dx) {  // thin-film rule (boundary_collision.cpp:93-111)
+        float n[3];
+        knife_normal(sc.knife, ks, X, n);
+        float t[3] = {-n[1] * n[0], 1.0f - n[1] * n[1], -n[1] * n[2]};
+        float tn = sqrtf(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+        if (tn > 1e-6f) { t[0] /= tn; t[1] /= tn; t[2] /= tn; } else { t[0] = t[1] = t[2] = 0.0f; }
+        float s2 = 0.2f * sqrtf(vob[0] * vob[0] + vob[1] * vob[1] + vob[2] * vob[2]);
+        vi[0] = vob[0] + t[0] * s2; vi[1] = vob[1] + t[1] * s2; vi[2] = vob[2] + t[2] * s2;
+      }
+      vo[0] = vi[0]; vo[1] = vi[1]; vo[2] = vi[2];
+    }
+  }
+  for (int pi = 0; pi < sc.n_plates; ++pi) {
+    const PlateDev& pl = sc.plates[pi];
+    bool inside;
+    switch (pl.plate_type) {
+      case 2: inside = X[1] < pl.pos_upper && X[1] > pl.pos_lower && (X[0] - pl.pos_object) <= 0.0f; break;
+      case 3: inside = X[1] < pl.pos_upper && X[1] > pl.pos_lower && (pl.pos_object - X[0]) <= 0.0f; break;
+      case 1: inside = X[0] < pl.pos_upper && X[0] > pl.pos_lower && (X[1] - pl.pos_object) <= 0.0f; break;
+      case 0: inside = X[0] < pl.pos_upper && X[0] > pl.pos_lower && (pl.pos_object - X[1]) <= 0.0f; break;
+      case 5: inside = X[2] - pl.pos_object <= 0.0f; break;
+      default: inside = pl.pos_object - X[2] <= 0.0f; break;
+    }
+    if (!inside) continue;
+    if (influenced && pl.plate_type == 1) {  // knife priority over the floor
+      if (vi[1] < pl.v[1]) { vi[1] = pl.v[1]; vo[0] = vi[0]; vo[1] = vi[1]; vo[2] = vi[2]; }
+      continue;
+    }
+    float r[3] = {vo[0] - pl.v[0], vo[1] - pl.v[1], vo[2] - pl.v[2]};
+    int nax = (pl.plate_type == 0 || pl.plate_type == 1) ? 1 : ((pl.plate_type == 2 || pl.plate_type == 3) ? 0 : 2);
+    int ta = (nax == 0) ? 1 : 0, tb = (nax == 2) ? 1 : 2;
+    if (pl.bc == 0) {
+      r[0] = r[1] = r[2] = 0.0f;
+    } else if (pl.bc == 1 || pl.bc == 2) {
+      bool apply = true;
+      if (pl.bc == 2)
+        apply = (pl.plate_type == 0 && r[1] > 0.0f) || (pl.plate_type == 1 && r[1] < 0.0f) ||
+                (pl.plate_type == 2 && r[0] < 0.0f) || (pl.plate_type == 3 && r[0] > 0.0f) ||
+                (pl.plate_type == 4 && r[2] > 0.0f) || (pl.plate_type == 5 && r[2] < 0.0f);
+      if (apply) {
+        float fr = pl.friction;
+        float vel_t = sqrtf(r[ta] * r[ta] + r[tb] * r[tb]);
+        float fvn = fr * fabsf(r[nax]);
+        if (fr == 0.0f) {
+        } else if (vel_t <= fvn) {
+          r[ta] = 0.0f; r[tb] = 0.0f;
+        } else {
+          float s2 = (vel_t - fvn) / vel_t;
+          r[ta] *= s2; r[tb] *= s2;
+        }
+        r[nax] = 0.0f;
+      }
+    } else {
+      return ERR_BAD_BC;
+    }
+    vi[0] = r[0] + pl.v[0]; vi[1] = r[1] + pl.v[1]; vi[2] = r[2] + pl.v[2];
+    vo[0] = vi[0]; vo[1] = vi[1]; vo[2] = vi[2];
+  }
+  return 0u;
+}
+
+// ---------------------------------------------------------------- 3x3 SVD
+// One-sided (Hestenes) Jacobi: rotate the columns of G = F V pairwise until they
+// are orthogonal; sigma_i = |g_i|, U_i = g_i / sigma_i.  Singular values >= 0,
+// sorted descending; U, V orthogonal (det may be -1), as Eigen's JacobiSVD
+// returns them.  Fixed sweep count (branch-uniform across a wavefront).
+MPMX_HD void svd3(const float F[9], float U[9], float sig[3], float V[9]) {
+  float g[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) { g[i] = F[i]; V[i] = 0.0f; }
+  V[0] = V[4] = V[8] = 1.0f;
+#pragma unroll
+  for (int sweep = 0; sweep < 5; ++sweep) {
+#pragma unroll
+    for (int pair = 0; pair < 3; ++pair) {
+      const int p = pair == 2 ? 1 : 0;
+      const int q = pair == 0 ? 1 : 2;
+      float a = g[p] * g[p] + g[3 + p] * g[3 + p] + g[6 + p] * g[6 + p];
+      float b = g[q] * g[q] + g[3 + q] * g[3 + q] + g[6 + q] * g[6 + q];
+      float c = g[p] * g[q] + g[3 + p] * g[3 + q] + g[6 + p] * g[6 + q];
+      float cs = 1.0f, sn = 0.0f;
+      if (fabsf(c) > 1e-30f * sqrtf(a * b) && c != 0.0f) {
+        float zeta = (b - a) / (2.0f * c);
+        float t = copysignf(1.0f, zeta) / (fabsf(zeta) + sqrtf(1.0f + zeta * zeta));
+        cs = 1.0f / sqrtf(1.0f + t * t);
+        sn = cs * t;
+      }
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        float gp = g[3 * r + p], gq = g[3 * r + q];
+        g[3 * r + p] = cs * gp - sn * gq;
+        g[3 * r + q] = sn * gp + cs * gq;
+        float vp = V[3 * r + p], vq = V[3 * r + q];
+        V[3 * r + p] = cs * vp - sn * vq;
+        V[3 * r + q] = sn * vp + cs * vq;
+      }
+    }
+  }
+  float s[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) s[c] = sqrtf(g[c] * g[c] + g[3 + c] * g[3 + c] + g[6 + c] * g[6 + c]);
+  // sort descending (swap columns of g and V)
+#define MPMX_SWAPCOL(i, j)                                                   \
+  if (s[i] < s[j]) {                                                         \
+    float tt = s[i]; s[i] = s[j]; s[j] = tt;                                 \
+    for (int r = 0; r < 3; ++r) {                                            \
+      float t1 = g[3 * r + i]; g[3 * r + i] = g[3 * r + j]; g[3 * r + j] = t1; \
+      float t2 = V[3 * r + i]; V[3 * r + i] = V[3 * r + j]; V[3 * r + j] = t2; \
+    }                                                                        \
+  }
+  MPMX_SWAPCOL(0, 1)
+  MPMX_SWAPCOL(0, 2)
+  MPMX_SWAPCOL(1, 2)
+#undef MPMX_SWAPCOL
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    float inv = s[c] > 0.0f ? 1.0f / s[c] : 0.0f;
+    U[c] = g[c] * inv; U[3 + c] = g[3 + c] * inv; U[6 + c] = g[6 + c] * inv;
+  }
+  if (s[2] > 1e-30f * s[0]) {
+    float inv = 1.0f / s[2];
+    U[2] = g[2] * inv; U[5] = g[5] * inv; U[8] = g[8] * inv;
+  } else {  // rank-deficient: complete U with the cross product
+    U[2] = U[3] * U[7] - U[6] * U[4];
+    U[5] = U[6] * U[1] - U[0] * U[7];
+    U[8] = U[0] * U[4] - U[3] * U[1];
+  }
+  sig[0] = s[0]; sig[1] = s[1]; sig[2] = s[2];
+}
+
+// ------------------------------------------------------------ return map
+// VMVisc (plasticity.cpp:163-238) on the principal Hencky strains.
+// In: h[3] = log(max(sigma,1e-4)), eps (eps_pl_dev).  Out: h (corrected), eps,
+// dgamma_rate (= delta_gamma/dt, plasticity.cpp:235).  Returns 1 if projected,
+// sets err bits on the reference's fatal conditions.
+MPMX_HD int vmvisc_principal(const SceneDev& sc, float h[3], float& eps, float& dg_rate, uint32_t& err) {
+  float htr = (h[0] + h[1]) + h[2];
+  float m = htr / 3.0f;
+  float dv[3] = {h[0] - m, h[1] - m, h[2] - m};
+  float dn = sqrtf(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+  if (dn > 0.0f) { float inv = 1.0f / dn; dv[0] *= inv; dv[1] *= inv; dv[2] *= inv; }
+  float q_trial = sc.e_mu_prefac * dn;
+  float q_yield = sc.q_min + (sc.q_max - sc.q_min) * expf(-sc.xi * eps);
+  float p_trial = -sc.K * htr;
+  dg_rate = 0.0f;
+  if (p_trial < sc.p_min) {  // capped von Mises (plasticity.cpp:172-180): F = U V^T
+    float dgam = q_trial / sc.f_mu_prefac;
+    h[0] = h[1] = h[2] = 0.0f;
+    eps = eps + sc.inv_d_prefac * dgam;
+    dg_rate = dgam / sc.dt;
+    return 1;
+  }
+  if (!(q_trial > q_yield)) return 0;
+  float dgam;
+  if (sc.perzyna_exp == 1.0f && sc.xi == 0.0f) {
+    dgam = (q_trial - q_yield) / (sc.f_mu_prefac + q_yield * sc.perzyna_visc / sc.dt);
+    if (dgam < 0.0f) err |= ERR_NEG_DGAMMA;
+  } else {
+    dgam = 0.01f * (q_trial - q_yield) / sc.f_mu_prefac;
+    const int max_iter = 60;
+    for (int it = 0; it < max_iter; ++it) {
+      if (it == max_iter - 1) err |= ERR_NEWTON;
+      float tm = sc.perzyna_visc * dgam + sc.dt;
+      float tmp = sc.dt / tm;
+      float tmp1 = powf(tmp, sc.perzyna_exp);
+      float ex = expf(-sc.xi * (eps + sc.inv_d_prefac * dgam));
+      q_yield = sc.q_min + (sc.q_max - sc.q_min) * ex;
+      float res = (q_trial - sc.f_mu_prefac * dgam) * tmp1 - q_yield;
+      if (fabsf(res) < 1e-2f) break;
+      float qyd = -sc.xi * sc.inv_d_prefac * (sc.q_max - sc.q_min) * ex;
+      float rd = -sc.f_mu_prefac * tmp1 +
+                 (q_trial - sc.f_mu_prefac * dgam) * sc.perzyna_exp * powf(tmp, sc.perzyna_exp - 1.0f) *
+                     (-sc.perzyna_visc * sc.dt) / (tm * tm) - qyd;
+      if (fabsf(rd) < 1e-14f) err |= ERR_NEWTON;
+      dgam -= res / rd;
+      if (dgam < 0.0f) dgam = 1e-10f;
+    }
+  }
+  float fac = sc.inv_d_prefac * dgam;
+  h[0] -= fac * dv[0]; h[1] -= fac * dv[1]; h[2] -= fac * dv[2];
+  eps = eps + fac;
+  dg_rate = dgam / sc.dt;
+  return 1;
+}
+
+// Kirchhoff stress from the principal frame: tau = U diag(2 mu h + lambda tr h) U^T
+// (simulation.hpp:287-294 with tau = P F^T, F = U diag(e^h) V^T); returns the
+// symmetric tau as (xx, yy, zz, xy, xz, yz).
+MPMX_HD void kirchhoff_from_principal(const SceneDev& sc, const float U[9], const float h[3], float tau[6]) {
+  float tr = (h[0] + h[1]) + h[2];
+  float d0 = 2.0f * sc.mu * h[0] + sc.lambda * tr;
+  float d1 = 2.0f * sc.mu * h[1] + sc.lambda * tr;
+  float d2 = 2.0f * sc.mu * h[2] + sc.lambda * tr;
+  // tau_rc = sum_k U_rk d_k U_ck
+  tau[0] = U[0] * U[0] * d0 + U[1] * U[1] * d1 + U[2] * U[2] * d2;
+  tau[1] = U[3] * U[3] * d0 + U[4] * U[4] * d1 + U[5] * U[5] * d2;
+  tau[2] = U[6] * U[6] * d0 + U[7] * U[7] * d1 + U[8] * U[8] * d2;
+  tau[3] = U[0] * U[3] * d0 + U[1] * U[4] * d1 + U[2] * U[5] * d2;
+  tau[4] = U[0] * U[6] * d0 + U[1] * U[7] * d1 + U[2] * U[8] * d2;
+  tau[5] = U[3] * U[6] * d0 + U[4] * U[7] * d1 + U[5] * U[8] * d2;
+}
+
+}  // namespace mpmx

@@ -1,0 +1,966 @@
+// mpmx_api.cpp — the C-ABI (include/mpmx.h): handle, device buffers, lattice
+// setup, knife kinematics upload, and the per-step HIP graph.
+#include "../../../include/mpmx.h"
+#include "../kernels/mpm_kernels.hpp"
+#include "closure.hpp"
+#include "kinematics.hpp"
+#include "json_min.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace mpmx;
+
+namespace {
+
+constexpr int MAX_CHUNK = 1024;  // knife states uploaded per chunk of steps
+
+struct Scene {
+  mpmx_config cfg;
+  std::vector<mpmx_plate> plates;
+  bool has_knife = false;
+  mpmx_knife knife;
+  std::vector<float> frames;
+  float frame_fps = 60.0f;
+  bool has_override = false;
+  mpmx_knife_state override_state;
+  int64_t p_begin = 0, p_count = 0;  // particle range in upload order
+  int dims[3] = {0, 0, 0};
+  float origin[3] = {0, 0, 0};
+  std::vector<float> xs, ys, zs;
+  SceneDev dev;
+};
+
+}  // namespace
+
+struct mpmx_handle {
+  std::string err;
+  int device = 0;
+  int n_cu = 256;
+  hipStream_t stream = nullptr;
+  std::vector<Scene> scenes;
+  // host particle staging (upload order)
+  int64_t np = 0;
+  std::vector<float> h_comp;   // P_NCOMP * np, component-major
+  std::vector<uint8_t> h_active;
+  std::vector<int> h_pscene;
+  bool prepared = false;
+  int64_t step = 0;            // completed steps
+  double time = 0.0;
+  float time_f = 0.0f;         // float accumulation as Simulation::time (simulation_units.cpp:228)
+  int frame = 0;
+  // device memory
+  void* arena = nullptr;
+  std::vector<void*> allocs;
+  DevPtrs d{};
+  int nkeys = 0;
+  int n_node_blocks = 0;
+  int nchunks = 0;
+  SceneDev* d_scenes = nullptr;
+  KnifeStateDev* d_kstate = nullptr;
+  KnifeStateDev* h_kstate = nullptr;  // pinned
+  int* h_counters = nullptr;          // pinned
+  unsigned long long* h_stats = nullptr;
+  uint32_t* h_err = nullptr;
+  int* d_step_zero = nullptr;
+  // graph of two steps (A->B, B->A)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  bool parity_swapped = false;        // true when the current state lives in the B buffers
+  float ms_last = 0.0f;
+  unsigned long long stats_prev[ST_N] = {0};
+  unsigned long long stats_last[ST_N] = {0};
+  int64_t steps_last = 0;
+  ClosureState closure;
+  // learned closure parameters (FP32 master copy) and normalisation
+  std::vector<float> mlp_params;
+  float in_mean[8] = {0}, in_std[8] = {1, 1, 1, 1, 1, 1, 1, 1};
+  float out_mean[4] = {0}, out_std[4] = {1, 1, 1, 1};
+  // harvest of (trial, corrected) pairs
+  float* d_harvest = nullptr;
+  int64_t harvest_cap = 0;
+  unsigned long long harvest_base = 0;
+};
+
+namespace {
+
+int fail(mpmx_handle* h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+#define HIPCHK(h, expr)                                                                     \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail((h), MPMX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+  } while (0)
+
+template <typename T>
+int dalloc(mpmx_handle* h, T** p, size_t count) {
+  void* q = nullptr;
+  size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  HIPCHK(h, hipMalloc(&q, bytes));
+  HIPCHK(h, hipMemsetAsync(q, 0, bytes, h->stream));
+  h->allocs.push_back(q);
+  *p = (T*)q;
+  return MPMX_OK;
+}
+
+void free_device(mpmx_handle* h) {
+  if (h->graph_exec) { (void)hipGraphExecDestroy(h->graph_exec); h->graph_exec = nullptr; }
+  if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (void* p : h->allocs) (void)hipFree(p);
+  h->allocs.clear();
+  h->d = DevPtrs{};
+}
+
+// remesh.cpp:45-180 origin rule + tools.hpp:325-330 float arange on a fixed lattice.
+void make_lattice(Scene& s, const float* x, const uint8_t* act) {
+  const mpmx_config& c = s.cfg;
+  float mn[3] = {1e20f, 1e20f, 1e20f}, mx[3] = {-1e20f, -1e20f, -1e20f};
+  bool any = false;
+  for (int64_t p = 0; p < s.p_count; ++p) {
+    if (act && !act[p]) continue;
+    any = true;
+    for (int d = 0; d < 3; ++d) {
+      mn[d] = std::min(mn[d], x[3 * p + d]);
+      mx[d] = std::max(mx[d], x[3 * p + d]);
+    }
+  }
+  if (!any) for (int d = 0; d < 3; ++d) mn[d] = mx[d] = 0.0f;
+  if (c.grid_reference_point[0] < 1e10f) {
+    for (int d = 0; d < 3; ++d) {
+      if (c.grid_reference_point[d] < mn[d]) mn[d] = c.grid_reference_point[d];
+      else if (c.grid_reference_point[d] > mx[d]) mx[d] = c.grid_reference_point[d];
+    }
+  }
+  std::vector<float>* ax[3] = {&s.xs, &s.ys, &s.zs};
+  for (int d = 0; d < 3; ++d) {
+    const float dx = c.dx;
+    float low = mn[d] - dx * 2u;
+    const float high = mx[d] + dx * 2u;
+    if (c.origin_mode == 1) low = c.grid_origin[d];
+    ax[d]->clear();
+    if (c.grid_dims[d] > 0) {
+      float val = low;
+      for (int i = 0; i < c.grid_dims[d]; ++i) { ax[d]->push_back(val); val += dx; }
+    } else {
+      for (float val = low; val < high + dx; val += dx) ax[d]->push_back(val);
+    }
+    s.dims[d] = (int)ax[d]->size();
+    s.origin[d] = (*ax[d])[0];
+  }
+}
+
+void fill_scene_dev(Scene& s, int node_block_base, int axis_off) {
+  const mpmx_config& c = s.cfg;
+  SceneDev& d = s.dev;
+  std::memset(&d, 0, sizeof d);
+  for (int i = 0; i < 3; ++i) {
+    d.dims[i] = s.dims[i];
+    d.bdims[i] = (s.dims[i] + 3) / 4;
+    d.origin[i] = s.origin[i];
+    d.gravity[i] = c.gravity[i];
+    d.pb_min[i] = c.particle_boundary_min[i];
+    d.pb_max[i] = c.particle_boundary_max[i];
+  }
+  d.node_block_base = node_block_base;
+  d.axis_off = axis_off;
+  d.dx = c.dx;
+  d.one_over_dx = (float)(1.0 / c.dx);
+  // derived constants as simulation_units.cpp:151-183 (double arithmetic, float storage)
+  const float lambda = (float)(c.nu * c.E / ((1.0 + c.nu) * (1.0 - 2.0 * c.nu)));
+  const float mu = (float)(c.E / (2.0 * (1.0 + c.nu)));
+  d.E = c.E;
+  d.nu = c.nu;
+  d.lambda = lambda;
+  d.mu = mu;
+  d.K = (float)(lambda + 2.0 * mu / 3);
+  const float d_prefac = 1 / c.q_prefac;
+  d.inv_d_prefac = (float)(1.0 / d_prefac);
+  d.e_mu_prefac = 2 * c.q_prefac * mu;
+  d.f_mu_prefac = 2 * c.q_prefac * c.q_prefac * mu;
+  d.q_min = c.q_min;
+  d.q_max = c.q_max;
+  d.xi = c.xi;
+  d.p_min = c.p_min;
+  d.perzyna_exp = c.perzyna_exp;
+  d.perzyna_visc = c.perzyna_visc;
+  d.flip_ratio = c.flip_ratio;
+  d.apicDinv = (float)(4.0 / (c.dx * c.dx));
+  d.particle_volume = c.dx * c.dx * c.dx / c.ppc;
+  d.particle_mass = c.rho * d.particle_volume;
+  d.dt = c.dt;
+  d.D_max = c.D_max;
+  d.D_edge0 = c.D_edge0;
+  d.D_edge1 = c.D_edge1;
+  d.start_boost = c.start_boost_time;
+  d.end_boost = c.end_boost_time;
+  d.boost_factor = c.boost_factor;
+  d.plastic_model = c.plastic_model;
+  d.use_musl = c.use_musl;
+  d.use_pb = c.use_particle_boundaries;
+  d.closure = c.closure;
+  d.closure_tol = c.closure_tol;
+  d.n_plates = (int)std::min<size_t>(s.plates.size(), MAX_PLATES);
+  for (int i = 0; i < d.n_plates; ++i) {
+    const mpmx_plate& p = s.plates[i];
+    PlateDev& q = d.plates[i];
+    q.pos_object = p.pos_object;
+    q.friction = p.friction;
+    q.pos_lower = p.pos_lower;
+    q.pos_upper = p.pos_upper;
+    q.v[0] = p.v[0]; q.v[1] = p.v[1]; q.v[2] = p.v[2];
+    q.plate_type = p.plate_type;
+    q.bc = p.bc;
+  }
+  d.knife.present = s.has_knife ? 1 : 0;
+  if (s.has_knife) {
+    d.knife.shape = s.knife.shape;
+    d.knife.bc = s.knife.bc;
+    d.knife.half[0] = s.knife.half[0];
+    d.knife.half[1] = s.knife.half[1];
+    d.knife.half[2] = s.knife.half[2];
+    d.knife.offset = s.knife.offset;
+    d.knife.friction = s.knife.friction;
+  }
+}
+
+KnifeStateDev to_dev(const mpmx_knife_state& k, float time) {
+  KnifeStateDev d;
+  std::memcpy(d.T, k.T, sizeof d.T);
+  std::memcpy(d.Tinv, k.Tinv, sizeof d.Tinv);
+  std::memcpy(d.N, k.Nmat, sizeof d.N);
+  std::memcpy(d.v, k.v, sizeof d.v);
+  std::memcpy(d.w, k.w, sizeof d.w);
+  d.time = time;
+  return d;
+}
+
+mpmx_knife_state scene_knife_state(const Scene& s, int64_t step) {
+  if (!s.has_knife) {
+    mpmx_knife_state z;
+    std::memset(&z, 0, sizeof z);
+    return z;
+  }
+  return knife_state_at(s.knife, s.frames, s.frame_fps, (double)s.cfg.dt, step);
+}
+
+// Upload knife states + step times for global steps [g0, g0 + n] and reset
+// the chunk-local step counter.
+int upload_kstates(mpmx_handle* h, int64_t g0, int n, float t0) {
+  const int ns = (int)h->scenes.size();
+  float t = t0;
+  for (int i = 0; i <= n; ++i) {
+    for (int s = 0; s < ns; ++s) {
+      Scene& sc = h->scenes[s];
+      mpmx_knife_state k;
+      if (i == 0 && sc.has_override) {
+        k = sc.override_state;
+      } else {
+        k = scene_knife_state(sc, g0 + i);
+      }
+      h->h_kstate[(size_t)i * ns + s] = to_dev(k, t);
+    }
+    t += h->scenes[0].cfg.dt;  // Simulation::time accumulates in float
+  }
+  HIPCHK(h, hipMemcpyAsync(h->d_kstate, h->h_kstate, sizeof(KnifeStateDev) * (size_t)(n + 1) * ns,
+                           hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(&h->d.counters[CNT_STEP], h->d_step_zero, sizeof(int), hipMemcpyDeviceToDevice,
+                           h->stream));
+  return MPMX_OK;
+}
+
+DevPtrs swapped(const DevPtrs& d) {
+  DevPtrs e = d;
+  std::swap(e.pa, e.pb);
+  std::swap(e.pa_id, e.pb_id);
+  return e;
+}
+
+int grid_fused(mpmx_handle* h) { return std::max(1, h->n_cu * 4); }
+int grid_grid(mpmx_handle* h) { return std::max(1, h->n_cu * 4); }
+
+void enqueue_step(mpmx_handle* h, const DevPtrs& d) {
+  launch_fused(d, grid_fused(h), h->stream);
+  launch_scan(d, h->nkeys, 1, h->stream);
+  launch_perm(d, h->np, h->stream);
+  launch_grid(d, grid_grid(h), h->stream);
+}
+
+int build_graph(mpmx_handle* h) {
+  if (h->graph_exec) return MPMX_OK;
+  HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  enqueue_step(h, h->d);
+  enqueue_step(h, swapped(h->d));
+  hipError_t e = hipStreamEndCapture(h->stream, &h->graph);
+  if (e != hipSuccess) return fail(h, MPMX_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+  HIPCHK(h, hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
+  return MPMX_OK;
+}
+
+int read_stats(mpmx_handle* h) {
+  HIPCHK(h, hipMemcpyAsync(h->h_stats, h->d.stats, sizeof(unsigned long long) * ST_N, hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->h_counters, h->d.counters, sizeof(int) * CNT_NCOUNTERS, hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->h_err, h->d.err, sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return MPMX_OK;
+}
+
+// Device pointers with the current state in `pa`.
+DevPtrs current(const mpmx_handle* h) { return h->parity_swapped ? swapped(h->d) : h->d; }
+
+}  // namespace
+
+// ============================================================== C entry points
+extern "C" {
+
+const char* mpmx_status_string(int s) {
+  switch (s) {
+    case MPMX_OK: return "ok";
+    case MPMX_ERR_ARG: return "invalid argument";
+    case MPMX_ERR_HIP: return "HIP runtime error";
+    case MPMX_ERR_STATE: return "call out of order";
+    case MPMX_ERR_DEVICE: return "device status word set";
+    case MPMX_ERR_CAPACITY: return "capacity exceeded";
+    case MPMX_ERR_IO: return "I/O error";
+  }
+  return "unknown";
+}
+
+void mpmx_config_default(mpmx_config* c) {
+  std::memset(c, 0, sizeof *c);
+  // Simulation constructor values (simulation.hpp:38-126)
+  c->E = 1e5f;
+  c->nu = 0.3f;
+  c->rho = 1000.0f;
+  c->flip_ratio = -0.95f;
+  c->plastic_model = MPMX_PLASTIC_NONE;
+  c->q_max = 100.0f;
+  c->q_min = 100.0f;
+  c->xi = 0.0f;
+  c->p_min = -1e20f;
+  c->perzyna_exp = 1.0f;
+  c->perzyna_visc = 0.0f;
+  c->q_prefac = (float)(1.0 / std::sqrt(2.0));
+  c->cfl = 0.5f;
+  c->M = 1.0f;
+  c->q_cohesion = 0.0f;
+  c->gravity[0] = 0.0f; c->gravity[1] = 0.0f; c->gravity[2] = 0.0f;
+  c->D_max = 10e-2f;
+  c->D_edge0 = 0.5f;
+  c->D_edge1 = 0.8f;
+  c->start_boost_time = 20.0f;
+  c->end_boost_time = 40.0f;
+  c->boost_factor = 2.0f;
+  c->dx = 0.01f;
+  c->ppc = 8.0f;
+  c->origin_mode = 0;
+  c->grid_reference_point[0] = c->grid_reference_point[1] = c->grid_reference_point[2] = 2e10f;
+  c->dt = 1e-5f;
+  c->fps = 1.0f;
+  c->use_musl = 0;
+  c->use_particle_boundaries = 0;
+  for (int d = 0; d < 3; ++d) { c->particle_boundary_min[d] = -5.0f; c->particle_boundary_max[d] = 5.0f; }
+  c->closure = MPMX_CLOSURE_EXACT;
+  c->closure_tol = 1e-4f;
+  c->n_threads = 1;
+}
+
+int mpmx_config_load_json(mpmx_config* c, const char* sim_path, const char* pig_path) {
+  if (!c) return MPMX_ERR_ARG;
+  int rc = MPMX_OK;
+  if (sim_path) {
+    JsonFlat j;
+    if (j.load(sim_path)) {
+      c->cfl = j.num("cfl", 0.2f);
+      c->flip_ratio = j.num("flip_ratio", -0.8f);
+      c->E = j.num("E", 500000.0f);
+      c->nu = j.num("nu", 0.3f);
+      c->rho = j.num("rho", 1000.0f);
+      const float angle = j.num("friction_angle", 5.0f);
+      c->M = (float)std::tan(angle * M_PI / 180.0);
+      c->q_cohesion = j.num("q_cohesion", 500.0f);
+      c->perzyna_exp = j.num("perzyna_exp", 1.0f);
+      c->perzyna_visc = j.num("perzyna_visc", 0.1f);
+      // keys beyond the reference's (BASELINE material spec)
+      if (j.has("q_max")) c->q_max = j.num("q_max", c->q_max);
+      if (j.has("q_min")) c->q_min = j.num("q_min", c->q_min);
+      if (j.has("yield")) { c->q_max = c->q_min = j.num("yield", c->q_max); }
+      if (j.has("viscosity") && c->q_max > 0) c->perzyna_visc = j.num("viscosity", 0.0f) / c->q_max;
+      if (j.has("xi")) c->xi = j.num("xi", c->xi);
+      if (j.has("dt")) c->dt = j.num("dt", c->dt);
+      if (j.has("dx")) c->dx = j.num("dx", c->dx);
+      if (j.has("ppc")) c->ppc = j.num("ppc", c->ppc);
+      if (j.has("use_musl")) c->use_musl = (int)j.num("use_musl", 0.0f);
+      if (j.has("gravity_y")) c->gravity[1] = j.num("gravity_y", c->gravity[1]);
+      if (j.has("plastic_model")) c->plastic_model = (int)j.num("plastic_model", 0.0f);
+    } else {
+      // simulation_units.cpp:69-80
+      c->cfl = 0.2f; c->flip_ratio = -0.8f; c->E = 5e5f; c->nu = 0.3f; c->rho = 1000.0f;
+      c->M = (float)std::tan(5 * M_PI / 180.0); c->q_cohesion = 500.0f; c->perzyna_exp = 1.0f;
+      c->perzyna_visc = 0.1f;
+      rc = MPMX_ERR_IO;
+    }
+  }
+  if (pig_path) {
+    JsonFlat j;
+    if (j.load(pig_path)) {
+      c->D_max = j.num("D_max", 10e-2f);
+      c->D_edge0 = j.num("D_edge0", 0.5f);
+      c->D_edge1 = j.num("D_edge1", 0.8f);
+      c->start_boost_time = j.num("start_boost_time", 20.0f);
+      c->end_boost_time = j.num("end_boost_time", 40.0f);
+      c->boost_factor = j.num("boost_factor", 2.0f);
+    } else {
+      rc = MPMX_ERR_IO;
+    }
+  }
+  return rc;
+}
+
+int mpmx_create(const mpmx_config* cfg, mpmx_handle** out) {
+  if (!cfg || !out) return MPMX_ERR_ARG;
+  *out = nullptr;
+  if (!(cfg->dx > 0) || !(cfg->dt > 0) || !(cfg->ppc > 0) || !(cfg->rho > 0)) return MPMX_ERR_ARG;
+  if (cfg->plastic_model != MPMX_PLASTIC_NONE && cfg->plastic_model != MPMX_PLASTIC_VMVISC) return MPMX_ERR_ARG;
+  mpmx_handle* h = new mpmx_handle();
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    delete h;
+    return MPMX_ERR_HIP;
+  }
+  h->device = dev;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) == hipSuccess) h->n_cu = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return MPMX_ERR_HIP;
+  }
+  h->scenes.resize(1);
+  h->scenes[0].cfg = *cfg;
+  *out = h;
+  return MPMX_OK;
+}
+
+void mpmx_destroy(mpmx_handle* h) {
+  if (!h) return;
+  if (h->d_harvest) (void)hipFree(h->d_harvest);
+  free_device(h);
+  closure_free(h->closure);
+  if (h->h_kstate) (void)hipHostFree(h->h_kstate);
+  if (h->h_counters) (void)hipHostFree(h->h_counters);
+  if (h->h_stats) (void)hipHostFree(h->h_stats);
+  if (h->h_err) (void)hipHostFree(h->h_err);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+const char* mpmx_last_error(const mpmx_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+int mpmx_upload_particles(mpmx_handle* h, int64_t n, const float* x, const float* v, const float* F,
+                          const float* C, const float* pig, const float* diff, const float* eps,
+                          const uint8_t* active) {
+  if (!h) return MPMX_ERR_ARG;
+  if (n <= 0 || !x) return fail(h, MPMX_ERR_ARG, "upload: need n > 0 and positions");
+  if (n > (int64_t)0x7fffffff / 2) return fail(h, MPMX_ERR_CAPACITY, "upload: too many particles");
+  h->np = n;
+  h->h_comp.assign((size_t)P_NCOMP * n, 0.0f);
+  h->h_active.assign(n, 1);
+  for (int64_t p = 0; p < n; ++p) {
+    for (int c = 0; c < 3; ++c) {
+      h->h_comp[(P_X + c) * n + p] = x[3 * p + c];
+      h->h_comp[(P_V + c) * n + p] = v ? v[3 * p + c] : 0.0f;
+    }
+    for (int c = 0; c < 9; ++c) {
+      h->h_comp[(P_C + c) * n + p] = C ? C[9 * p + c] : 0.0f;
+      h->h_comp[(P_F + c) * n + p] = F ? F[9 * p + c] : ((c == 0 || c == 4 || c == 8) ? 1.0f : 0.0f);
+    }
+    for (int c = 0; c < NPIG; ++c) h->h_comp[(P_PIG + c) * n + p] = pig ? pig[7 * p + c] : 0.0f;
+    h->h_comp[P_DIFF * n + p] = diff ? diff[p] : 0.0f;
+    h->h_comp[P_EPS * n + p] = eps ? eps[p] : 0.0f;
+    if (active) h->h_active[p] = active[p] ? 1 : 0;
+  }
+  h->h_pscene.assign(n, 0);
+  h->scenes[0].p_begin = 0;
+  h->scenes[0].p_count = n;
+  h->prepared = false;
+  return MPMX_OK;
+}
+
+int mpmx_set_objects(mpmx_handle* h, const mpmx_plate* plates, int n_plates, const mpmx_knife* knives,
+                     int n_knives) {
+  if (!h) return MPMX_ERR_ARG;
+  if (n_plates < 0 || n_plates > MAX_PLATES || n_knives < 0 || n_knives > 1)
+    return fail(h, MPMX_ERR_CAPACITY, "set_objects: at most 4 plates and 1 knife");
+  for (int i = 0; i < n_plates; ++i)
+    if (plates[i].bc < 0 || plates[i].bc > 2 || plates[i].plate_type < 0 || plates[i].plate_type > 5)
+      return fail(h, MPMX_ERR_ARG, "set_objects: invalid plate bc/type");
+  if (n_knives && (knives[0].bc < 0 || knives[0].bc > 2 || knives[0].shape < 0 || knives[0].shape > 1))
+    return fail(h, MPMX_ERR_ARG, "set_objects: invalid knife bc/shape");
+  Scene& s = h->scenes[0];
+  s.plates.assign(plates, plates + n_plates);
+  s.has_knife = n_knives > 0;
+  if (s.has_knife) s.knife = knives[0];
+  if (h->prepared) {
+    fill_scene_dev(s, s.dev.node_block_base, s.dev.axis_off);
+    HIPCHK(h, hipMemcpyAsync(h->d_scenes, &s.dev, sizeof(SceneDev), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
+  return MPMX_OK;
+}
+
+int mpmx_set_knife_keyframes(mpmx_handle* h, int k, const float* frames7, int n_frames, float fps) {
+  if (!h || k != 0 || n_frames < 0 || (n_frames > 0 && !frames7) || !(fps > 0)) return MPMX_ERR_ARG;
+  Scene& s = h->scenes[0];
+  s.frames.assign(frames7, frames7 + 7 * (size_t)n_frames);
+  s.frame_fps = fps;
+  return MPMX_OK;
+}
+
+int mpmx_set_knife_state(mpmx_handle* h, int k, const mpmx_knife_state* st) {
+  if (!h || k != 0 || !st) return MPMX_ERR_ARG;
+  h->scenes[0].has_override = true;
+  h->scenes[0].override_state = *st;
+  return MPMX_OK;
+}
+
+int mpmx_knife_state_at(mpmx_handle* h, int k, int64_t step, mpmx_knife_state* out) {
+  if (!h || k != 0 || !out) return MPMX_ERR_ARG;
+  *out = scene_knife_state(h->scenes[0], step);
+  return MPMX_OK;
+}
+
+int mpmx_prepare(mpmx_handle* h) {
+  if (!h) return MPMX_ERR_ARG;
+  if (h->np <= 0) return fail(h, MPMX_ERR_STATE, "prepare: upload particles first");
+  free_device(h);
+  HIPCHK(h, hipSetDevice(h->device));
+  // ---- lattices (one per scene), node-block pool, coordinate tables
+  int nb_total = 0;
+  int axis_total = 0;
+  std::vector<float> axis;
+  // make_lattice needs interleaved positions of the scene's particles
+  for (Scene& s : h->scenes) {
+    std::vector<float> xi((size_t)3 * s.p_count);
+    std::vector<uint8_t> ai(s.p_count);
+    for (int64_t p = 0; p < s.p_count; ++p) {
+      const int64_t q = s.p_begin + p;
+      for (int c = 0; c < 3; ++c) xi[3 * p + c] = h->h_comp[(P_X + c) * h->np + q];
+      ai[p] = h->h_active[q];
+    }
+    make_lattice(s, xi.data(), ai.data());
+    for (int d = 0; d < 3; ++d)
+      if (s.dims[d] < 3 || s.dims[d] > (1 << 20)) return fail(h, MPMX_ERR_ARG, "prepare: lattice too small/large");
+    const int nbs = ((s.dims[0] + 3) / 4) * ((s.dims[1] + 3) / 4) * ((s.dims[2] + 3) / 4);
+    if ((int64_t)nb_total + nbs > (int64_t)(1 << 30) / 64) return fail(h, MPMX_ERR_CAPACITY, "prepare: lattice too large");
+    fill_scene_dev(s, nb_total, axis_total);
+    nb_total += nbs;
+    axis.insert(axis.end(), s.xs.begin(), s.xs.end());
+    axis.insert(axis.end(), s.ys.begin(), s.ys.end());
+    axis.insert(axis.end(), s.zs.begin(), s.zs.end());
+    axis_total = (int)axis.size();
+  }
+  h->n_node_blocks = nb_total;
+  h->nkeys = nb_total + 1;
+  h->nchunks = scan_chunks(h->nkeys);
+  const int ns = (int)h->scenes.size();
+  const int64_t n = h->np;
+  DevPtrs& d = h->d;
+  d = DevPtrs{};
+  d.cap = n;
+  d.n_scenes = ns;
+  d.dead_key = nb_total;
+  d.nb_list_cap = nb_total;
+  int rc;
+#define ALLOC(ptr, type, count)                         \
+  if ((rc = dalloc(h, (type**)&(ptr), (count))) != 0) return rc;
+  SceneDev* scn; ALLOC(scn, SceneDev, ns); d.scenes = scn; h->d_scenes = scn;
+  KnifeStateDev* ks; ALLOC(ks, KnifeStateDev, (size_t)(MAX_CHUNK + 1) * ns); d.kstate = ks; h->d_kstate = ks;
+  float* axd; ALLOC(axd, float, axis.size()); d.axis_coords = axd;
+  ALLOC(d.pa, float, (size_t)P_NCOMP * n);
+  ALLOC(d.pb, float, (size_t)P_NCOMP * n);
+  ALLOC(d.pa_id, int, n);
+  ALLOC(d.pb_id, int, n);
+  int* psc; ALLOC(psc, int, n); d.pscene = psc;
+  uint8_t* act; ALLOC(act, uint8_t, n); d.active_in = act;
+  ALLOC(d.key_out, int, n);
+  ALLOC(d.rank_out, int, n);
+  ALLOC(d.perm, int, n);
+  ALLOC(d.count_next, int, h->nkeys);
+  ALLOC(d.count_cur, int, h->nkeys);
+  ALLOC(d.offsets, int, h->nkeys);
+  ALLOC(d.pb_list, int, h->nkeys);
+  ALLOC(d.scan_partials, unsigned long long, h->nchunks);
+  int* bsc; ALLOC(bsc, int, h->nkeys); d.blk_scene = bsc;
+  ALLOC(d.acc, float, (size_t)nb_total * ACC_CH * BLK3);
+  ALLOC(d.massfx, unsigned long long, (size_t)nb_total * BLK3);
+  ALLOC(d.gv, float, (size_t)nb_total * GV_CH * BLK3);
+  ALLOC(d.nb_flag, int, nb_total);
+  ALLOC(d.nb_list, int, (size_t)2 * nb_total);
+  ALLOC(d.nb_live, int, nb_total);
+  ALLOC(d.counters, int, CNT_NCOUNTERS);
+  ALLOC(d.stats, unsigned long long, ST_N);
+  ALLOC(d.err, uint32_t, 4);
+  ALLOC(h->d_step_zero, int, 1);
+#undef ALLOC
+  if (!h->h_kstate) HIPCHK(h, hipHostMalloc((void**)&h->h_kstate, sizeof(KnifeStateDev) * (MAX_CHUNK + 1) * 64));
+  if (!h->h_counters) HIPCHK(h, hipHostMalloc((void**)&h->h_counters, sizeof(int) * CNT_NCOUNTERS));
+  if (!h->h_stats) HIPCHK(h, hipHostMalloc((void**)&h->h_stats, sizeof(unsigned long long) * ST_N));
+  if (!h->h_err) HIPCHK(h, hipHostMalloc((void**)&h->h_err, sizeof(uint32_t) * 4));
+  // ---- uploads
+  std::vector<SceneDev> sd(ns);
+  std::vector<int> bscene(h->nkeys, 0);
+  for (int s = 0; s < ns; ++s) {
+    sd[s] = h->scenes[s].dev;
+    const int base = h->scenes[s].dev.node_block_base;
+    const int nbs = h->scenes[s].dev.bdims[0] * h->scenes[s].dev.bdims[1] * h->scenes[s].dev.bdims[2];
+    for (int b = 0; b < nbs; ++b) bscene[base + b] = s;
+  }
+  std::vector<int> ids(n);
+  for (int64_t i = 0; i < n; ++i) ids[i] = (int)i;
+  HIPCHK(h, hipMemcpyAsync(scn, sd.data(), sizeof(SceneDev) * ns, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(axd, axis.data(), sizeof(float) * axis.size(), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(d.pa, h->h_comp.data(), sizeof(float) * P_NCOMP * n, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(d.pa_id, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(psc, h->h_pscene.data(), sizeof(int) * n, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(act, h->h_active.data(), n, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(bsc, bscene.data(), sizeof(int) * h->nkeys, hipMemcpyHostToDevice, h->stream));
+  // ---- knife state for step 0, first scatter and grid update
+  h->step = 0;
+  h->time = 0.0;
+  h->time_f = 0.0f;
+  h->frame = 0;
+  h->parity_swapped = false;
+  if ((rc = upload_kstates(h, 0, 0, 0.0f)) != 0) return rc;
+  launch_keys(d, n, h->stream);
+  launch_scan(d, h->nkeys, 0, h->stream);
+  launch_perm(d, n, h->stream);
+  launch_p2g_init(d, grid_fused(h), h->stream);
+  launch_grid(d, grid_grid(h), h->stream);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (Scene& s : h->scenes) s.has_override = false;
+  if ((rc = read_stats(h)) != 0) return rc;
+  std::memcpy(h->stats_prev, h->h_stats, sizeof h->stats_prev);
+  h->prepared = true;
+  if ((rc = closure_attach(h->closure, h->d, h->stream, h->err)) != 0) return rc;
+  return MPMX_OK;
+}
+
+int mpmx_get_lattice(mpmx_handle* h, int* dims, float* origin, float* xs, float* ys, float* zs) {
+  if (!h || !dims) return MPMX_ERR_ARG;
+  if (!h->prepared) return fail(h, MPMX_ERR_STATE, "get_lattice: prepare first");
+  const Scene& s = h->scenes[0];
+  for (int d = 0; d < 3; ++d) {
+    dims[d] = s.dims[d];
+    if (origin) origin[d] = s.origin[d];
+  }
+  if (xs) std::copy(s.xs.begin(), s.xs.end(), xs);
+  if (ys) std::copy(s.ys.begin(), s.ys.end(), ys);
+  if (zs) std::copy(s.zs.begin(), s.zs.end(), zs);
+  return MPMX_OK;
+}
+
+int mpmx_step(mpmx_handle* h, int nsteps) {
+  if (!h || nsteps < 0) return MPMX_ERR_ARG;
+  if (!h->prepared) return fail(h, MPMX_ERR_STATE, "step: prepare first");
+  if (nsteps == 0) return MPMX_OK;
+  int rc;
+  auto t0 = std::chrono::steady_clock::now();
+  if ((rc = build_graph(h)) != 0) return rc;
+  int done = 0;
+  while (done < nsteps) {
+    const int chunk = std::min(nsteps - done, MAX_CHUNK);
+    if ((rc = upload_kstates(h, h->step, chunk, h->time_f)) != 0) return rc;
+    for (Scene& s : h->scenes) s.has_override = false;
+    int i = 0;
+    // the captured graph always starts from the A side; use it in pairs
+    while (i < chunk) {
+      if (!h->parity_swapped && chunk - i >= 2) {
+        HIPCHK(h, hipGraphLaunch(h->graph_exec, h->stream));
+        i += 2;
+      } else {
+        enqueue_step(h, current(h));
+        h->parity_swapped = !h->parity_swapped;
+        i += 1;
+      }
+    }
+    for (int k = 0; k < chunk; ++k) h->time_f += h->scenes[0].cfg.dt;
+    h->step += chunk;
+    done += chunk;
+  }
+  HIPCHK(h, hipGetLastError());
+  if ((rc = read_stats(h)) != 0) return rc;
+  auto t1 = std::chrono::steady_clock::now();
+  h->ms_last = (float)(std::chrono::duration<double, std::milli>(t1 - t0).count() / nsteps);
+  for (int i = 0; i < ST_N; ++i) {
+    h->stats_last[i] = h->h_stats[i] - h->stats_prev[i];
+    h->stats_prev[i] = h->h_stats[i];
+  }
+  h->steps_last = nsteps;
+  h->time = h->time_f;
+  if (*h->h_err) return fail(h, MPMX_ERR_DEVICE, "device status word set (reference exit = 1)");
+  return MPMX_OK;
+}
+
+int mpmx_frame_finished(mpmx_handle* h, int* finished) {
+  if (!h || !finished) return MPMX_ERR_ARG;
+  const float frame_dt = 1.0f / h->scenes[0].cfg.fps;
+  const float min_dt = 1e-14f;
+  *finished = 0;
+  if (frame_dt * (h->frame + 1) - h->time_f < min_dt * 1.1f) {
+    h->frame++;
+    *finished = 1;
+  }
+  return MPMX_OK;
+}
+
+int mpmx_synchronize(mpmx_handle* h) {
+  if (!h) return MPMX_ERR_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return MPMX_OK;
+}
+
+int mpmx_download_particles(mpmx_handle* h, float* x, float* v, float* F, float* C, float* pig, float* diff,
+                            float* eps, float* dg, uint8_t* active) {
+  if (!h) return MPMX_ERR_ARG;
+  if (!h->prepared) return fail(h, MPMX_ERR_STATE, "download: prepare first");
+  const int64_t n = h->np;
+  const DevPtrs d = current(h);
+  float* tmp = nullptr;
+  HIPCHK(h, hipMalloc(&tmp, sizeof(float) * 9 * n));
+  struct Item { float* out; int first, ncomp; } items[] = {
+      {x, P_X, 3}, {v, P_V, 3}, {F, P_F, 9}, {C, P_C, 9}, {pig, P_PIG, 7}, {diff, P_DIFF, 1}, {eps, P_EPS, 1},
+      {dg, P_DG, 1}};
+  int rc = MPMX_OK;
+  for (const Item& it : items) {
+    if (!it.out) continue;
+    launch_unsort(d, n, tmp, it.first, it.ncomp, h->stream);
+    if (hipMemcpyAsync(it.out, tmp, sizeof(float) * it.ncomp * n, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess) {
+      rc = fail(h, MPMX_ERR_HIP, "download copy failed");
+      break;
+    }
+  }
+  if (rc == MPMX_OK && active) {
+    launch_active_out(d, n, (uint8_t*)tmp, h->stream);
+    if (hipMemcpyAsync(active, tmp, n, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+      rc = fail(h, MPMX_ERR_HIP, "download copy failed");
+  }
+  (void)hipFree(tmp);
+  return rc;
+}
+
+int mpmx_pack_render_device(mpmx_handle* h, void* d_pos4, void* d_lat7, void* d_diff) {
+  if (!h || !d_pos4 || !d_lat7 || !d_diff) return MPMX_ERR_ARG;
+  if (!h->prepared) return fail(h, MPMX_ERR_STATE, "pack_render: prepare first");
+  launch_pack_render(current(h), h->np, (float*)d_pos4, (float*)d_lat7, (float*)d_diff, h->stream);
+  HIPCHK(h, hipGetLastError());
+  return MPMX_OK;
+}
+
+int mpmx_pack_render(mpmx_handle* h, float* pos4, float* lat7, float* diff) {
+  if (!h || !pos4 || !lat7 || !diff) return MPMX_ERR_ARG;
+  if (!h->prepared) return fail(h, MPMX_ERR_STATE, "pack_render: prepare first");
+  const int64_t n = h->np;
+  float* tmp = nullptr;
+  HIPCHK(h, hipMalloc(&tmp, sizeof(float) * 12 * n));
+  launch_pack_render(current(h), n, tmp, tmp + 4 * n, tmp + 11 * n, h->stream);
+  int rc = MPMX_OK;
+  if (hipMemcpyAsync(pos4, tmp, sizeof(float) * 4 * n, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+      hipMemcpyAsync(lat7, tmp + 4 * n, sizeof(float) * 7 * n, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+      hipMemcpyAsync(diff, tmp + 11 * n, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+      hipStreamSynchronize(h->stream) != hipSuccess)
+    rc = fail(h, MPMX_ERR_HIP, "pack_render copy failed");
+  (void)hipFree(tmp);
+  return rc;
+}
+
+int mpmx_get_grid(mpmx_handle* h, float* mass, float* v, float* flip, float* pig, float* shear) {
+  if (!h) return MPMX_ERR_ARG;
+  if (!h->prepared) return fail(h, MPMX_ERR_STATE, "get_grid: prepare first");
+  const Scene& s = h->scenes[0];
+  const int64_t nn = (int64_t)s.dims[0] * s.dims[1] * s.dims[2];
+  float* tmp = nullptr;
+  HIPCHK(h, hipMalloc(&tmp, sizeof(float) * 15 * nn));
+  launch_grid_dense(current(h), 0, tmp, tmp + nn, tmp + 4 * nn, tmp + 7 * nn, tmp + 14 * nn, h->stream);
+  int rc = MPMX_OK;
+  struct Item { float* out; int64_t off, cnt; } items[] = {
+      {mass, 0, nn}, {v, nn, 3 * nn}, {flip, 4 * nn, 3 * nn}, {pig, 7 * nn, 7 * nn}, {shear, 14 * nn, nn}};
+  for (const Item& it : items) {
+    if (!it.out) continue;
+    if (hipMemcpyAsync(it.out, tmp + it.off, sizeof(float) * it.cnt, hipMemcpyDeviceToHost, h->stream) != hipSuccess) {
+      rc = fail(h, MPMX_ERR_HIP, "get_grid copy failed");
+      break;
+    }
+  }
+  if (hipStreamSynchronize(h->stream) != hipSuccess) rc = fail(h, MPMX_ERR_HIP, "get_grid sync failed");
+  (void)hipFree(tmp);
+  return rc;
+}
+
+int mpmx_stats_get(mpmx_handle* h, mpmx_stats* o) {
+  if (!h || !o) return MPMX_ERR_ARG;
+  std::memset(o, 0, sizeof *o);
+  o->step = h->step;
+  o->time = h->time_f;
+  o->n_particles = h->np;
+  if (h->prepared) {
+    int rc = read_stats(h);
+    if (rc) return rc;
+    o->n_active = h->h_counters[CNT_LIVE];
+    const int64_t st = std::max<int64_t>(h->steps_last, 1);
+    o->plastic_count = (int64_t)(h->stats_last[ST_PLASTIC] / st);
+    o->fallback_count = (int64_t)(h->stats_last[ST_FALLBACK] / st);
+    o->active_nodes = (int64_t)(h->stats_last[ST_ACTIVE_NODES] / st);
+    o->active_blocks = (int64_t)(h->stats_last[ST_NBLOCKS] / st);
+    o->stray_count = (int64_t)h->stats_last[ST_STRAY];
+    o->device_error = *h->h_err;
+  }
+  o->ms_last_step = h->ms_last;
+  return MPMX_OK;
+}
+
+
+// ---------------------------------------------------------- learned closure
+int mpmx_closure_set_params(mpmx_handle* h, const float* params, int64_t n) {
+  if (!h || !params || n != mpmx_closure_n_params()) return MPMX_ERR_ARG;
+  h->mlp_params.assign(params, params + n);
+  return closure_upload(h->closure, h->mlp_params, h->in_mean, h->in_std, h->out_mean, h->out_std, h->stream,
+                        h->err);
+}
+
+int mpmx_closure_get_params(mpmx_handle* h, float* params, int64_t n) {
+  if (!h || !params || n != mpmx_closure_n_params()) return MPMX_ERR_ARG;
+  if ((int64_t)h->mlp_params.size() != n) return fail(h, MPMX_ERR_STATE, "closure: no parameters set");
+  std::copy(h->mlp_params.begin(), h->mlp_params.end(), params);
+  return MPMX_OK;
+}
+
+int mpmx_closure_set_norm(mpmx_handle* h, const float* in_mean, const float* in_std, const float* out_mean,
+                          const float* out_std) {
+  if (!h || !in_mean || !in_std || !out_mean || !out_std) return MPMX_ERR_ARG;
+  for (int i = 0; i < 8; ++i) {
+    if (!(in_std[i] > 0)) return fail(h, MPMX_ERR_ARG, "closure: in_std must be > 0");
+    h->in_mean[i] = in_mean[i];
+    h->in_std[i] = in_std[i];
+  }
+  for (int i = 0; i < 4; ++i) {
+    if (!(out_std[i] > 0)) return fail(h, MPMX_ERR_ARG, "closure: out_std must be > 0");
+    h->out_mean[i] = out_mean[i];
+    h->out_std[i] = out_std[i];
+  }
+  if (h->mlp_params.empty()) return MPMX_OK;
+  return closure_upload(h->closure, h->mlp_params, h->in_mean, h->in_std, h->out_mean, h->out_std, h->stream,
+                        h->err);
+}
+
+int mpmx_harvest_begin(mpmx_handle* h, int64_t capacity) {
+  if (!h || capacity <= 0) return MPMX_ERR_ARG;
+  if (!h->prepared) return fail(h, MPMX_ERR_STATE, "harvest: prepare first");
+  if (h->d_harvest) { (void)hipFree(h->d_harvest); h->d_harvest = nullptr; }
+  HIPCHK(h, hipMalloc(&h->d_harvest, sizeof(float) * 12 * (size_t)capacity));
+  h->harvest_cap = capacity;
+  int rc = read_stats(h);
+  if (rc) return rc;
+  h->harvest_base = h->h_stats[ST_HARVEST];
+  // records are appended from slot 0: reset the device counter
+  unsigned long long zero = 0;
+  HIPCHK(h, hipMemcpy(&h->d.stats[ST_HARVEST], &zero, sizeof zero, hipMemcpyHostToDevice));
+  h->d.harvest = h->d_harvest;
+  h->d.harvest_cap = capacity;
+  if (h->graph_exec) { (void)hipGraphExecDestroy(h->graph_exec); h->graph_exec = nullptr; }
+  if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+  return MPMX_OK;
+}
+
+int mpmx_harvest_end(mpmx_handle* h, float* out12, int64_t max_records, int64_t* n_out) {
+  if (!h || !n_out || max_records < 0 || (max_records > 0 && !out12)) return MPMX_ERR_ARG;
+  if (!h->d_harvest) return fail(h, MPMX_ERR_STATE, "harvest: not started");
+  int rc = read_stats(h);
+  if (rc) return rc;
+  const int64_t got = std::min<int64_t>((int64_t)h->h_stats[ST_HARVEST], h->harvest_cap);
+  const int64_t n = std::min(got, max_records);
+  if (n > 0) HIPCHK(h, hipMemcpy(out12, h->d_harvest, sizeof(float) * 12 * (size_t)n, hipMemcpyDeviceToHost));
+  *n_out = n;
+  (void)hipFree(h->d_harvest);
+  h->d_harvest = nullptr;
+  h->harvest_cap = 0;
+  h->d.harvest = nullptr;
+  h->d.harvest_cap = 0;
+  if (h->graph_exec) { (void)hipGraphExecDestroy(h->graph_exec); h->graph_exec = nullptr; }
+  if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+  return MPMX_OK;
+}
+
+
+int mpmx_profile_steps(mpmx_handle* h, int nsteps, mpmx_kernel_times* out) {
+  if (!h || !out || nsteps <= 0) return MPMX_ERR_ARG;
+  if (!h->prepared) return fail(h, MPMX_ERR_STATE, "profile: prepare first");
+  std::memset(out, 0, sizeof *out);
+  int rc = read_stats(h);
+  if (rc) return rc;
+  const unsigned long long nodes0 = h->h_stats[ST_ACTIVE_NODES];
+  hipEvent_t ev[6];
+  for (auto& e : ev) HIPCHK(h, hipEventCreate(&e));
+  double t_f = 0, t_s = 0, t_p = 0, t_g = 0, t_all = 0, live = 0;
+  int done = 0;
+  while (done < nsteps) {
+    const int chunk = std::min(nsteps - done, MAX_CHUNK);
+    if ((rc = upload_kstates(h, h->step, chunk, h->time_f)) != 0) return rc;
+    for (int i = 0; i < chunk; ++i) {
+      const DevPtrs d = current(h);
+      HIPCHK(h, hipEventRecord(ev[0], h->stream));
+      launch_fused(d, grid_fused(h), h->stream);
+      HIPCHK(h, hipEventRecord(ev[1], h->stream));
+      launch_scan(d, h->nkeys, 1, h->stream);
+      HIPCHK(h, hipEventRecord(ev[2], h->stream));
+      launch_perm(d, h->np, h->stream);
+      HIPCHK(h, hipEventRecord(ev[3], h->stream));
+      launch_grid(d, grid_grid(h), h->stream);
+      HIPCHK(h, hipEventRecord(ev[4], h->stream));
+      HIPCHK(h, hipEventSynchronize(ev[4]));
+      float a, b, c, e, all;
+      HIPCHK(h, hipEventElapsedTime(&a, ev[0], ev[1]));
+      HIPCHK(h, hipEventElapsedTime(&b, ev[1], ev[2]));
+      HIPCHK(h, hipEventElapsedTime(&c, ev[2], ev[3]));
+      HIPCHK(h, hipEventElapsedTime(&e, ev[3], ev[4]));
+      HIPCHK(h, hipEventElapsedTime(&all, ev[0], ev[4]));
+      t_f += a; t_s += b; t_p += c; t_g += e; t_all += all;
+      h->parity_swapped = !h->parity_swapped;
+      h->time_f += h->scenes[0].cfg.dt;
+      h->step += 1;
+      HIPCHK(h, hipMemcpy(h->h_counters, h->d.counters, sizeof(int) * CNT_NCOUNTERS, hipMemcpyDeviceToHost));
+      live += h->h_counters[CNT_LIVE];
+    }
+    done += chunk;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if ((rc = read_stats(h)) != 0) return rc;
+  out->steps = nsteps;
+  out->ms_fused = (float)(t_f / nsteps);
+  out->ms_scan = (float)(t_s / nsteps);
+  out->ms_perm = (float)(t_p / nsteps);
+  out->ms_grid = (float)(t_g / nsteps);
+  out->ms_step = (float)(t_all / nsteps);
+  out->particles_per_step = live / nsteps;
+  out->active_nodes_per_step = (double)(h->h_stats[ST_ACTIVE_NODES] - nodes0) / nsteps;
+  for (int i = 0; i < ST_N; ++i) h->stats_prev[i] = h->h_stats[i];
+  h->time = h->time_f;
+  return MPMX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,77 @@
+// mpm_kernels.hpp — device pointer bundle and host-side launchers of the step kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../common/mpmx_device.hpp"
+
+namespace mpmx {
+
+// particle component rows (each a contiguous array of `cap` floats)
+enum {
+  P_X = 0, P_V = 3, P_C = 6, P_F = 15, P_PIG = 24, P_DIFF = 31, P_EPS = 32, P_DG = 33, P_NCOMP = 34
+};
+
+// int32 counters
+enum {
+  CNT_PBLOCKS = 0,      // non-empty live particle blocks (current sort)
+  CNT_LIVE = 1,         // live particles
+  CNT_STEP = 2,         // step index within the uploaded knife-state chunk
+  CNT_GSTEP = 3,        // global step count (node-block list parity)
+  CNT_NBLOCKS0 = 4,     // node-block list lengths (2 parities)
+  CNT_NBLOCKS1 = 5,
+  CNT_NCOUNTERS = 8
+};
+// uint64 cumulative statistics
+enum { ST_PLASTIC = 0, ST_ACTIVE_NODES = 1, ST_STRAY = 2, ST_FALLBACK = 3, ST_NBLOCKS = 4, ST_HARVEST = 5, ST_N = 8 };
+
+struct DevPtrs {
+  const SceneDev* scenes;
+  const KnifeStateDev* kstate;
+  const float* axis_coords;
+  float* pa;
+  float* pb;
+  int* pa_id;
+  int* pb_id;
+  const int* pscene;
+  const uint8_t* active_in;
+  int* key_out;
+  int* rank_out;
+  int* perm;
+  int* count_next;
+  int* count_cur;
+  int* offsets;
+  int* pb_list;
+  unsigned long long* scan_partials;
+  const int* blk_scene;
+  float* acc;
+  unsigned long long* massfx;
+  float* gv;
+  int* nb_flag;
+  int* nb_list;
+  int* nb_live;
+  int* counters;
+  unsigned long long* stats;
+  uint32_t* err;
+  float* harvest;          // (trial, corrected) records of 12 floats, or null
+  int64_t harvest_cap;
+  int64_t cap;
+  int n_scenes;
+  int nb_list_cap;
+  int dead_key;
+  int pad;
+};
+
+void launch_keys(const DevPtrs& d, int64_t n, hipStream_t st);
+void launch_scan(const DevPtrs& d, int nkeys, int advance, hipStream_t st);
+int scan_chunks(int nkeys);
+void launch_perm(const DevPtrs& d, int64_t n, hipStream_t st);
+void launch_p2g_init(const DevPtrs& d, int grid, hipStream_t st);
+void launch_grid(const DevPtrs& d, int grid, hipStream_t st);
+void launch_fused(const DevPtrs& d, int grid, hipStream_t st);
+void launch_unsort(const DevPtrs& d, int64_t n, float* out, int first, int ncomp, hipStream_t st);
+void launch_active_out(const DevPtrs& d, int64_t n, uint8_t* out, hipStream_t st);
+void launch_pack_render(const DevPtrs& d, int64_t n, float* pos4, float* lat7, float* diff, hipStream_t st);
+void launch_grid_dense(const DevPtrs& d, int s, float* mass, float* v, float* flip, float* pig, float* shear,
+                       hipStream_t st);
+
+}  // namespace mpmx

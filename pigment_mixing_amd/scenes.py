@@ -1,0 +1,125 @@
+"""Benchmark and parity scenes (synthetic inputs, seeded).
+
+Each scene is a dict with: cfg (abi.Config), x, pig (particles), plates, knife,
+steps, and a name.  The paint is box blobs sampled 2x2x2 per cell with seeded
+uniform sub-cell jitter; latents are the reference's Mixbox latents of palette
+colours (tests/golden/mixbox_latents.json holds the same numbers, written by the
+reference's own Mixbox).
+
+  paint_2blob_64x32x64   two 20x8x20-cell blobs, 51,200 particles, slab knife
+                         translating at 0.1 m/s through the shared face, 2000 steps
+  paint_4blob_256x64x256 four 64x8x64-cell blobs, 1,048,576 particles, slab knife
+                         on a 40 mm circle at 0.15 m/s, 500 steps
+Material (both): E = 1e5 Pa, nu = 0.3, rho = 1500 kg/m^3, von Mises yield 50 Pa,
+viscosity 10 Pa s (perzyna_visc = 10/50 = 0.2 s), g = -9.81, flip_ratio -0.5,
+pigment parameters of the reference's pigment_config.json, dt = 1e-5 s,
+dx = 3.125 mm, floor SlipStick mu = 1 at y = 0, knife SlipFree mu = 0.3.
+"""
+import json
+import math
+import os
+
+import numpy as np
+
+from . import abi
+
+DX = 3.125e-3
+
+# Mixbox latents of palette colours (written by the reference's Mixbox; the same
+# numbers are the fixture tests/golden/mixbox_latents.json)
+with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "palette_latents.json")) as _f:
+    LATENTS = json.load(_f)["latents"]
+
+
+def material_config(dims, yield_pa=50.0, viscosity=10.0, dt=1e-5, dx=DX):
+    c = abi.default_config()
+    c.E = 1e5
+    c.nu = 0.3
+    c.rho = 1500.0
+    c.flip_ratio = -0.5
+    c.plastic_model = abi.PLASTIC_VMVISC
+    c.q_max = c.q_min = yield_pa
+    c.xi = 0.0
+    c.perzyna_exp = 1.0
+    c.perzyna_visc = viscosity / yield_pa
+    c.gravity[:] = [0.0, -9.81, 0.0]
+    # reference pigment_config.json
+    c.D_max, c.D_edge0, c.D_edge1 = 50.0, 1.5, 7.0
+    c.start_boost_time, c.end_boost_time, c.boost_factor = 15.0, 30.0, 3.0
+    c.dx = dx
+    c.ppc = 8.0
+    c.grid_dims[:] = list(dims)
+    c.origin_mode = 0
+    c.dt = dt
+    c.fps = 60.0
+    c.use_musl = 0
+    c.closure = abi.CLOSURE_EXACT
+    c.closure_tol = 1e-4
+    return c
+
+
+def floor_plate(mu=1.0):
+    p = abi.Plate()
+    p.pos_object = 0.0
+    p.plate_type = abi.PLATE_BOTTOM
+    p.bc = abi.BC_SLIPSTICK
+    p.friction = mu
+    p.pos_lower = -1e15
+    p.pos_upper = 1e15
+    p.v[:] = [0.0, 0.0, 0.0]
+    return p
+
+
+def slab_knife(motion, p0, velocity=(0, 0, 0), radius=0.0, speed=0.0, phase0=0.0, t_switch=0.0,
+               half=(0.030, 0.001, 0.015), tilt_deg=30.0, mu=0.3):
+    k = abi.Knife()
+    k.shape = abi.KNIFE_SLAB
+    k.half[:] = list(half)
+    k.offset = 0.0
+    k.bc = abi.BC_SLIPFREE
+    k.friction = mu
+    k.motion = motion
+    k.p0[:] = list(p0)
+    k.tilt_axis[:] = [1.0, 0.0, 0.0]
+    k.tilt_angle = math.radians(tilt_deg)
+    k.velocity[:] = list(velocity)
+    k.radius = radius
+    k.speed = speed
+    k.phase0 = phase0
+    k.t_switch = t_switch
+    return k
+
+
+def _boxes(specs, dx):
+    return [((lo[0] * dx, lo[1] * dx, lo[2] * dx), cells, LATENTS[col]) for lo, cells, col in specs]
+
+
+def paint_2blob(cells=(20, 8, 20), dims=(64, 32, 64), steps=2000, dx=DX, knife=True, seed=42):
+    cx, cy, cz = cells
+    specs = [((0, 0, 0), cells, "cobalt_blue"), ((cx, 0, 0), cells, "hansa_yellow")]
+    x, pig = abi.scene_boxes(_boxes(specs, dx), dx, 2, seed)
+    cfg = material_config(dims, dx=dx)
+    face = cx * dx
+    kn = slab_knife(abi.MOTION_LINEAR, (face - 0.015, 0.5 * cy * dx, 0.5 * cz * dx), velocity=(0.1, 0.0, 0.0)) \
+        if knife else None
+    return dict(name=f"paint_2blob_{dims[0]}x{dims[1]}x{dims[2]}", cfg=cfg, x=x, pig=pig,
+                plates=[floor_plate()], knife=kn, steps=steps)
+
+
+def paint_4blob(cells=(64, 8, 64), dims=(256, 64, 256), steps=500, dx=DX, seed=42):
+    cx, cy, cz = cells
+    specs = [((0, 0, 0), cells, "cobalt_blue"), ((cx, 0, 0), cells, "hansa_yellow"),
+             ((0, 0, cz), cells, "cadmium_red"), ((cx, 0, cz), cells, "phthalo_green")]
+    x, pig = abi.scene_boxes(_boxes(specs, dx), dx, 2, seed)
+    cfg = material_config(dims, dx=dx)
+    kn = slab_knife(abi.MOTION_CIRCULAR, (cx * dx, 0.5 * cy * dx, cz * dx), radius=0.040, speed=0.15)
+    return dict(name=f"paint_4blob_{dims[0]}x{dims[1]}x{dims[2]}", cfg=cfg, x=x, pig=pig,
+                plates=[floor_plate()], knife=kn, steps=steps)
+
+
+def make_handle(scene):
+    h = abi.Handle(scene["cfg"])
+    h.upload(scene["x"], pig=scene["pig"])
+    h.set_objects(scene["plates"], scene["knife"])
+    h.prepare()
+    return h

@@ -1,0 +1,147 @@
+"""Host-side checks of libmpmx.so that need no GPU: the library loads and
+exports every entry point include/mpmx.h declares, the seeded scene generator
+matches an independent numpy statement of its documented hash, the config
+loader reads the reference's JSON files, the knife kinematics agree with the
+oracle's, and the palette latents equal the reference-Mixbox fixture."""
+import ctypes
+import json
+import math
+import os
+import re
+
+import numpy as np
+
+from pigment_mixing_amd import abi, scenes
+from tests import oracle_py as orc
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for fn in os.listdir(os.path.join(REPO, "include")):
+        if not fn.endswith(".h"):
+            continue
+        txt = open(os.path.join(REPO, "include", fn)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b(mpmx_\w+)\s*\(", txt, flags=re.M):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(abi.LIB_PATH)
+    names = declared_functions()
+    assert len(names) > 25
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, f"declared in include/*.h but not exported: {missing}"
+
+
+def _splitmix64(z):
+    z = (z + np.uint64(0x9E3779B97F4A7C15))
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def test_scene_generator_matches_numpy_statement():
+    dx = np.float32(3.125e-3)
+    boxes = [((0.0, 0.0, 0.0), (3, 2, 4), scenes.LATENTS["cobalt_blue"]),
+             ((3 * 3.125e-3, 0.0, 0.0), (2, 2, 2), scenes.LATENTS["hansa_yellow"])]
+    x, pig = abi.scene_boxes(boxes, float(dx), 2, 42)
+    ref = []
+    pid = 0
+    with np.errstate(over="ignore"):
+        for lo, cells, lat in boxes:
+            lo = np.array(lo, np.float32)
+            for ci in range(cells[0]):
+                for cj in range(cells[1]):
+                    for ck in range(cells[2]):
+                        for si in range(2):
+                            for sj in range(2):
+                                for sk in range(2):
+                                    p = []
+                                    for a, (c, s) in enumerate(zip((ci, cj, ck), (si, sj, sk))):
+                                        h = _splitmix64(np.uint64(42) * np.uint64(0x100000001B3) +
+                                                        np.uint64(3 * pid + a))
+                                        u = np.float32(int(h >> np.uint64(40))) * np.float32(1.0 / 16777216.0)
+                                        p.append(lo[a] + (np.float32(c) + (np.float32(s) + u) / np.float32(2)) * dx)
+                                    ref.append(p)
+                                    pid += 1
+    np.testing.assert_array_equal(x, np.array(ref, np.float32))
+    assert x.shape[0] == (24 + 8) * 8
+    np.testing.assert_array_equal(pig[0], np.float32(scenes.LATENTS["cobalt_blue"]))
+
+
+def test_benchmark_scene_sizes():
+    s0 = scenes.paint_2blob()
+    assert s0["x"].shape[0] == 51200
+    # jitter stays inside each blob's cells; blobs rest on y >= 0
+    assert s0["x"][:, 1].min() >= 0 and s0["x"][:, 1].max() < 8 * scenes.DX
+
+
+def test_config_json_matches_reference_loader():
+    c = abi.default_config()
+    lib = abi.load()
+    g = os.path.join(REPO, "tests", "golden")
+    rc = lib.mpmx_config_load_json(ctypes.byref(c), os.path.join(g, "sim_config.json").encode(),
+                                   os.path.join(g, "pigment_config.json").encode())
+    assert rc == 0
+    sim = json.load(open(os.path.join(g, "sim_config.json")))
+    pig = json.load(open(os.path.join(g, "pigment_config.json")))
+    assert math.isclose(c.cfl, sim["cfl"], rel_tol=1e-6)
+    assert math.isclose(c.flip_ratio, sim["flip_ratio"], rel_tol=1e-6)
+    assert math.isclose(c.E, sim["E"], rel_tol=1e-6)
+    assert math.isclose(c.nu, sim["nu"], rel_tol=1e-6)
+    assert math.isclose(c.M, math.tan(math.radians(sim["friction_angle"])), rel_tol=1e-6)
+    assert math.isclose(c.perzyna_visc, sim["perzyna_visc"], rel_tol=1e-6)
+    for k in ("D_max", "D_edge0", "D_edge1", "boost_factor"):
+        assert math.isclose(getattr(c, k), pig[k], rel_tol=1e-6)
+    assert math.isclose(c.start_boost_time, pig["start_boost_time"])
+    # missing file -> the reference's fallbacks (simulation_units.cpp:69-80)
+    c2 = abi.default_config()
+    rc = lib.mpmx_config_load_json(ctypes.byref(c2), b"/nonexistent.json", None)
+    assert rc == abi.ERR_IO
+    assert all(math.isclose(a, b, rel_tol=1e-6) for a, b in ((c2.E, 5e5), (c2.flip_ratio, -0.8), (c2.perzyna_visc, 0.1)))
+
+
+def test_palette_latents_equal_reference_mixbox_fixture():
+    fx = json.load(open(os.path.join(REPO, "tests", "golden", "mixbox_latents.json")))
+    for name, lat in scenes.LATENTS.items():
+        np.testing.assert_array_equal(np.float32(lat), np.float32(fx["colors"][name]["latent"]))
+        assert abs(sum(lat[:4]) - 1) < 1e-6        # opaque pigment concentrations sum to 1
+    # the reference Mixbox decodes a 50/50 blue+yellow latent mix to green
+    r, g, b = fx["mix_blue_yellow_50_50_srgb"]
+    assert g > r and g > b
+
+
+def test_knife_kinematics_library_matches_oracle():
+    """The library's host kinematics (used on the GPU path) against the oracle's own."""
+    lib = abi.load()
+    for k in (scenes.slab_knife(abi.MOTION_LINEAR, (0.1, 0.0125, 0.03), velocity=(0.1, 0.0, 0.0)),
+              scenes.slab_knife(abi.MOTION_CIRCULAR, (0.2, 0.0125, 0.2), radius=0.04, speed=0.15),
+              scenes.slab_knife(abi.MOTION_LINEAR_THEN_CIRCULAR, (0.1, 0.0125, 0.1), velocity=(0.2, 0, 0),
+                                radius=0.03, speed=0.2, t_switch=1e-3)):
+        cfg = scenes.material_config((16, 16, 16))
+        h = ctypes.c_void_p()
+        # host-only handle creation needs a device; compare the pure kinematics via two
+        # steps of the oracle formula instead when no device is present
+        for step in (0, 1, 99, 150):
+            o = orc.knife_state(k, 1e-5, step)
+            ks = abi.KnifeState()
+            got = _lib_knife_state(k, 1e-5, step)
+            np.testing.assert_allclose(got["T"], o["T"], atol=1e-6)
+            np.testing.assert_allclose(got["Tinv"], o["Tinv"], atol=1e-6)
+            np.testing.assert_allclose(got["v"], o["v"], atol=1e-4)
+            np.testing.assert_allclose(got["w"], o["w"], atol=1e-3)
+
+
+def _lib_knife_state(k, dt, step):
+    lib = abi.load()
+    f = lib.mpmx_knife_state_eval
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.POINTER(abi.Knife), abi.c_float_p, ctypes.c_int, ctypes.c_float, ctypes.c_double,
+                  ctypes.c_int64, ctypes.POINTER(abi.KnifeState)]
+    ks = abi.KnifeState()
+    assert f(ctypes.byref(k), None, 0, 60.0, dt, step, ctypes.byref(ks)) == 0
+    return dict(T=np.array(ks.T), Tinv=np.array(ks.Tinv), v=np.array(ks.v), w=np.array(ks.w))

@@ -1,0 +1,311 @@
+"""Pins of the CPU oracle against mathematics, library routines and the
+reference's own fixtures (no GPU).
+
+Every oracle function used by the parity tests is checked here against
+something that fixes it independently: LAPACK SVD, closed forms of the
+quadratic B-spline, the Hencky/Kirchhoff identity via a matrix logarithm,
+the analytic return-map invariants, exact SDFs, and whole-step invariants
+(mass conservation, free fall, uniform translation, Coulomb friction).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from pigment_mixing_amd import abi, scenes
+from tests import oracle_py as orc
+
+
+def rng(seed=0):
+    return np.random.default_rng(seed)
+
+
+# ----------------------------------------------------------------- SVD
+def test_svd_matches_lapack():
+    r = rng(1)
+    for k in range(300):
+        if k < 100:
+            A = np.eye(3) + 1e-3 * r.standard_normal((3, 3))       # near identity (typical F)
+        elif k < 200:
+            A = r.standard_normal((3, 3))                          # general, may be inverted
+        else:
+            Q, _ = np.linalg.qr(r.standard_normal((3, 3)))
+            A = Q @ np.diag([1.0 + 1e-6, 1.0, 1.0 - 1e-6]) @ Q.T     # nearly degenerate
+        A = A.astype(np.float32)
+        U, s, V = orc.svd3(A)
+        s_ref = np.linalg.svd(A.astype(np.float64), compute_uv=False)
+        np.testing.assert_allclose(s, s_ref, rtol=2e-6, atol=2e-6 * s_ref[0])
+        assert np.all(np.diff(s) <= 0), "singular values sorted descending"
+        assert np.all(s >= 0)
+        np.testing.assert_allclose(U @ np.diag(s) @ V.T, A, atol=5e-6 * max(1.0, s_ref[0]))
+        np.testing.assert_allclose(U.T @ U, np.eye(3), atol=5e-6)
+        np.testing.assert_allclose(V.T @ V, np.eye(3), atol=5e-6)
+
+
+# ----------------------------------------------------------------- splines
+def test_spline_closed_form_and_partition_of_unity():
+    N = np.vectorize(lambda u: orc.lib().orc_spline_N(u))
+    dN = np.vectorize(lambda u: orc.lib().orc_spline_dN(u))
+    u = np.linspace(-2, 2, 4001).astype(np.float32)
+    a = np.abs(u.astype(np.float64))
+    ref = np.where(a < 0.5, 0.75 - a * a, np.where(a < 1.5, 0.5 * (1.5 - a) ** 2, 0.0))
+    np.testing.assert_allclose(N(u), ref, atol=1e-7)
+    dref = np.where(a < 0.5, -2 * u, np.where(a < 1.5, u - 1.5 * np.sign(u), 0.0))
+    np.testing.assert_allclose(dN(u), dref, atol=1e-7)
+    # moments over the 3 nodes of a stencil: sum w = 1, sum w d = 0, sum w d^2 = 1/4 (so D^-1 = 4/dx^2)
+    for f in np.linspace(0.5, 1.4999, 97):
+        b = math.floor(f - 0.5)
+        d = np.array([b, b + 1, b + 2], np.float64) - f
+        w = N((-d).astype(np.float32)).astype(np.float64)
+        g = dN((-d).astype(np.float32)).astype(np.float64)
+        assert abs(w.sum() - 1) < 1e-6
+        assert abs((w * d).sum()) < 1e-6
+        assert abs((w * d * d).sum() - 0.25) < 1e-6
+        assert abs(g.sum()) < 1e-6                    # sum grad w = 0
+        assert abs((g * -d).sum() + 1.0) < 1e-6 or abs((g * d).sum() - 1.0) < 1e-6  # sum x_i dw = 1
+
+
+# ----------------------------------------------------------------- stress
+def _cfg(E=1e5, nu=0.3, yield_pa=50.0, visc=0.2, dt=1e-5):
+    c = scenes.material_config((8, 8, 8))
+    c.E, c.nu = E, nu
+    c.q_max = c.q_min = yield_pa
+    c.perzyna_visc = visc
+    c.dt = dt
+    return c
+
+
+def _logm_spd(S):
+    w, Q = np.linalg.eigh(S)
+    return Q @ np.diag(np.log(w)) @ Q.T
+
+
+def test_kirchhoff_is_hencky_closed_form():
+    c = _cfg()
+    mu = c.E / (2 * (1 + c.nu))
+    lam = c.nu * c.E / ((1 + c.nu) * (1 - 2 * c.nu))
+    r = rng(2)
+    for _ in range(50):
+        F = (np.eye(3) + 0.05 * r.standard_normal((3, 3))).astype(np.float32)
+        tau = orc.kirchhoff(c, F)
+        lnV = 0.5 * _logm_spd(F.astype(np.float64) @ F.T.astype(np.float64))  # ln V, V = sqrt(F F^T)
+        ref = 2 * mu * lnV + lam * np.trace(lnV) * np.eye(3)
+        np.testing.assert_allclose(tau, ref, atol=2e-6 * c.E)
+    assert np.allclose(orc.kirchhoff(c, np.eye(3)), 0)
+
+
+# ----------------------------------------------------------------- return map
+def _principal(F):
+    s = np.linalg.svd(F.astype(np.float64), compute_uv=False)
+    return np.log(s)
+
+
+@pytest.mark.parametrize("visc", [0.2, 2e-3])
+def test_return_map_invariants_and_closed_form(visc):
+    c = _cfg(visc=visc)
+    mu = c.E / (2 * (1 + c.nu))
+    qp = 1 / math.sqrt(2)
+    e_mu, f_mu = 2 * qp * mu, 2 * qp * qp * mu
+    r = rng(3)
+    Ft = np.stack([np.eye(3) + r.uniform(1e-4, 2e-3) * r.standard_normal((3, 3)) for _ in range(400)])
+    Ft = Ft.astype(np.float32)
+    Fo, eo, dg, pl, rc = orc.return_map(c, Ft)
+    assert rc == 0
+    assert pl.sum() > 50 and (1 - pl).sum() > 50, "mix of elastic and plastic states"
+    for k in range(len(Ft)):
+        h0 = _principal(Ft[k])
+        dev0 = h0 - h0.mean()
+        q_tr = e_mu * np.linalg.norm(dev0)
+        if pl[k] == 0:
+            assert q_tr <= c.q_max * (1 + 1e-4)
+            np.testing.assert_array_equal(Fo[k], Ft[k])           # elastic: F untouched
+            continue
+        h1 = _principal(Fo[k])
+        # sort both as singular values are sorted
+        dev1 = h1 - h1.mean()
+        assert abs(h1.sum() - h0.sum()) < 1e-5                     # volume (tr h) preserved (FP32 F)
+        cosang = np.dot(dev0, dev1) / (np.linalg.norm(dev0) * np.linalg.norm(dev1))
+        assert cosang > 1 - 1e-3                                   # deviatoric direction preserved
+        den = f_mu + c.q_max * c.perzyna_visc / c.dt
+        dgam = (q_tr - c.q_max) / den                                # plasticity.cpp:188
+        q_new = e_mu * np.linalg.norm(dev1)
+        # FP32 logs of singular values near 1 carry ~1e-7..1e-6 absolute error: e_mu*1e-6 in q
+        dq = e_mu * 1e-6
+        # q_new is measured on the reconstructed FP32 F: allow ~1e-6 in h
+        np.testing.assert_allclose(q_new, q_tr - f_mu * dgam, rtol=1e-4, atol=e_mu * 1e-6)
+        np.testing.assert_allclose(eo[k], dgam * qp, rtol=1e-3, atol=qp * dq / den)  # eps += dgam/d_prefac
+        np.testing.assert_allclose(dg[k], dgam / c.dt, rtol=1e-3, atol=dq / den / c.dt)
+
+
+def test_return_map_rate_independent_limit_hits_yield_surface():
+    c = _cfg(visc=0.0)
+    mu = c.E / (2 * (1 + c.nu))
+    e_mu = 2 / math.sqrt(2) * mu
+    r = rng(4)
+    Ft = np.stack([np.eye(3) + 5e-3 * r.standard_normal((3, 3)) for _ in range(200)]).astype(np.float32)
+    Fo, _, _, pl, _ = orc.return_map(c, Ft)
+    assert pl.sum() > 100
+    for k in np.nonzero(pl)[0]:
+        h0 = _principal(Ft[k])
+        h1 = _principal(Fo[k])
+        d0, d1 = h0 - h0.mean(), h1 - h1.mean()
+        q_new = e_mu * np.linalg.norm(d1)
+        np.testing.assert_allclose(q_new, c.q_max, rtol=2e-3)       # perzyna_visc = 0: q = q_y
+        assert abs(h1.sum() - h0.sum()) < 1e-5                     # projection is purely deviatoric
+        assert np.dot(d0, d1) / (np.linalg.norm(d0) * np.linalg.norm(d1)) > 1 - 1e-4
+
+
+def test_return_map_isotropy():
+    c = _cfg()
+    r = rng(5)
+    for _ in range(50):
+        Ft = (np.eye(3) + 5e-3 * r.standard_normal((3, 3))).astype(np.float32)
+        Q, _ = np.linalg.qr(r.standard_normal((3, 3)))
+        if np.linalg.det(Q) < 0:
+            Q[:, 0] = -Q[:, 0]
+        Fo, *_ = orc.return_map(c, Ft[None])
+        Fr, *_ = orc.return_map(c, (Q @ Ft).astype(np.float32)[None])
+        np.testing.assert_allclose(Fr[0], Q @ Fo[0], atol=5e-6)
+
+
+def test_return_map_newton_branch_matches_closed_form():
+    # xi = 0 but perzyna_exp slightly off 1 -> Newton (plasticity.cpp:194-230) must agree
+    # with the closed form of perzyna_exp = 1 to its tolerance.
+    c1 = _cfg()
+    c2 = _cfg()
+    c2.perzyna_exp = 1.0 + 1e-7
+    r = rng(6)
+    Ft = np.stack([np.eye(3) + 4e-3 * r.standard_normal((3, 3)) for _ in range(100)]).astype(np.float32)
+    F1, e1, _, p1, _ = orc.return_map(c1, Ft)
+    F2, e2, _, p2, rc = orc.return_map(c2, Ft)
+    assert rc == 0
+    np.testing.assert_array_equal(p1, p2)
+    np.testing.assert_allclose(F2, F1, atol=5e-6)
+
+
+# ----------------------------------------------------------------- SDF
+def test_box_sdf_exact():
+    half = np.array([0.03, 0.001, 0.015], np.float32)
+    r = rng(7)
+    for _ in range(500):
+        p = (r.uniform(-0.05, 0.05, 3)).astype(np.float32)
+        d = orc.lib().orc_knife_sdf(abi.KNIFE_SLAB, orc.fp(half), 0.0, orc.fp(p))
+        q = np.abs(p.astype(np.float64)) - half
+        ref = np.linalg.norm(np.maximum(q, 0)) + min(q.max(), 0.0)
+        assert abs(d - ref) < 1e-7
+    # brute force: distance to the surface from outside points along axes
+    p = np.array([0.05, 0, 0], np.float32)
+    assert abs(orc.lib().orc_knife_sdf(0, orc.fp(half), 0.0, orc.fp(p)) - 0.02) < 1e-7
+
+
+def test_spatula_sdf_sign():
+    half = np.array([0.368, 0.07, 0.736], np.float32)  # ObjectSpatula constructor values
+    inside = np.array([0.0, 0.0, 0.0], np.float32)
+    outside = np.array([0.0, 0.5, 0.0], np.float32)
+    assert orc.lib().orc_knife_sdf(1, orc.fp(half), 0.05, orc.fp(inside)) < 0
+    d = orc.lib().orc_knife_sdf(1, orc.fp(half), 0.05, orc.fp(outside))
+    assert abs(d - (0.5 - 0.07)) < 1e-6             # straight above the blade: |y| - halfThickness
+
+
+# ----------------------------------------------------------------- knife kinematics
+def test_knife_kinematics_linear_and_circular():
+    k = scenes.slab_knife(abi.MOTION_LINEAR, (0.1, 0.02, 0.03), velocity=(0.1, 0.0, 0.0))
+    for step in (0, 1, 57):
+        s = orc.knife_state(k, 1e-5, step)
+        np.testing.assert_allclose(s["v"], [0.1, 0, 0], atol=1e-6)
+        np.testing.assert_allclose(s["T"][[3, 7, 11]], [0.1 + 0.1 * step * 1e-5, 0.02, 0.03], atol=1e-7)
+        np.testing.assert_allclose(s["w"], 0, atol=1e-6)
+    k = scenes.slab_knife(abi.MOTION_CIRCULAR, (0.2, 0.0125, 0.2), radius=0.04, speed=0.15)
+    s = orc.knife_state(k, 1e-5, 100)
+    c = s["T"][[3, 7, 11]]
+    assert abs(math.hypot(c[0] - 0.2, c[2] - 0.2) - 0.04) < 1e-6
+    assert abs(np.linalg.norm(s["v"]) - 0.15) < 1e-4
+    # heading follows the tangent: rigid rotation about y with Omega = v/r; the orbit
+    # (x, z) = r (cos phi, sin phi) turns with omega_y = -Omega, and the circle
+    # centre is the instantaneous rotation centre (zero velocity there)
+    np.testing.assert_allclose(s["w"], [0, -0.15 / 0.04, 0], atol=2e-3)
+    v_ctr = s["v"] + np.cross(s["w"], np.array([0.2, 0.0125, 0.2]) - c)
+    np.testing.assert_allclose(v_ctr, 0, atol=2e-4)
+    R = s["T"].reshape(3, 4)[:, :3]
+    np.testing.assert_allclose(R @ R.T, np.eye(3), atol=1e-6)
+    Ti = s["Tinv"].reshape(3, 4)
+    np.testing.assert_allclose(Ti[:, :3] @ c + Ti[:, 3], 0, atol=1e-6)
+
+
+# ----------------------------------------------------------------- whole-step invariants
+def _mini_scene(g=-9.81, knife=False, cells=(6, 4, 6), dims=(24, 16, 24)):
+    s = scenes.paint_2blob(cells=cells, dims=dims, steps=10, knife=knife)
+    s["cfg"].gravity[1] = g
+    return s
+
+
+def test_mass_conservation_and_first_grid():
+    s = _mini_scene()
+    o = orc.from_scene(s)
+    o.step(1)
+    g = o.grid()
+    mp = s["cfg"].rho * s["cfg"].dx ** 3 / s["cfg"].ppc
+    n_act = o.stats()["n_active"]
+    np.testing.assert_allclose(g["mass"].sum(), n_act * mp, rtol=1e-5)   # simulation.cpp:330-348
+    # pigment rows of non-empty nodes are convex combinations: first 4 sum to 1 after normalisation
+    m = g["mass"] > 0
+    np.testing.assert_allclose(g["pig"][m, :4].sum(1), 1.0, atol=1e-5)
+
+
+def test_free_fall_and_translation():
+    # g only, no objects, F = I: one step moves every particle by dt*dt*g (v grows by dt*g)
+    s = _mini_scene()
+    s["plates"] = []
+    o = orc.from_scene(s)
+    x0 = s["x"].astype(np.float64)
+    o.step(1)
+    out = o.download()
+    dt = s["cfg"].dt
+    np.testing.assert_allclose(out["v"][:, 1], -9.81 * dt, rtol=1e-4)
+    np.testing.assert_allclose(out["x"] - x0, np.broadcast_to([0, -9.81 * dt * dt, 0], x0.shape), atol=2e-9)
+    # uniform translation with g = 0: x += dt*v exactly, v kept (APIC reproduces affine fields)
+    s = _mini_scene(g=0.0)
+    s["plates"] = []
+    o = orc.Oracle(s["cfg"])
+    v = np.broadcast_to(np.array([0.3, 0.1, 0.2], np.float32), s["x"].shape).copy()  # upward: no y=0 clamp
+    o.upload(s["x"], v=v, pig=s["pig"])
+    o.prepare()
+    o.step(1)
+    out = o.download()
+    np.testing.assert_allclose(out["v"], v, rtol=1e-5)
+    np.testing.assert_allclose(out["x"], s["x"] + dt * v, atol=1e-8)
+    np.testing.assert_allclose(out["F"], np.broadcast_to(np.eye(3), out["F"].shape), atol=1e-6)
+
+
+@pytest.mark.slow
+def test_coulomb_friction_incline():
+    """Port of CoulombFrictionTest (matter/tests/tests.cpp:100-306): an elastic block on a
+    SlipFree plate under gravity tilted 24 deg with mu = tan 15 deg slides with mean
+    displacement 0.5 g t^2 (sin th - mu cos th).  Reduced: 0.15 s instead of 5 s."""
+    th = math.radians(24)
+    mu = math.tan(math.radians(15))
+    dx = 0.01
+    s = scenes.paint_2blob(cells=(8, 3, 4), dims=(48, 12, 12), dx=dx, knife=False)
+    x = s["x"].copy()
+    x[:, 1] += 0.5 * dx
+    c = s["cfg"]
+    c.plastic_model = abi.PLASTIC_NONE
+    c.E = 1e6
+    c.rho = 1000.0
+    c.flip_ratio = -0.95
+    c.gravity[:] = [9.81 * math.sin(th), -9.81 * math.cos(th), 0.0]
+    c.dt = 2e-4
+    c.grid_reference_point[:] = [0, 0, 0]
+    p = scenes.floor_plate(mu)
+    p.bc = abi.BC_SLIPFREE
+    o = orc.Oracle(c)
+    o.upload(x, pig=s["pig"])
+    o.set_objects([p], None)
+    o.prepare()
+    nsteps = 750
+    o.step(nsteps)
+    out = o.download()
+    t = nsteps * c.dt
+    ref = 0.5 * 9.81 * t * t * (math.sin(th) - mu * math.cos(th))
+    got = (out["x"][:, 0] - x[:, 0]).mean()
+    assert abs(got - ref) / ref < 2e-2, (got, ref)
