@@ -9,8 +9,9 @@ Perzyna (yield 50 Pa, viscosity 10 Pa s), slab knife on a 40 mm circle at
 
 Prints ONE JSON line.  `value` = live particles x K / wall seconds of K steps
 (graph-launched, inputs resident in HBM, synchronised before and after).
-`roofline` is for the dominant kernel (k_fused), its mean duration measured
-with HIP events on the library's stream over a second run of K steps.
+`roofline` is for the dominant kernel of the step (by time), its mean
+duration measured with HIP events on the library's stream over a second run of
+K steps; every kernel's figure is listed under roofline.kernels.
 `cpu_baseline` times the CPU restatement of the reference step (OpenMP, the
 reference's cost structure) on a bounded sample of the same workload.
 """
@@ -29,14 +30,21 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "particle-steps/s at 1M particles, learned vs plain return map, ≤1e-3 rel err"
 
 
-def fused_bytes(np_live, n_active):
-    # G2P particle read 92 B + particle write 136 B (+ the next step's P2G node
-    # writes 60 B and the G2P node reads 56 B per active node); the P2G particle
-    # read of a two-pass step is not needed (payload stays in registers).
-    return 228.0 * np_live + 116.0 * n_active
+# Algorithmic bytes per launch (FP32; particle rows and active-node rows the
+# kernel must move): G2P reads x v F pig eps (92 B) and writes x v C F pig diff
+# eps dgamma (136 B) per particle and reads v flip pig shear (56 B) per active
+# node; P2G reads x v C F-stress pig diff (128 B) per particle and writes mass,
+# momentum, force, pig, shear (60 B) per active node; the grid update reads 60 B
+# and writes 56 B per active node.
+def kernel_bytes(np_live, n_active):
+    return {
+        "k_g2p": 228.0 * np_live + 56.0 * n_active,
+        "k_p2g": 128.0 * np_live + 60.0 * n_active,
+        "k_grid": 116.0 * n_active,
+    }
 
 
-def step_bytes_two_pass(np_live, n_active):
+def step_bytes(np_live, n_active):
     return 356.0 * np_live + 232.0 * n_active
 
 
@@ -91,11 +99,15 @@ def main():
 
     kt = h.profile(args.steps)
     n_act = kt.active_nodes_per_step
-    bytes_fused = fused_bytes(kt.particles_per_step, n_act)
-    achieved = bytes_fused / (kt.ms_fused * 1e-3) / 1e9
-    step_eff = step_bytes_two_pass(kt.particles_per_step, n_act) / (kt.ms_step * 1e-3) / 1e9
-    print(f"[bench] kernels ms: fused={kt.ms_fused:.4f} scan={kt.ms_scan:.4f} perm={kt.ms_perm:.4f} "
-          f"grid={kt.ms_grid:.4f} step={kt.ms_step:.4f} active_nodes={n_act:.0f}", file=sys.stderr, flush=True)
+    kb = kernel_bytes(kt.particles_per_step, n_act)
+    kms = {"k_g2p": kt.ms_g2p, "k_p2g": kt.ms_p2g, "k_grid": kt.ms_grid}
+    kgbs = {k: kb[k] / (kms[k] * 1e-3) / 1e9 for k in kb}
+    dom = max(kms, key=kms.get)
+    step_gbs = step_bytes(kt.particles_per_step, n_act) / (kt.ms_step * 1e-3) / 1e9
+    print(f"[bench] kernels ms: g2p={kt.ms_g2p:.4f} sort={kt.ms_sort:.4f} p2g={kt.ms_p2g:.4f} "
+          f"grid={kt.ms_grid:.4f} step={kt.ms_step:.4f} active_nodes={n_act:.0f} "
+          f"GB/s g2p={kgbs['k_g2p']:.0f} p2g={kgbs['k_p2g']:.0f} grid={kgbs['k_grid']:.0f}",
+          file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -127,16 +139,19 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_fused (G2P + F update + return map + advection + next-step P2G scatter)",
-            "achieved": achieved,
+            "kernel": dom,
+            "achieved": kgbs[dom],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "frac": kgbs[dom] / HBM_PEAK_GBS,
             "traffic": None,
-            "algorithmic_bytes_per_launch": bytes_fused,
-            "kernel_ms": kt.ms_fused,
+            "algorithmic_bytes_per_launch": kb[dom],
+            "kernel_ms": kms[dom],
+            "kernels": {k: {"ms": kms[k], "GB/s": kgbs[k], "frac": kgbs[k] / HBM_PEAK_GBS,
+                            "algorithmic_bytes": kb[k]} for k in kb},
+            "sort_ms": kt.ms_sort,
             "step_ms_events": kt.ms_step,
-            "two_pass_step_bytes_GBs": step_eff,
+            "step_frac": step_gbs / HBM_PEAK_GBS,
             "active_nodes": n_act,
         },
         "cpu_baseline": cpu,

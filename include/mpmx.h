@@ -164,10 +164,10 @@ typedef struct {
   int64_t n_particles;
   int64_t n_active;          /* active particles                                      */
   int64_t plastic_count;     /* particles projected in the last step                  */
-  int64_t fallback_count;    /* learned closure: particles sent to the exact map      */
-  int64_t active_nodes;      /* nodes with mass > 0 in the last step                  */
-  int64_t active_blocks;     /* 4x4x4 node blocks touched in the last step            */
-  int64_t stray_count;       /* particles that left their bin tile (slow path)        */
+  int64_t fallback_count;    /* learned closure: particles sent to the exact map, last step */
+  int64_t active_nodes;      /* nodes with mass > 0, mean over the last mpmx_step call */
+  int64_t active_blocks;     /* 4x4x4 node blocks touched, mean over the last call    */
+  int64_t stray_count;       /* particles scattered outside their LDS tile, last call */
   uint32_t device_error;     /* bit mask; nonzero <=> reference exit = 1              */
   float   ms_last_step;      /* wall ms of the last mpmx_step call / its steps        */
 } mpmx_stats;
@@ -284,9 +284,9 @@ int mpmx_stats_get(mpmx_handle* h, mpmx_stats* out);
 /* Per-kernel timing: mean device duration (ms) of each kernel of the step,
  * measured with HIP events on the handle's stream. */
 typedef struct {
-  float ms_fused;            /* G2P + return map + advection + next P2G scatter */
-  float ms_scan;             /* key prefix sums (two kernels) */
-  float ms_perm;             /* slot permutation */
+  float ms_g2p;              /* G2P + F update + return map + advection (k_g2p) */
+  float ms_sort;             /* block/cell prefix sums + permutation */
+  float ms_p2g;              /* P2G scatter of mass, momentum, force, latent, shear */
   float ms_grid;             /* grid update + boundary collision */
   float ms_step;             /* whole step */
   int64_t steps;

@@ -91,7 +91,7 @@ class Stats(ctypes.Structure):
 
 class KernelTimes(ctypes.Structure):
     _fields_ = [
-        ("ms_fused", ctypes.c_float), ("ms_scan", ctypes.c_float), ("ms_perm", ctypes.c_float),
+        ("ms_g2p", ctypes.c_float), ("ms_sort", ctypes.c_float), ("ms_p2g", ctypes.c_float),
         ("ms_grid", ctypes.c_float), ("ms_step", ctypes.c_float), ("steps", ctypes.c_int64),
         ("particles_per_step", ctypes.c_double), ("active_nodes_per_step", ctypes.c_double),
     ]

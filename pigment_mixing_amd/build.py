@@ -18,6 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libmpmx.so")
+DEMO = os.path.join(OBJ, "simulation_api_demo")
 
 SOURCES = [
     "kernels/mpm_kernels.hip",
@@ -69,6 +70,14 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # C++ driver of the Simulation facade (used by a GPU test)
+    demo_src = os.path.join(REPO, "tests", "cpp", "simulation_api_demo.cpp")
+    if os.path.exists(demo_src):
+        cmd = ["g++", "-std=c++17", "-O2", f"-I{os.path.join(REPO, 'include')}", demo_src, "-o", DEMO,
+               f"-L{LIBDIR}", "-lmpmx", f"-Wl,-rpath,{LIBDIR}", "-Wl,-rpath,$ORIGIN/../lib"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"demo build failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     if verbose:
         print("built", LIB)
     return LIB

@@ -33,9 +33,18 @@ constexpr int ACC_CH = 14;        // mom3, force3, pig7, shear1 (float accumulat
 constexpr int GV_CH = 15;         // v3, flip3, pig7, shear1, mass1
 constexpr int NPIG = 7;
 constexpr int MAX_PLATES = 4;
-constexpr double MASS_FX_SCALE = 2147483648.0;        // 2^31
-constexpr float MASS_FX_SCALE_F = 2147483648.0f;
-constexpr double MASS_FX_INV = 1.0 / 2147483648.0;
+// Node mass (sum of weights) is accumulated in u64 fixed point, 2^52 per unit
+// weight: integer adds are order-independent, so grid masses are bitwise
+// reproducible whatever the atomic order; a node holds up to 4096 units.
+constexpr double MASS_FX_SCALE = 4503599627370496.0;        // 2^52
+constexpr double MASS_FX_INV = 1.0 / 4503599627370496.0;
+MPMX_HD unsigned long long mass_to_fx(float w) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __double2ull_rn((double)w * MASS_FX_SCALE);
+#else
+  return (unsigned long long)((double)w * MASS_FX_SCALE + 0.5);
+#endif
+}
 
 // channel indices
 enum { A_MOM = 0, A_FRC = 3, A_PIG = 6, A_SHR = 13 };

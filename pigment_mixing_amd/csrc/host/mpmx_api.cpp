@@ -290,10 +290,13 @@ DevPtrs swapped(const DevPtrs& d) {
 int grid_fused(mpmx_handle* h) { return std::max(1, h->n_cu * 4); }
 int grid_grid(mpmx_handle* h) { return std::max(1, h->n_cu * 4); }
 
+// One step: G2P (A -> B, counts for the new sort), sort, P2G from B, grid update.
 void enqueue_step(mpmx_handle* h, const DevPtrs& d) {
-  launch_fused(d, grid_fused(h), h->stream);
+  launch_g2p(d, grid_fused(h), h->stream);
   launch_scan(d, h->nkeys, 1, h->stream);
+  launch_cellscan(d, grid_grid(h), h->stream);
   launch_perm(d, h->np, h->stream);
+  launch_p2g(d, grid_fused(h), 1, h->stream);
   launch_grid(d, grid_grid(h), h->stream);
 }
 
@@ -597,7 +600,11 @@ int mpmx_prepare(mpmx_handle* h) {
   uint8_t* act; ALLOC(act, uint8_t, n); d.active_in = act;
   ALLOC(d.key_out, int, n);
   ALLOC(d.rank_out, int, n);
+  ALLOC(d.cell_out, int, n);
   ALLOC(d.perm, int, n);
+  ALLOC(d.cell_cnt, int, (size_t)h->nkeys * BLK3);
+  ALLOC(d.cell_off, int, (size_t)h->nkeys * BLK3);
+  ALLOC(d.cell_num, int, (size_t)h->nkeys * BLK3);
   ALLOC(d.count_next, int, h->nkeys);
   ALLOC(d.count_cur, int, h->nkeys);
   ALLOC(d.offsets, int, h->nkeys);
@@ -606,7 +613,7 @@ int mpmx_prepare(mpmx_handle* h) {
   int* bsc; ALLOC(bsc, int, h->nkeys); d.blk_scene = bsc;
   ALLOC(d.acc, float, (size_t)nb_total * ACC_CH * BLK3);
   ALLOC(d.massfx, unsigned long long, (size_t)nb_total * BLK3);
-  ALLOC(d.gv, float, (size_t)nb_total * GV_CH * BLK3);
+  ALLOC(d.gv, float, (size_t)nb_total * GV_STRIDE * BLK3);
   ALLOC(d.nb_flag, int, nb_total);
   ALLOC(d.nb_list, int, (size_t)2 * nb_total);
   ALLOC(d.nb_live, int, nb_total);
@@ -644,10 +651,12 @@ int mpmx_prepare(mpmx_handle* h) {
   h->frame = 0;
   h->parity_swapped = false;
   if ((rc = upload_kstates(h, 0, 0, 0.0f)) != 0) return rc;
+  launch_init_tau(d, n, h->stream);
   launch_keys(d, n, h->stream);
   launch_scan(d, h->nkeys, 0, h->stream);
+  launch_cellscan(d, grid_grid(h), h->stream);
   launch_perm(d, n, h->stream);
-  launch_p2g_init(d, grid_fused(h), h->stream);
+  launch_p2g(d, grid_fused(h), 0, h->stream);
   launch_grid(d, grid_grid(h), h->stream);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -823,8 +832,8 @@ int mpmx_stats_get(mpmx_handle* h, mpmx_stats* o) {
     if (rc) return rc;
     o->n_active = h->h_counters[CNT_LIVE];
     const int64_t st = std::max<int64_t>(h->steps_last, 1);
-    o->plastic_count = (int64_t)(h->stats_last[ST_PLASTIC] / st);
-    o->fallback_count = (int64_t)(h->stats_last[ST_FALLBACK] / st);
+    o->plastic_count = (int64_t)h->h_stats[ST_PLASTIC_LAST];
+    o->fallback_count = (int64_t)h->h_stats[ST_FALLBACK_LAST];
     o->active_nodes = (int64_t)(h->stats_last[ST_ACTIVE_NODES] / st);
     o->active_blocks = (int64_t)(h->stats_last[ST_NBLOCKS] / st);
     o->stray_count = (int64_t)h->stats_last[ST_STRAY];
@@ -917,6 +926,8 @@ int mpmx_profile_steps(mpmx_handle* h, int nsteps, mpmx_kernel_times* out) {
   hipEvent_t ev[6];
   for (auto& e : ev) HIPCHK(h, hipEventCreate(&e));
   double t_f = 0, t_s = 0, t_p = 0, t_g = 0, t_all = 0, live = 0;
+  // t_f: k_g2p (+ dead copy), t_s: sort (scans, cell scan, permutation),
+  // t_p: k_p2g, t_g: k_grid
   int done = 0;
   while (done < nsteps) {
     const int chunk = std::min(nsteps - done, MAX_CHUNK);
@@ -924,11 +935,13 @@ int mpmx_profile_steps(mpmx_handle* h, int nsteps, mpmx_kernel_times* out) {
     for (int i = 0; i < chunk; ++i) {
       const DevPtrs d = current(h);
       HIPCHK(h, hipEventRecord(ev[0], h->stream));
-      launch_fused(d, grid_fused(h), h->stream);
+      launch_g2p(d, grid_fused(h), h->stream);
       HIPCHK(h, hipEventRecord(ev[1], h->stream));
       launch_scan(d, h->nkeys, 1, h->stream);
-      HIPCHK(h, hipEventRecord(ev[2], h->stream));
+      launch_cellscan(d, grid_grid(h), h->stream);
       launch_perm(d, h->np, h->stream);
+      HIPCHK(h, hipEventRecord(ev[2], h->stream));
+      launch_p2g(d, grid_fused(h), 1, h->stream);
       HIPCHK(h, hipEventRecord(ev[3], h->stream));
       launch_grid(d, grid_grid(h), h->stream);
       HIPCHK(h, hipEventRecord(ev[4], h->stream));
@@ -951,9 +964,9 @@ int mpmx_profile_steps(mpmx_handle* h, int nsteps, mpmx_kernel_times* out) {
   for (auto& e : ev) (void)hipEventDestroy(e);
   if ((rc = read_stats(h)) != 0) return rc;
   out->steps = nsteps;
-  out->ms_fused = (float)(t_f / nsteps);
-  out->ms_scan = (float)(t_s / nsteps);
-  out->ms_perm = (float)(t_p / nsteps);
+  out->ms_g2p = (float)(t_f / nsteps);
+  out->ms_sort = (float)(t_s / nsteps);
+  out->ms_p2g = (float)(t_p / nsteps);
   out->ms_grid = (float)(t_g / nsteps);
   out->ms_step = (float)(t_all / nsteps);
   out->particles_per_step = live / nsteps;

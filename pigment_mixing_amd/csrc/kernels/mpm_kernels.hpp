@@ -8,8 +8,11 @@ namespace mpmx {
 
 // particle component rows (each a contiguous array of `cap` floats)
 enum {
-  P_X = 0, P_V = 3, P_C = 6, P_F = 15, P_PIG = 24, P_DIFF = 31, P_EPS = 32, P_DG = 33, P_NCOMP = 34
+  P_X = 0, P_V = 3, P_C = 6, P_F = 15, P_PIG = 24, P_DIFF = 31, P_EPS = 32, P_DG = 33,
+  P_TAU = 34,  // symmetric Kirchhoff stress of the current F: xx, yy, zz, xy, xz, yz
+  P_NCOMP = 40
 };
+constexpr int GV_STRIDE = 16;  // floats per node in the gather grid (v3 flip3 pig7 shear mass pad)
 
 // int32 counters
 enum {
@@ -22,7 +25,10 @@ enum {
   CNT_NCOUNTERS = 8
 };
 // uint64 cumulative statistics
-enum { ST_PLASTIC = 0, ST_ACTIVE_NODES = 1, ST_STRAY = 2, ST_FALLBACK = 3, ST_NBLOCKS = 4, ST_HARVEST = 5, ST_N = 8 };
+enum {
+  ST_PLASTIC = 0, ST_ACTIVE_NODES = 1, ST_STRAY = 2, ST_FALLBACK = 3, ST_NBLOCKS = 4, ST_HARVEST = 5,
+  ST_PLASTIC_LAST = 6, ST_FALLBACK_LAST = 7, ST_PLASTIC_CUR = 8, ST_FALLBACK_CUR = 9, ST_N = 12
+};
 
 struct DevPtrs {
   const SceneDev* scenes;
@@ -36,7 +42,11 @@ struct DevPtrs {
   const uint8_t* active_in;
   int* key_out;
   int* rank_out;
+  int* cell_out;           // cell (0..63) of the particle in its block, -1 if dead
   int* perm;
+  int* cell_cnt;           // per (block, cell) particle counts of the next sort
+  int* cell_off;           // per (block, cell) offsets inside the block
+  int* cell_num;           // per (block, cell) counts of the current sort
   int* count_next;
   int* count_cur;
   int* offsets;
@@ -61,13 +71,15 @@ struct DevPtrs {
   int pad;
 };
 
+void launch_init_tau(const DevPtrs& d, int64_t n, hipStream_t st);
 void launch_keys(const DevPtrs& d, int64_t n, hipStream_t st);
 void launch_scan(const DevPtrs& d, int nkeys, int advance, hipStream_t st);
 int scan_chunks(int nkeys);
+void launch_cellscan(const DevPtrs& d, int grid, hipStream_t st);
 void launch_perm(const DevPtrs& d, int64_t n, hipStream_t st);
-void launch_p2g_init(const DevPtrs& d, int grid, hipStream_t st);
+void launch_g2p(const DevPtrs& d, int grid, hipStream_t st);
+void launch_p2g(const DevPtrs& d, int grid, int src_is_b, hipStream_t st);
 void launch_grid(const DevPtrs& d, int grid, hipStream_t st);
-void launch_fused(const DevPtrs& d, int grid, hipStream_t st);
 void launch_unsort(const DevPtrs& d, int64_t n, float* out, int first, int ncomp, hipStream_t st);
 void launch_active_out(const DevPtrs& d, int64_t n, uint8_t* out, hipStream_t st);
 void launch_pack_render(const DevPtrs& d, int64_t n, float* pos4, float* lat7, float* diff, hipStream_t st);
