@@ -19,6 +19,7 @@ OBJ = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libmpmx.so")
 DEMO = os.path.join(OBJ, "simulation_api_demo")
+COPY_PROBE = os.path.join(OBJ, "hbm_copy")   # HBM-counter calibration (plain copy)
 
 SOURCES = [
     "kernels/mpm_kernels.hip",
@@ -78,6 +79,11 @@ def build(verbose=False):
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"demo build failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    probe_src = os.path.join(CSRC, "bench", "hbm_copy.hip")
+    cmd = [hipcc(), "-x", "hip", *COMMON, probe_src, "-o", COPY_PROBE]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"copy probe build failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     if verbose:
         print("built", LIB)
     return LIB

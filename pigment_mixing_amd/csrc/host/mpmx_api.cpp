@@ -287,8 +287,12 @@ DevPtrs swapped(const DevPtrs& d) {
   return e;
 }
 
-int grid_fused(mpmx_handle* h) { return std::max(1, h->n_cu * 4); }
-int grid_grid(mpmx_handle* h) { return std::max(1, h->n_cu * 4); }
+// Persistent grids are a multiple of 8 workgroups: the block walkers split
+// their lists into one contiguous segment per XCD (8 on gfx950).
+int grid_fused(mpmx_handle* h) { return std::max(8, (h->n_cu * 4 + 7) / 8 * 8); }
+int grid_grid(mpmx_handle* h) { return std::max(8, (h->n_cu * 4 + 7) / 8 * 8); }
+// P2G runs one-wavefront workgroups: 16 per CU
+int grid_p2g(mpmx_handle* h) { return std::max(8, (h->n_cu * 16 + 7) / 8 * 8); }
 
 // One step: G2P (A -> B, counts for the new sort), sort, P2G from B, grid update.
 void enqueue_step(mpmx_handle* h, const DevPtrs& d) {
@@ -296,7 +300,7 @@ void enqueue_step(mpmx_handle* h, const DevPtrs& d) {
   launch_scan(d, h->nkeys, 1, h->stream);
   launch_cellscan(d, grid_grid(h), h->stream);
   launch_perm(d, h->np, h->stream);
-  launch_p2g(d, grid_fused(h), 1, h->stream);
+  launch_p2g(d, grid_p2g(h), 1, h->stream);
   launch_grid(d, grid_grid(h), h->stream);
 }
 
@@ -656,7 +660,7 @@ int mpmx_prepare(mpmx_handle* h) {
   launch_scan(d, h->nkeys, 0, h->stream);
   launch_cellscan(d, grid_grid(h), h->stream);
   launch_perm(d, n, h->stream);
-  launch_p2g(d, grid_fused(h), 0, h->stream);
+  launch_p2g(d, grid_p2g(h), 0, h->stream);
   launch_grid(d, grid_grid(h), h->stream);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -941,7 +945,7 @@ int mpmx_profile_steps(mpmx_handle* h, int nsteps, mpmx_kernel_times* out) {
       launch_cellscan(d, grid_grid(h), h->stream);
       launch_perm(d, h->np, h->stream);
       HIPCHK(h, hipEventRecord(ev[2], h->stream));
-      launch_p2g(d, grid_fused(h), 1, h->stream);
+      launch_p2g(d, grid_p2g(h), 1, h->stream);
       HIPCHK(h, hipEventRecord(ev[3], h->stream));
       launch_grid(d, grid_grid(h), h->stream);
       HIPCHK(h, hipEventRecord(ev[4], h->stream));

@@ -78,6 +78,25 @@ __device__ __forceinline__ void block_coords(const SceneDev& sc, int key, int& b
   bk = lb % sc.bdims[2];
 }
 
+// Persistent workgroups walk the block list XCD by XCD: workgroups are handed
+// to the 8 XCDs round-robin (blockIdx.x % 8), so XCD x takes the x-th
+// contiguous eighth of the (spatially ordered) block list and neighbouring
+// blocks share halo nodes in that XCD's L2.
+struct XcdRange {
+  int e, end, stride;
+};
+__device__ __forceinline__ XcdRange xcd_range(int nblk) {
+  const int nx = 8;
+  const int xcd = blockIdx.x % nx;
+  const int per = gridDim.x / nx;
+  const int seg = (nblk + nx - 1) / nx;
+  XcdRange r;
+  r.e = xcd * seg + (int)(blockIdx.x / nx);
+  r.end = min(nblk, (xcd + 1) * seg);
+  r.stride = per;
+  return r;
+}
+
 // Count a particle for the next sort: block count (+ rank for dead particles)
 // and per-cell rank for live ones.  All lanes call.
 __device__ __forceinline__ void count_for_sort(const DevPtrs& d, bool valid, int key, int cell, int64_t j) {
@@ -249,30 +268,50 @@ __global__ void k_perm(DevPtrs d, int64_t n) {
 }
 
 // ------------------------------------------------------------------ k_g2p
-__device__ __forceinline__ void gather_node(const float* __restrict__ gv, const SceneDev& sc, int gi, int gj,
-                                            int gk, float4& a, float4& b, float4& c, float4& e) {
-  const int nb = node_block_of(sc, gi, gj, gk);
-  const float4* g = reinterpret_cast<const float4*>(gv + ((size_t)nb * BLK3 + node_local(gi, gj, gk)) * GV_STRIDE);
-  a = g[0];
-  b = g[1];
-  c = g[2];
-  e = g[3];
-}
+constexpr int GT = 6;              // gather tile edge: stencils of bins [B0, B0+4) span nodes [B0, B0+6)
+constexpr int GT3 = GT * GT * GT;  // 216 nodes x 16 floats = 13.8 KB
 
-__global__ void __launch_bounds__(256) k_g2p(DevPtrs d) {
+// Read-only inputs come in as __restrict__ kernel arguments (scene table, node
+// coordinates, particle input side, permutation, grid values), so uniform reads
+// of them become scalar loads that the particle stores cannot clobber.
+__global__ void __launch_bounds__(256) k_g2p(DevPtrs d, const SceneDev* __restrict__ scenes,
+                                             const float* __restrict__ axis, const float* __restrict__ A,
+                                             const int* __restrict__ perm, const float* __restrict__ gv) {
+  __shared__ float4 gtile[GT3 * 4];
+  __shared__ float sax[3][GT];       // node coordinates B0 .. B0+5 per axis
+  __shared__ unsigned int plastic_wg;
   const int step = d.counters[CNT_STEP];
   const int nblk = d.counters[CNT_PBLOCKS];
-  const float* __restrict__ A = d.pa;
   float* __restrict__ Bf = d.pb;
   const int64_t cap = d.cap;
-  for (int e = blockIdx.x; e < nblk; e += gridDim.x) {
-    const int key = d.pb_list[e];
-    const int s = d.blk_scene[key];
-    const SceneDev& sc = d.scenes[s];
+  if (threadIdx.x == 0) plastic_wg = 0u;
+  const XcdRange xr = xcd_range(nblk);
+  for (int e = xr.e; e < xr.end; e += xr.stride) {
+    const int key = __builtin_amdgcn_readfirstlane(d.pb_list[e]);
+    const int s = __builtin_amdgcn_readfirstlane(d.blk_scene[key]);
+    const SceneDev& sc = scenes[s];
+    int bi, bj, bk;
+    block_coords(sc, key, bi, bj, bk);
+    const int B0i = bi * 4, B0j = bj * 4, B0k = bk * 4;
+    __syncthreads();  // previous block done with the tile
+    // stage the gather grid of nodes [B0, B0+6)^3 (coalesced 16-byte rows)
+    for (int q = threadIdx.x; q < GT3 * 4; q += blockDim.x) {
+      const int t = q >> 2, part = q & 3;
+      const int gi = min(B0i + t / (GT * GT), sc.dims[0] - 1);
+      const int gj = min(B0j + (t / GT) % GT, sc.dims[1] - 1);
+      const int gk = min(B0k + t % GT, sc.dims[2] - 1);
+      const float4* g = reinterpret_cast<const float4*>(
+          gv + ((size_t)node_block_of(sc, gi, gj, gk) * BLK3 + node_local(gi, gj, gk)) * GV_STRIDE);
+      gtile[q] = g[part];
+    }
+    if (threadIdx.x < 3 * GT) {
+      const int a = threadIdx.x / GT, o = threadIdx.x % GT;
+      const int B0a = a == 0 ? B0i : (a == 1 ? B0j : B0k);
+      const int first = a == 0 ? 0 : (a == 1 ? sc.dims[0] : sc.dims[0] + sc.dims[1]);
+      sax[a][o] = axis[sc.axis_off + first + min(B0a + o, sc.dims[a] - 1)];
+    }
+    __syncthreads();
     const float time = d.kstate[(size_t)step * d.n_scenes + s].time;
-    const float* xs = d.axis_coords + sc.axis_off;
-    const float* ys = xs + sc.dims[0];
-    const float* zs = ys + sc.dims[1];
     // pigment boost (g2p.cpp:107-119)
     float tf = 1.0f;
     if (time > sc.start_boost && sc.end_boost > sc.start_boost) {
@@ -289,7 +328,7 @@ __global__ void __launch_bounds__(256) k_g2p(DevPtrs d) {
       const int64_t j = start + t;
       int newkey = d.dead_key, newcell = -1, plastic = 0;
       if (valid) {
-        const int src = d.perm[j];
+        const int src = perm[j];
         float x[3], v[3], F[9], pig[NPIG];
 #pragma unroll
         for (int c = 0; c < 3; ++c) { x[c] = A[(P_X + c) * cap + src]; v[c] = A[(P_V + c) * cap + src]; }
@@ -305,9 +344,11 @@ __global__ void __launch_bounds__(256) k_g2p(DevPtrs d) {
         b[0] = stencil_base(x[0], sc.origin[0], sc.one_over_dx);
         b[1] = stencil_base(x[1], sc.origin[1], sc.one_over_dx);
         b[2] = stencil_base(x[2], sc.origin[2], sc.one_over_dx);
-        axis_weights(x[0], xs, b[0], sc.one_over_dx, w[0]);
-        axis_weights(x[1], ys, b[1], sc.one_over_dx, w[1]);
-        axis_weights(x[2], zs, b[2], sc.one_over_dx, w[2]);
+        // stencil bases relative to the staged coordinates (0 .. 3)
+        const int r0 = b[0] - B0i, r1 = b[1] - B0j, r2 = b[2] - B0k;
+        axis_weights(x[0], sax[0], r0, sc.one_over_dx, w[0]);
+        axis_weights(x[1], sax[1], r1, sc.one_over_dx, w[1]);
+        axis_weights(x[2], sax[2], r2, sc.one_over_dx, w[2]);
         float pic[3] = {0, 0, 0}, fl[3] = {0, 0, 0}, Bp[9], L[9], gp[NPIG], gs = 0.0f;
 #pragma unroll
         for (int c = 0; c < 9; ++c) { Bp[c] = 0.0f; L[c] = 0.0f; }
@@ -316,7 +357,7 @@ __global__ void __launch_bounds__(256) k_g2p(DevPtrs d) {
         const float inv_dx = sc.one_over_dx;
         // 9 stencil rows (oi, oj), each with its 3 nodes along z unrolled: keeps one
         // row of gather loads and weights live instead of all 27 nodes.
-        const float zc[3] = {zs[b[2]] - x[2], zs[b[2] + 1] - x[2], zs[b[2] + 2] - x[2]};
+        const float zc[3] = {sax[2][r2] - x[2], sax[2][r2 + 1] - x[2], sax[2][r2 + 2] - x[2]};
 #pragma unroll 1
         for (int row = 0; row < 9; ++row) {
           const int oi = row / 3, oj = row - 3 * oi;
@@ -324,12 +365,12 @@ __global__ void __launch_bounds__(256) k_g2p(DevPtrs d) {
           const float dNx = oi == 0 ? w[0].dN[0] : (oi == 1 ? w[0].dN[1] : w[0].dN[2]);
           const float Ny = oj == 0 ? w[1].N[0] : (oj == 1 ? w[1].N[1] : w[1].N[2]);
           const float dNy = oj == 0 ? w[1].dN[0] : (oj == 1 ? w[1].dN[1] : w[1].dN[2]);
-          const float pd0 = xs[b[0] + oi] - x[0];
-          const float pd1 = ys[b[1] + oj] - x[1];
+          const float pd0 = sax[0][r0 + oi] - x[0];
+          const float pd1 = sax[1][r1 + oj] - x[1];
           const float nij = Nx * Ny;
           const float dij0 = dNx * Ny;
           const float dij1 = Nx * dNy;
-          const int gi = b[0] + oi, gj = b[1] + oj;
+          const int trow = ((r0 + oi) * GT + (r1 + oj)) * GT + r2;
 #pragma unroll
           for (int ok = 0; ok < 3; ++ok) {
             const float pd2 = zc[ok];
@@ -337,8 +378,8 @@ __global__ void __launch_bounds__(256) k_g2p(DevPtrs d) {
             const float g0 = (dij0 * w[2].N[ok]) * inv_dx;
             const float g1 = (dij1 * w[2].N[ok]) * inv_dx;
             const float g2 = (nij * w[2].dN[ok]) * inv_dx;
-            float4 n0, n1, n2, n3;
-            gather_node(d.gv, sc, gi, gj, b[2] + ok, n0, n1, n2, n3);
+            const float4* gn = &gtile[(trow + ok) * 4];
+            const float4 n0 = gn[0], n1 = gn[1], n2 = gn[2], n3 = gn[3];
             const float gvv[3] = {n0.x, n0.y, n0.z};
             const float gfl[3] = {n0.w, n1.x, n1.y};
             const float gpg[NPIG] = {n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, n3.x};
@@ -480,9 +521,11 @@ __global__ void __launch_bounds__(256) k_g2p(DevPtrs d) {
       }
       count_for_sort(d, valid, newkey, newcell, j);
       const uint64_t pm = __ballot(valid && plastic);
-      if (__lane_id() == 0 && pm) atomicAdd(&d.stats[ST_PLASTIC_CUR], (unsigned long long)__popcll(pm));
+      if (__lane_id() == 0 && pm) atomicAdd(&plastic_wg, (unsigned)__popcll(pm));
     }
   }
+  __syncthreads();
+  if (threadIdx.x == 0 && plastic_wg) atomicAdd(&d.stats[ST_PLASTIC_CUR], (unsigned long long)plastic_wg);
 }
 
 // Dead particles keep their state: copy them to the output side and rank them.
@@ -503,210 +546,265 @@ __global__ void k_copy_dead(DevPtrs d) {
 }
 
 // ------------------------------------------------------------------ k_p2g
-// One workgroup (6 wavefronts) per particle block of the new sort.  A pre-pass
-// evaluates every particle's per-axis spline weights once into LDS; then lane =
-// cell of the block (4x4x4 bins) and wavefront = channel group:
+// Work item = (particle block of the new sort, channel group); one wavefront
+// per item, lane = cell of the block (4x4x4 bins):
 //   0: mass (u64 fixed point)   1: APIC momentum     2: Kirchhoff force
 //   3: latent 0..2              4: latent 3..5       5: latent 6 + shear intensity
-// Each lane accumulates its cell's contributions to the 27 stencil nodes in
-// registers and adds them to a 6x6x6-node LDS tile; lanes of one instruction hold
-// distinct cells, so they never hit the same tile node.
+// Each lane walks its cell's particles (cell-sorted, so the stencil base is the
+// cell for all of them), accumulates the 27 stencil nodes x 3 channels in
+// registers, then adds them to the wavefront's private 6x6x6-node LDS tile
+// (lanes of one instruction hold distinct cells: no two hit one node) and the
+// tile goes to the grid accumulators with global atomics.  Items are
+// independent, so there is no workgroup-wide phase: many small waves per CU
+// overlap their particle loads with other waves' arithmetic.
 constexpr int PT = 6;                 // tile edge (nodes) = 4 bins + 2
 constexpr int PT3 = PT * PT * PT;     // 216
-constexpr int P2G_THREADS = 384;
-constexpr int P2G_MAXP = 768;         // particles staged per pass (larger blocks loop)
+constexpr int P2G_GROUPS = 6;
 
-struct P2GShared {
-  float acc[ACC_CH][PT3];
+union P2GTile {
+  float acc[3][PT3];
   unsigned long long mass[PT3];
-  float wN[9][P2G_MAXP];             // N_x[0..2], N_y[0..2], N_z[0..2] per staged particle
-  unsigned int touched;
 };
 
 __device__ __forceinline__ int tile_index(int ci, int cj, int ck, int n) {
   return ((ci + n / 9) * PT + (cj + (n / 3) % 3)) * PT + (ck + n % 3);
 }
 
-__global__ void __launch_bounds__(P2G_THREADS) k_p2g(DevPtrs d, int src_is_b) {
-  __shared__ P2GShared sh;
+// Per-item inputs shared by the channel-group bodies.
+struct P2GItem {
+  const float* P;      // particle side holding the sorted step's state
+  int64_t cap;
+  const int* pp;       // permutation entries of this lane's cell (in bounds)
+  int cnt, kmax;       // particles of this lane's cell, deepest cell of the wave
+  float X[3][3];       // node coordinates of the lane's stencil per axis
+  float inv_dx, apicDinv;
+};
+
+// Accumulate this lane's cell into acc[27][3] for channel group G.  The loop
+// runs kmax (wave-uniform) times; a lane past its own count contributes exact
+// zeros through selects, so the accumulators never sit behind a branch.  Every
+// load is in bounds for every lane (an empty cell reads its block's first slot,
+// a finished lane re-reads its last particle).
+template <int G>
+__device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[27][3]) {
+  const float* __restrict__ P = it.P;
+  const int64_t cap = it.cap;
+  const float inv_dx = it.inv_dx;
+  const int klast = max(it.cnt - 1, 0);
+  int src = it.pp[0];
+  for (int k = 0; k < it.kmax; ++k) {
+    const bool act = k < it.cnt;
+    const int cur = src;
+    src = it.pp[min(k + 1, klast)];   // next particle's slot, loaded ahead
+    const float x0r = P[(P_X + 0) * cap + cur], x1r = P[(P_X + 1) * cap + cur], x2r = P[(P_X + 2) * cap + cur];
+    const float x0 = act ? x0r : it.X[0][1];
+    const float x1 = act ? x1r : it.X[1][1];
+    const float x2 = act ? x2r : it.X[2][1];
+    float N[9];
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+      const float n0 = spline_N(__fmul_rn(__fsub_rn(x0, it.X[0][o]), inv_dx));
+      N[o] = act ? n0 : 0.0f;
+      N[3 + o] = spline_N(__fmul_rn(__fsub_rn(x1, it.X[1][o]), inv_dx));
+      N[6 + o] = spline_N(__fmul_rn(__fsub_rn(x2, it.X[2][o]), inv_dx));
+    }
+    if constexpr (G == 0) {
+#pragma unroll
+      for (int n = 0; n < 27; ++n) {
+        const float wgt = (N[n / 9] * N[3 + (n / 3) % 3]) * N[6 + n % 3];
+        unsigned long long m = ((unsigned long long)__float_as_uint(acc[n][1]) << 32) |
+                               (unsigned long long)__float_as_uint(acc[n][0]);
+        m += wgt > 1e-25f ? mass_to_fx(wgt) : 0ull;
+        acc[n][0] = __uint_as_float((unsigned)m);
+        acc[n][1] = __uint_as_float((unsigned)(m >> 32));
+      }
+    } else if constexpr (G == 1) {
+      float v[3], C[9];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const float t = P[(P_V + c) * cap + cur];
+        v[c] = act ? t : 0.0f;
+      }
+#pragma unroll
+      for (int c = 0; c < 9; ++c) {
+        const float t = P[(P_C + c) * cap + cur] * it.apicDinv;
+        C[c] = act ? t : 0.0f;
+      }
+#pragma unroll
+      for (int n = 0; n < 27; ++n) {
+        float wgt = (N[n / 9] * N[3 + (n / 3) % 3]) * N[6 + n % 3];
+        wgt = wgt > 1e-25f ? wgt : 0.0f;
+        const float pd0 = it.X[0][n / 9] - x0, pd1 = it.X[1][(n / 3) % 3] - x1, pd2 = it.X[2][n % 3] - x2;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+          acc[n][r] += (v[r] + (C[3 * r] * pd0 + C[3 * r + 1] * pd1 + C[3 * r + 2] * pd2)) * wgt;
+      }
+    } else if constexpr (G == 2) {
+      float dN[9], tau[6];
+#pragma unroll
+      for (int o = 0; o < 3; ++o) {
+        dN[o] = spline_dN(__fmul_rn(__fsub_rn(x0, it.X[0][o]), inv_dx)) * inv_dx;
+        dN[3 + o] = spline_dN(__fmul_rn(__fsub_rn(x1, it.X[1][o]), inv_dx)) * inv_dx;
+        dN[6 + o] = spline_dN(__fmul_rn(__fsub_rn(x2, it.X[2][o]), inv_dx)) * inv_dx;
+      }
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        const float t = P[(P_TAU + c) * cap + cur];
+        tau[c] = act ? t : 0.0f;
+      }
+#pragma unroll
+      for (int n = 0; n < 27; ++n) {
+        const int oi = n / 9, oj = (n / 3) % 3, ok = n % 3;
+        const float g0 = (dN[oi] * N[3 + oj]) * N[6 + ok];
+        const float g1 = (N[oi] * dN[3 + oj]) * N[6 + ok];
+        const float g2 = (N[oi] * N[3 + oj]) * dN[6 + ok];
+        acc[n][0] += tau[0] * g0 + tau[3] * g1 + tau[4] * g2;
+        acc[n][1] += tau[3] * g0 + tau[1] * g1 + tau[5] * g2;
+        acc[n][2] += tau[4] * g0 + tau[5] * g1 + tau[2] * g2;
+      }
+    } else {
+      // latent / shear: three (group 5: two) weighted scalars
+      constexpr int q0 = G == 3 ? P_PIG : (G == 4 ? P_PIG + 3 : P_PIG + 6);
+      constexpr int q1 = G == 5 ? P_DIFF : q0 + 1;
+      const float ta = P[q0 * cap + cur];
+      const float tb = P[q1 * cap + cur];
+      const float qa = act ? ta : 0.0f;
+      const float qb = act ? tb : 0.0f;
+      float qc = 0.0f;
+      if constexpr (G != 5) {
+        const float tc = P[(q0 + 2) * cap + cur];
+        qc = act ? tc : 0.0f;
+      }
+#pragma unroll
+      for (int n = 0; n < 27; ++n) {
+        const float wgt = (N[n / 9] * N[3 + (n / 3) % 3]) * N[6 + n % 3];
+        acc[n][0] += qa * wgt;
+        acc[n][1] += qb * wgt;
+        if constexpr (G != 5) acc[n][2] += qc * wgt;
+      }
+    }
+  }
+}
+
+// Lanes (distinct cells) add their 27 nodes into the wave's tile.
+template <int G>
+__device__ __forceinline__ void p2g_to_tile(P2GTile& tile, int ci, int cj, int ck, int cnt, const float (&acc)[27][3]) {
+  if (cnt <= 0) return;
+  if constexpr (G == 0) {
+#pragma unroll
+    for (int n = 0; n < 27; ++n)
+      atomicAdd(&tile.mass[tile_index(ci, cj, ck, n)],
+                ((unsigned long long)__float_as_uint(acc[n][1]) << 32) | (unsigned long long)__float_as_uint(acc[n][0]));
+  } else {
+#pragma unroll
+    for (int n = 0; n < 27; ++n) {
+      const int tix = tile_index(ci, cj, ck, n);
+      atomicAdd(&tile.acc[0][tix], acc[n][0]);
+      atomicAdd(&tile.acc[1][tix], acc[n][1]);
+      if constexpr (G != 5) atomicAdd(&tile.acc[2][tix], acc[n][2]);
+    }
+  }
+}
+
+template <int G>
+__device__ __forceinline__ void p2g_group(const P2GItem& it, P2GTile& tile, int ci, int cj, int ck) {
+  float acc[27][3];
+#pragma unroll
+  for (int n = 0; n < 27; ++n) { acc[n][0] = acc[n][1] = acc[n][2] = 0.0f; }
+  p2g_accumulate<G>(it, acc);
+  p2g_to_tile<G>(tile, ci, cj, ck, it.cnt, acc);
+}
+
+__global__ void __launch_bounds__(64) k_p2g(DevPtrs d, const SceneDev* __restrict__ scenes,
+                                            const float* __restrict__ axis, const float* __restrict__ P,
+                                            const int* __restrict__ perm) {
+  __shared__ P2GTile tile;
+  __shared__ unsigned int touched;
   const int nblk = d.counters[CNT_PBLOCKS];
   const int parity = d.counters[CNT_GSTEP] & 1;
-  const float* __restrict__ P = src_is_b ? d.pb : d.pa;
-  const int64_t cap = d.cap;
-  const int group = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x;
   const int ci = lane >> 4, cj = (lane >> 2) & 3, ck = lane & 3;
-  for (int e = blockIdx.x; e < nblk; e += gridDim.x) {
-    const int key = d.pb_list[e];
-    const SceneDev& sc = d.scenes[d.blk_scene[key]];
+  // items of this XCD: its contiguous segment of the block list x 6 groups
+  const int nx = 8;
+  const int xcd = blockIdx.x % nx;
+  const int per = gridDim.x / nx;
+  const int seg = (nblk + nx - 1) / nx;
+  const int e0 = xcd * seg;
+  const int nitems = max(0, min(nblk, e0 + seg) - e0) * P2G_GROUPS;
+  for (int t = lane; t < 3 * PT3; t += 64) (&tile.acc[0][0])[t] = 0.0f;
+  if (lane == 0) touched = 0u;
+  __syncthreads();
+  for (int item = blockIdx.x / nx; item < nitems; item += per) {
+    const int e = e0 + item / P2G_GROUPS;
+    const int group = item % P2G_GROUPS;
+    const int key = __builtin_amdgcn_readfirstlane(d.pb_list[e]);
+    const SceneDev& sc = scenes[__builtin_amdgcn_readfirstlane(d.blk_scene[key])];
     int bi, bj, bk;
     block_coords(sc, key, bi, bj, bk);
     const int B0i = bi * 4, B0j = bj * 4, B0k = bk * 4;
-    for (int i = threadIdx.x; i < ACC_CH * PT3; i += blockDim.x) (&sh.acc[0][0])[i] = 0.0f;
-    for (int i = threadIdx.x; i < PT3; i += blockDim.x) sh.mass[i] = 0ull;
-    if (threadIdx.x == 0) sh.touched = 0u;
-    const int boff = d.offsets[key];
-    const int nb_p = d.count_cur[key];
-    const int cnt = d.cell_num[key * BLK3 + lane];
+    P2GItem it;
+    it.P = P;
+    it.cap = d.cap;
+    it.inv_dx = sc.one_over_dx;
+    it.apicDinv = sc.apicDinv;
+    it.cnt = d.cell_num[key * BLK3 + lane];
     const int coff = d.cell_off[key * BLK3 + lane];
-    const float* xs = d.axis_coords + sc.axis_off;
-    const float* ys = xs + sc.dims[0];
-    const float* zs = ys + sc.dims[1];
-    const float inv_dx = sc.one_over_dx;
+    it.pp = perm + d.offsets[key] + (it.cnt > 0 ? coff : 0);
+    int kmax = it.cnt;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) kmax = max(kmax, __shfl_xor(kmax, off));
+    it.kmax = __builtin_amdgcn_readfirstlane(kmax);
     // node coordinates of this cell's stencil (clamped for empty edge cells)
-    const int b0 = B0i + ci, b1 = B0j + cj, b2 = B0k + ck;
-    float X0[3], X1[3], X2[3];
+    const float* xs = axis + sc.axis_off;
 #pragma unroll
     for (int o = 0; o < 3; ++o) {
-      X0[o] = xs[min(b0 + o, sc.dims[0] - 1)];
-      X1[o] = ys[min(b1 + o, sc.dims[1] - 1)];
-      X2[o] = zs[min(b2 + o, sc.dims[2] - 1)];
+      it.X[0][o] = xs[min(B0i + ci + o, sc.dims[0] - 1)];
+      it.X[1][o] = xs[sc.dims[0] + min(B0j + cj + o, sc.dims[1] - 1)];
+      it.X[2][o] = xs[sc.dims[0] + sc.dims[1] + min(B0k + ck + o, sc.dims[2] - 1)];
     }
-    // channel-group accumulators; the mass group keeps its u64 sums bit-cast in
-    // acc[n][0] (low word) and acc[n][1] (high word)
-    float acc[27][3];
-#pragma unroll
-    for (int n = 0; n < 27; ++n) { acc[n][0] = acc[n][1] = acc[n][2] = 0.0f; }
-    for (int c0 = 0; c0 < nb_p; c0 += P2G_MAXP) {
-      __syncthreads();  // previous pass done with wN
-      const int nstage = min(P2G_MAXP, nb_p - c0);
-      for (int s = threadIdx.x; s < nstage; s += blockDim.x) {
-        const int src = d.perm[boff + c0 + s];
-        const float x0 = P[(P_X + 0) * cap + src], x1 = P[(P_X + 1) * cap + src], x2 = P[(P_X + 2) * cap + src];
-        const int p0 = stencil_base(x0, sc.origin[0], inv_dx);
-        const int p1 = stencil_base(x1, sc.origin[1], inv_dx);
-        const int p2 = stencil_base(x2, sc.origin[2], inv_dx);
-#pragma unroll
-        for (int o = 0; o < 3; ++o) {
-          sh.wN[o][s] = spline_N(__fmul_rn(__fsub_rn(x0, xs[p0 + o]), inv_dx));
-          sh.wN[3 + o][s] = spline_N(__fmul_rn(__fsub_rn(x1, ys[p1 + o]), inv_dx));
-          sh.wN[6 + o][s] = spline_N(__fmul_rn(__fsub_rn(x2, zs[p2 + o]), inv_dx));
-        }
+    switch (group) {
+      case 0: p2g_group<0>(it, tile, ci, cj, ck); break;
+      case 1: p2g_group<1>(it, tile, ci, cj, ck); break;
+      case 2: p2g_group<2>(it, tile, ci, cj, ck); break;
+      case 3: p2g_group<3>(it, tile, ci, cj, ck); break;
+      case 4: p2g_group<4>(it, tile, ci, cj, ck); break;
+      default: p2g_group<5>(it, tile, ci, cj, ck); break;
+    }
+    __syncthreads();
+    // ---- tile -> grid accumulators (read and clear)
+    if (group == 0) {
+      for (int t = lane; t < PT3; t += 64) {
+        const unsigned long long m = tile.mass[t];
+        if (m == 0ull) continue;
+        tile.mass[t] = 0ull;
+        const int ti = t / (PT * PT), tj = (t / PT) % PT, tk = t % PT;
+        const int gi = B0i + ti, gj = B0j + tj, gk = B0k + tk;
+        atomicAdd(&d.massfx[(size_t)node_block_of(sc, gi, gj, gk) * BLK3 + node_local(gi, gj, gk)], m);
+        atomicOr(&touched, 1u << ((((ti >> 2) * 2) + (tj >> 2)) * 2 + (tk >> 2)));
       }
       __syncthreads();
-      // this lane's particles inside the staged range
-      const int k0 = max(0, c0 - coff), k1 = min(cnt, c0 + nstage - coff);
-      for (int k = k0; __ballot(k < k1); ++k) {
-        if (k >= k1) continue;
-        const int s = coff + k - c0;
-        const int src = d.perm[boff + coff + k];
-        float N[9];
-#pragma unroll
-        for (int q = 0; q < 9; ++q) N[q] = sh.wN[q][s];
-        if (group == 0) {
-#pragma unroll
-          for (int n = 0; n < 27; ++n) {
-            const float wgt = (N[n / 9] * N[3 + (n / 3) % 3]) * N[6 + n % 3];
-            if (wgt > 1e-25f) {
-              unsigned long long m = ((unsigned long long)__float_as_uint(acc[n][1]) << 32) |
-                                     (unsigned long long)__float_as_uint(acc[n][0]);
-              m += mass_to_fx(wgt);
-              acc[n][0] = __uint_as_float((unsigned)m);
-              acc[n][1] = __uint_as_float((unsigned)(m >> 32));
-            }
-          }
-        } else if (group == 1) {
-          const float x0 = P[(P_X + 0) * cap + src], x1 = P[(P_X + 1) * cap + src], x2 = P[(P_X + 2) * cap + src];
-          float v[3], C[9];
-#pragma unroll
-          for (int c = 0; c < 3; ++c) v[c] = P[(P_V + c) * cap + src];
-#pragma unroll
-          for (int c = 0; c < 9; ++c) C[c] = P[(P_C + c) * cap + src] * sc.apicDinv;
-#pragma unroll
-          for (int n = 0; n < 27; ++n) {
-            const float wgt = (N[n / 9] * N[3 + (n / 3) % 3]) * N[6 + n % 3];
-            if (!(wgt > 1e-25f)) continue;
-            const float pd0 = X0[n / 9] - x0, pd1 = X1[(n / 3) % 3] - x1, pd2 = X2[n % 3] - x2;
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-              acc[n][r] += (v[r] + (C[3 * r] * pd0 + C[3 * r + 1] * pd1 + C[3 * r + 2] * pd2)) * wgt;
-          }
-        } else if (group == 2) {
-          const float x0 = P[(P_X + 0) * cap + src], x1 = P[(P_X + 1) * cap + src], x2 = P[(P_X + 2) * cap + src];
-          float dN[9], tau[6];
-#pragma unroll
-          for (int o = 0; o < 3; ++o) {
-            dN[o] = spline_dN(__fmul_rn(__fsub_rn(x0, X0[o]), inv_dx)) * inv_dx;
-            dN[3 + o] = spline_dN(__fmul_rn(__fsub_rn(x1, X1[o]), inv_dx)) * inv_dx;
-            dN[6 + o] = spline_dN(__fmul_rn(__fsub_rn(x2, X2[o]), inv_dx)) * inv_dx;
-          }
-#pragma unroll
-          for (int c = 0; c < 6; ++c) tau[c] = P[(P_TAU + c) * cap + src];
-#pragma unroll
-          for (int n = 0; n < 27; ++n) {
-            const int oi = n / 9, oj = (n / 3) % 3, ok = n % 3;
-            const float g0 = (dN[oi] * N[3 + oj]) * N[6 + ok];
-            const float g1 = (N[oi] * dN[3 + oj]) * N[6 + ok];
-            const float g2 = (N[oi] * N[3 + oj]) * dN[6 + ok];
-            acc[n][0] += tau[0] * g0 + tau[3] * g1 + tau[4] * g2;
-            acc[n][1] += tau[3] * g0 + tau[1] * g1 + tau[5] * g2;
-            acc[n][2] += tau[4] * g0 + tau[5] * g1 + tau[2] * g2;
-          }
-        } else {
-          // latent / shear: three weighted scalars per group
-          float q[3];
-          if (group == 3) {
-            q[0] = P[(P_PIG + 0) * cap + src]; q[1] = P[(P_PIG + 1) * cap + src]; q[2] = P[(P_PIG + 2) * cap + src];
-          } else if (group == 4) {
-            q[0] = P[(P_PIG + 3) * cap + src]; q[1] = P[(P_PIG + 4) * cap + src]; q[2] = P[(P_PIG + 5) * cap + src];
-          } else {
-            q[0] = P[(P_PIG + 6) * cap + src]; q[1] = P[P_DIFF * cap + src]; q[2] = 0.0f;
-          }
-#pragma unroll
-          for (int n = 0; n < 27; ++n) {
-            const float wgt = (N[n / 9] * N[3 + (n / 3) % 3]) * N[6 + n % 3];
-            acc[n][0] += q[0] * wgt;
-            acc[n][1] += q[1] * wgt;
-            acc[n][2] += q[2] * wgt;
-          }
-        }
+      if (lane < 8 && (touched & (1u << lane))) {
+        const int nb = sc.node_block_base + ((bi + (lane >> 2)) * sc.bdims[1] + (bj + ((lane >> 1) & 1))) * sc.bdims[2] +
+                       (bk + (lane & 1));
+        activate_node_block(d, nb, parity);
       }
-    }
-    // ---- lanes (distinct cells) add their 27 nodes into the LDS tile
-    if (cnt > 0) {
-      if (group == 0) {
-#pragma unroll
-        for (int n = 0; n < 27; ++n)
-          atomicAdd(&sh.mass[tile_index(ci, cj, ck, n)],
-                    ((unsigned long long)__float_as_uint(acc[n][1]) << 32) | (unsigned long long)__float_as_uint(acc[n][0]));
-      } else if (group == 5) {
-#pragma unroll
-        for (int n = 0; n < 27; ++n) {
-          const int t = tile_index(ci, cj, ck, n);
-          atomicAdd(&sh.acc[A_PIG + 6][t], acc[n][0]);
-          atomicAdd(&sh.acc[A_SHR][t], acc[n][1]);
-        }
-      } else {
-        const int ch0 = group == 1 ? A_MOM : (group == 2 ? A_FRC : (group == 3 ? A_PIG : A_PIG + 3));
-#pragma unroll
-        for (int n = 0; n < 27; ++n) {
-          const int t = tile_index(ci, cj, ck, n);
-          atomicAdd(&sh.acc[ch0 + 0][t], acc[n][0]);
-          atomicAdd(&sh.acc[ch0 + 1][t], acc[n][1]);
-          atomicAdd(&sh.acc[ch0 + 2][t], acc[n][2]);
-        }
+      __syncthreads();
+      if (lane == 0) touched = 0u;
+    } else {
+      // group 5's channels are latent 6 and shear (A_PIG + 6, A_SHR = A_PIG + 7)
+      const int ch0 = group == 1 ? A_MOM : (group == 2 ? A_FRC : (group == 3 ? A_PIG : (group == 4 ? A_PIG + 3 : A_PIG + 6)));
+      const int nch = group == 5 ? 2 : 3;
+      for (int t = lane; t < PT3; t += 64) {
+        const float a0 = tile.acc[0][t], a1 = tile.acc[1][t], a2 = tile.acc[2][t];
+        if (a0 == 0.0f && a1 == 0.0f && a2 == 0.0f) continue;
+        tile.acc[0][t] = 0.0f; tile.acc[1][t] = 0.0f; tile.acc[2][t] = 0.0f;
+        const int ti = t / (PT * PT), tj = (t / PT) % PT, tk = t % PT;
+        const int gi = B0i + ti, gj = B0j + tj, gk = B0k + tk;
+        float* g = d.acc + (size_t)node_block_of(sc, gi, gj, gk) * ACC_CH * BLK3 + node_local(gi, gj, gk);
+        if (a0 != 0.0f) atomicAdd(g + (ch0 + 0) * BLK3, a0);
+        if (a1 != 0.0f) atomicAdd(g + (ch0 + 1) * BLK3, a1);
+        if (nch == 3 && a2 != 0.0f) atomicAdd(g + (ch0 + 2) * BLK3, a2);
       }
-    }
-    __syncthreads();
-    // ---- tile -> grid accumulators
-    for (int t = threadIdx.x; t < PT3; t += blockDim.x) {
-      const unsigned long long m = sh.mass[t];
-      if (m == 0ull) continue;
-      const int ti = t / (PT * PT), tj = (t / PT) % PT, tk = t % PT;
-      const int gi = B0i + ti, gj = B0j + tj, gk = B0k + tk;
-      const int nb = node_block_of(sc, gi, gj, gk);
-      const int loc = node_local(gi, gj, gk);
-#pragma unroll
-      for (int c = 0; c < ACC_CH; ++c) atomicAdd(&d.acc[((size_t)nb * ACC_CH + c) * BLK3 + loc], sh.acc[c][t]);
-      atomicAdd(&d.massfx[(size_t)nb * BLK3 + loc], m);
-      atomicOr(&sh.touched, 1u << ((((ti >> 2) * 2) + (tj >> 2)) * 2 + (tk >> 2)));
-    }
-    __syncthreads();
-    if (threadIdx.x < 8 && (sh.touched & (1u << threadIdx.x))) {
-      const int r = threadIdx.x;
-      const int nb = sc.node_block_base + ((bi + (r >> 2)) * sc.bdims[1] + (bj + ((r >> 1) & 1))) * sc.bdims[2] +
-                     (bk + (r & 1));
-      activate_node_block(d, nb, parity);
     }
     __syncthreads();
   }
@@ -715,6 +813,9 @@ __global__ void __launch_bounds__(P2G_THREADS) k_p2g(DevPtrs d, int src_is_b) {
 // ------------------------------------------------------------------ k_grid
 // One wavefront per active node block; lane = node.
 __global__ void __launch_bounds__(256) k_grid(DevPtrs d) {
+  __shared__ unsigned int active_wg;
+  if (threadIdx.x == 0) active_wg = 0u;
+  __syncthreads();
   const int step = d.counters[CNT_STEP];
   const int parity = d.counters[CNT_GSTEP] & 1;
   if (blockIdx.x == 0 && threadIdx.x == 0) d.counters[CNT_NBLOCKS0 + (parity ^ 1)] = 0;
@@ -766,7 +867,7 @@ __global__ void __launch_bounds__(256) k_grid(DevPtrs d) {
       out[G_MASS] = mass;
     }
     const uint64_t act = __ballot(mfx != 0ull);
-    if (lane == 0 && act) atomicAdd(&d.stats[ST_ACTIVE_NODES], (unsigned long long)__popcll(act));
+    if (lane == 0 && act) atomicAdd(&active_wg, (unsigned)__popcll(act));
     float4* g = reinterpret_cast<float4*>(d.gv + ((size_t)nb * BLK3 + lane) * GV_STRIDE);
     g[0] = make_float4(out[0], out[1], out[2], out[3]);
     g[1] = make_float4(out[4], out[5], out[6], out[7]);
@@ -777,6 +878,8 @@ __global__ void __launch_bounds__(256) k_grid(DevPtrs d) {
     d.massfx[(size_t)nb * BLK3 + lane] = 0ull;
     if (lane == 0) d.nb_flag[nb] = 0;
   }
+  __syncthreads();
+  if (threadIdx.x == 0 && active_wg) atomicAdd(&d.stats[ST_ACTIVE_NODES], (unsigned long long)active_wg);
 }
 
 // --------------------------------------------------------------- downloads
@@ -857,11 +960,13 @@ void launch_perm(const DevPtrs& d, int64_t n, hipStream_t st) {
   if (n > 0) hipLaunchKernelGGL(k_perm, dim3(blocks_for(n, 256)), dim3(256), 0, st, d, n);
 }
 void launch_g2p(const DevPtrs& d, int grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_g2p, dim3(grid), dim3(256), 0, st, d);
+  hipLaunchKernelGGL(k_g2p, dim3(grid), dim3(256), 0, st, d, (const SceneDev*)d.scenes, (const float*)d.axis_coords,
+                     (const float*)d.pa, (const int*)d.perm, (const float*)d.gv);
   hipLaunchKernelGGL(k_copy_dead, dim3(64), dim3(256), 0, st, d);
 }
 void launch_p2g(const DevPtrs& d, int grid, int src_is_b, hipStream_t st) {
-  hipLaunchKernelGGL(k_p2g, dim3(grid), dim3(P2G_THREADS), 0, st, d, src_is_b);
+  hipLaunchKernelGGL(k_p2g, dim3(grid), dim3(64), 0, st, d, (const SceneDev*)d.scenes, (const float*)d.axis_coords,
+                     (const float*)(src_is_b ? d.pb : d.pa), (const int*)d.perm);
 }
 void launch_grid(const DevPtrs& d, int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_grid, dim3(grid), dim3(256), 0, st, d);
