@@ -442,6 +442,10 @@ int mpmx_create(const mpmx_config* cfg, mpmx_handle** out) {
   *out = nullptr;
   if (!(cfg->dx > 0) || !(cfg->dt > 0) || !(cfg->ppc > 0) || !(cfg->rho > 0)) return MPMX_ERR_ARG;
   if (cfg->plastic_model != MPMX_PLASTIC_NONE && cfg->plastic_model != MPMX_PLASTIC_VMVISC) return MPMX_ERR_ARG;
+  // MUSL's second scatter (musl.cpp:6-98) has no device kernel: refuse the
+  // flag rather than run a USL step under a MUSL configuration.
+  if (cfg->use_musl != 0) return MPMX_ERR_ARG;
+  if (cfg->closure != MPMX_CLOSURE_EXACT && cfg->closure != MPMX_CLOSURE_LEARNED) return MPMX_ERR_ARG;
   mpmx_handle* h = new mpmx_handle();
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) {
