@@ -7,9 +7,18 @@ Perzyna (yield 50 Pa, viscosity 10 Pa s), slab knife on a 40 mm circle at
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 
-Prints ONE JSON line.  `value` = live particles x K / wall seconds of K steps
-(graph-launched, inputs resident in HBM, synchronised before and after).
-`roofline` is for the dominant kernel of the step (by time), its mean
+Prints ONE JSON line.  Two runs of the same scene from the same initial state:
+  plain   : exact von Mises/Perzyna return map;
+  learned : the learned closure (BF16 MLP on MFMA, refined mode: one Newton step
+            from the predicted plastic increment, accepted when the yield
+            residual is within 1e-4 q_y, else the exact map), its network
+            trained on the GPU on pairs harvested from the plain run.
+`value` is the learned run's live particles x K / wall seconds of K steps
+(graph-launched, inputs resident in HBM, synchronised before and after); the
+plain run's figure is under `plain`.  `learned` reports the held-out principal
+Kirchhoff-stress error of accepted predictions against the exact map (bar
+1e-3) and the acceptance / fallback rates.
+`roofline` is for the dominant kernel of the learned step (by time), its mean
 duration measured with HIP events on the library's stream over a second run of
 K steps; every kernel's figure is listed under roofline.kernels.
 `cpu_baseline` times the CPU restatement of the reference step (OpenMP, the
@@ -65,6 +74,48 @@ def cpu_baseline(scene, steps=2, threads=None):
                       f"(reference cost structure), -O3 -ffast-math"}
 
 
+def timed_run(h, steps, warmup):
+    h.step(warmup)
+    h.sync()
+    t0 = time.perf_counter()
+    h.step(steps)
+    h.sync()
+    wall = time.perf_counter() - t0
+    return wall, h.stats()
+
+
+def kernel_report(kt):
+    n_act = kt.active_nodes_per_step
+    kb = kernel_bytes(kt.particles_per_step, n_act)
+    kms = {"k_g2p": kt.ms_g2p, "k_p2g": kt.ms_p2g, "k_grid": kt.ms_grid}
+    kgbs = {k: kb[k] / (kms[k] * 1e-3) / 1e9 for k in kb}
+    dom = max(kms, key=kms.get)
+    step_gbs = step_bytes(kt.particles_per_step, n_act) / (kt.ms_step * 1e-3) / 1e9
+    return {
+        "bound": "hbm",
+        "kernel": dom,
+        "achieved": kgbs[dom],
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": kgbs[dom] / HBM_PEAK_GBS,
+        "traffic": None,
+        "algorithmic_bytes_per_launch": kb[dom],
+        "kernel_ms": kms[dom],
+        "kernels": {k: {"ms": kms[k], "GB/s": kgbs[k], "frac": kgbs[k] / HBM_PEAK_GBS,
+                        "algorithmic_bytes": kb[k]} for k in kb},
+        "sort_ms": kt.ms_sort,
+        "step_ms_events": kt.ms_step,
+        "step_frac": step_gbs / HBM_PEAK_GBS,
+        "active_nodes": n_act,
+    }
+
+
+def principal_tau(hp, E, nu):
+    mu = E / (2 * (1 + nu))
+    lam = nu * E / ((1 + nu) * (1 - 2 * nu))
+    return 2 * mu * hp + lam * hp.sum(1, keepdims=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,42 +123,71 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--harvest-steps", type=int, default=40)
+    ap.add_argument("--train-steps", type=int, default=3000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (multi-GPU sharding is design-only); use --gpus 1",
               file=sys.stderr)
         sys.exit(2)
 
-    from pigment_mixing_amd import scenes
+    from pigment_mixing_amd import abi, scenes
 
     scene = scenes.paint_4blob()
     n = scene["x"].shape[0]
+    cfg = scene["cfg"]
     print(f"[bench] scene {scene['name']} particles={n}", file=sys.stderr, flush=True)
+
+    # ---- plain (exact return map)
     h = scenes.make_handle(scene)
-    h.step(args.warmup)
-    h.sync()
-    print("[bench] warmup done", file=sys.stderr, flush=True)
-
-    t0 = time.perf_counter()
-    h.step(args.steps)
-    h.sync()
-    wall = time.perf_counter() - t0
-    st = h.stats()
-    live = st.n_active
-    value = live * args.steps / wall
-    print(f"[bench] timed {args.steps} steps in {wall:.3f}s", file=sys.stderr, flush=True)
-
-    kt = h.profile(args.steps)
-    n_act = kt.active_nodes_per_step
-    kb = kernel_bytes(kt.particles_per_step, n_act)
-    kms = {"k_g2p": kt.ms_g2p, "k_p2g": kt.ms_p2g, "k_grid": kt.ms_grid}
-    kgbs = {k: kb[k] / (kms[k] * 1e-3) / 1e9 for k in kb}
-    dom = max(kms, key=kms.get)
-    step_gbs = step_bytes(kt.particles_per_step, n_act) / (kt.ms_step * 1e-3) / 1e9
-    print(f"[bench] kernels ms: g2p={kt.ms_g2p:.4f} sort={kt.ms_sort:.4f} p2g={kt.ms_p2g:.4f} "
-          f"grid={kt.ms_grid:.4f} step={kt.ms_step:.4f} active_nodes={n_act:.0f} "
-          f"GB/s g2p={kgbs['k_g2p']:.0f} p2g={kgbs['k_p2g']:.0f} grid={kgbs['k_grid']:.0f}",
+    wall_p, st_p = timed_run(h, args.steps, args.warmup)
+    value_p = st_p.n_active * args.steps / wall_p
+    kt_p = h.profile(args.steps)
+    rf_p = kernel_report(kt_p)
+    print(f"[bench] plain: {args.steps} steps in {wall_p:.3f}s -> {value_p:.4g} particle-steps/s; kernels ms "
+          f"g2p={kt_p.ms_g2p:.4f} sort={kt_p.ms_sort:.4f} p2g={kt_p.ms_p2g:.4f} grid={kt_p.ms_grid:.4f}",
           file=sys.stderr, flush=True)
+
+    # ---- harvest (trial, corrected) pairs from the plain run, train on the GPU
+    cap = 6_000_000
+    h.harvest_begin(cap)
+    h.step(args.harvest_steps)
+    rec = h.harvest_end(cap)
+    h.close()
+    rng = np.random.default_rng(0)
+    order = rng.permutation(rec.shape[0])
+    n_hold = max(1, rec.shape[0] // 10)
+    hold, train = rec[order[:n_hold]], rec[order[n_hold:]]
+    cfg.closure = 2          # MPMX_CLOSURE_LEARNED_REFINED
+    cfg.closure_tol = 1e-4
+    hl = scenes.make_handle(scene)
+    cfg.closure = 0
+    rep, _ = hl.closure_train(train, steps=args.train_steps, batch=8192, lr=2e-3, lr_final=5e-5, seed=7,
+                              report_every=500)
+    print(f"[bench] harvested {rec.shape[0]} pairs over {args.harvest_steps} steps; trained {rep.steps} steps "
+          f"in {rep.ms:.0f} ms, loss {rep.loss_first:.3e} -> {rep.loss_full:.3e}", file=sys.stderr, flush=True)
+    trial = np.concatenate([hold[:, 0:3], np.zeros((hold.shape[0], 1), np.float32)], axis=1)
+    lrn, ex, acc = hl.closure_eval(trial)
+    te = principal_tau(ex[:, 0:3].astype(np.float64), cfg.E, cfg.nu)
+    if acc.any():
+        tl = principal_tau(lrn[acc, 0:3].astype(np.float64), cfg.E, cfg.nu)
+        rel = np.linalg.norm(tl - te[acc], axis=1) / np.linalg.norm(te[acc], axis=1)
+        err_max, err_med = float(rel.max()), float(np.median(rel))
+    else:
+        err_max = err_med = None
+
+    # ---- learned closure run (same scene, same steps)
+    wall, st = timed_run(hl, args.steps, args.warmup)
+    value = st.n_active * args.steps / wall
+    print(f"[bench] learned: {args.steps} steps in {wall:.3f}s -> {value:.4g} particle-steps/s; last step plastic "
+          f"{st.plastic_count} accepted {st.learned_count} fallback {st.fallback_count}; held-out stress err max "
+          f"{err_max}", file=sys.stderr, flush=True)
+    kt = hl.profile(args.steps)
+    rf = kernel_report(kt)
+    print(f"[bench] learned kernels ms: g2p={kt.ms_g2p:.4f} sort={kt.ms_sort:.4f} p2g={kt.ms_p2g:.4f} "
+          f"grid={kt.ms_grid:.4f} step={kt.ms_step:.4f} active_nodes={kt.active_nodes_per_step:.0f}",
+          file=sys.stderr, flush=True)
+    st_last = hl.stats()
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -126,34 +206,31 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded box blobs, Mixbox latents of palette colours)",
+        "data": "synthetic (seeded box blobs, Mixbox latents of palette colours); closure trained on pairs "
+                "harvested from the plain run of the same scene",
         "config": {
             "workload": scene["name"],
             "particles": n,
             "lattice": [256, 64, 256],
             "dx_m": scenes.DX,
-            "dt_s": float(scene["cfg"].dt),
-            "return_map": "exact von Mises/Perzyna",
+            "dt_s": float(cfg.dt),
+            "return_map": "learned closure (BF16 MLP 8-64x4-4 on MFMA, refined, tol 1e-4 q_y) with exact fallback",
             "knife": "slab 60x2x30 mm tilted 30 deg, circular sweep r=40 mm at 0.15 m/s",
             "parallelism": "single GPU",
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": dom,
-            "achieved": kgbs[dom],
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": kgbs[dom] / HBM_PEAK_GBS,
-            "traffic": None,
-            "algorithmic_bytes_per_launch": kb[dom],
-            "kernel_ms": kms[dom],
-            "kernels": {k: {"ms": kms[k], "GB/s": kgbs[k], "frac": kgbs[k] / HBM_PEAK_GBS,
-                            "algorithmic_bytes": kb[k]} for k in kb},
-            "sort_ms": kt.ms_sort,
-            "step_ms_events": kt.ms_step,
-            "step_frac": step_gbs / HBM_PEAK_GBS,
-            "active_nodes": n_act,
+        "plain": {"value": value_p, "ms_per_step": wall_p / args.steps * 1e3, "roofline": rf_p},
+        "learned": {
+            "mode": "refined", "closure_tol": 1e-4,
+            "speedup_vs_plain": value / value_p,
+            "last_step": {"plastic": st_last.plastic_count, "accepted": st_last.learned_count,
+                          "fallback": st_last.fallback_count},
+            "fallback_rate_last_step": st_last.fallback_count / max(1, st_last.plastic_count),
+            "held_out": {"pairs": int(hold.shape[0]), "accepted_frac": float(acc.mean()),
+                         "stress_rel_err_max": err_max, "stress_rel_err_median": err_med, "bar": 1e-3},
+            "training": {"pairs": int(train.shape[0]), "steps": int(rep.steps), "batch": 8192,
+                         "ms": rep.ms, "loss_first": rep.loss_first, "loss_full": rep.loss_full},
         },
+        "roofline": rf,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

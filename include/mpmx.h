@@ -62,7 +62,12 @@ typedef enum { MPMX_PLASTIC_NONE = 0, MPMX_PLASTIC_VMVISC = 5 } mpmx_plastic_mod
 /* Return-map closure used inside the step. */
 typedef enum {
   MPMX_CLOSURE_EXACT = 0,    /* von Mises/Perzyna return map (plasticity.cpp:163-238) */
-  MPMX_CLOSURE_LEARNED = 1   /* BF16 MLP on MFMA, exact map as per-particle fallback */
+  MPMX_CLOSURE_LEARNED = 1,  /* BF16 MLP on MFMA predicts (h', dgamma); accepted when
+                                the predicted state's yield residual is within
+                                closure_tol * q_y, else the exact map (fallback)     */
+  MPMX_CLOSURE_LEARNED_REFINED = 2 /* as 1, but the predicted dgamma starts one Newton
+                                step on the consistency residual and h' follows the
+                                flow rule; same acceptance test and fallback          */
 } mpmx_closure;
 
 /* Knife (palette knife / spatula) shapes. */
@@ -171,6 +176,8 @@ typedef struct {
   int64_t stray_count;       /* particles scattered outside their LDS tile, last call */
   uint32_t device_error;     /* bit mask; nonzero <=> reference exit = 1              */
   float   ms_last_step;      /* wall ms of the last mpmx_step call / its steps        */
+  int64_t learned_count;     /* learned closure: predictions accepted in the last step */
+  int64_t newton_iters;      /* exact map: Newton iterations (plasticity.cpp:196-229), sum over the last call */
 } mpmx_stats;
 
 typedef struct mpmx_handle mpmx_handle;
@@ -334,10 +341,60 @@ int mpmx_closure_set_norm(mpmx_handle* h, const float* in_mean, const float* in_
 
 /* Harvest (trial, corrected) pairs from the exact solver: while enabled, each
  * step appends up to `capacity` records of 12 floats
- * (8 inputs: h1,h2,h3, ||dev h||, q_y, perzyna_visc, dt*E, nu; 4 targets:
- *  h1',h2',h3', dgamma) for particles whose trial state is plastic. */
+ * (8 inputs: h1,h2,h3 (trial principal Hencky strains), ||dev h||, q_y,
+ *  eta = q_y * perzyna_visc (Pa s), dt*E, nu; 4 targets: h1',h2',h3' (returned
+ *  strains), dgamma (plastic increment)) for particles whose trial state is
+ *  plastic (plasticity.cpp:163-238). */
 int mpmx_harvest_begin(mpmx_handle* h, int64_t capacity);
 int mpmx_harvest_end(mpmx_handle* h, float* out12, int64_t max_records, int64_t* n_out);
+
+/* Training of the closure on the device (own forward/backward MFMA kernels,
+ * Adam, cosine learning-rate decay from lr to lr_final).  The network learns
+ * normalised (h' - h_trial, dgamma) from the 8 inputs; normalisation is the
+ * per-column mean / standard deviation of the records. */
+typedef struct {
+  int64_t steps;       /* Adam steps                                         */
+  int   batch;         /* samples per step, a positive multiple of 64        */
+  float lr, lr_final;  /* learning rate, cosine-decayed to lr_final          */
+  float beta1, beta2, eps;
+  uint64_t seed;       /* parameter init and minibatch sampling              */
+  int   init;          /* 1: seeded Xavier init; 0: continue from the handle's parameters */
+  int   report_every;  /* steps between minibatch-loss samples               */
+} mpmx_train_config;
+
+typedef struct {
+  float loss_first, loss_last;  /* minibatch MSE (normalised) at the first / last step */
+  float loss_full;              /* MSE over all records with the final parameters      */
+  int64_t steps, n_curve;       /* steps run; loss samples written to `curve`          */
+  double ms;                    /* wall time of the training loop                      */
+  float norm[24];               /* in_mean[8] in_std[8] out_mean[4] out_std[4]         */
+} mpmx_train_report;
+
+void mpmx_train_config_default(mpmx_train_config* tc);
+/* Trains on n harvest records (12 floats each, layout as above) and installs
+ * the result (parameters + normalisation) in the handle.  No counterpart in
+ * the reference (the learned closure is this build's addition). */
+int mpmx_closure_train(mpmx_handle* h, const float* records12, int64_t n, const mpmx_train_config* tc,
+                       mpmx_train_report* report, float* curve, int64_t curve_cap);
+
+/* The closure's accuracy in principal space: for n trial states (h1 h2 h3 of
+ * the trial principal Hencky strain, eps_pl_dev) runs the learned map (the
+ * handle's closure mode, tolerance and network) and the exact map
+ * (plasticity.cpp:163-238) with scene 0's material.  learned4 / exact4 get
+ * (h1' h2' h3' dgamma) per state; learned4 is NaN where the prediction was not
+ * accepted (accepted[i] = 0).  Needs a prepared handle with a learned closure
+ * mode and parameters.  No counterpart in the reference. */
+int mpmx_closure_eval(mpmx_handle* h, const float* trial4, int64_t n, float* learned4, float* exact4,
+                      uint8_t* accepted);
+
+/* Handle-free evaluation of the network on the device, for checks against a
+ * CPU implementation: norm24 = in_mean[8] in_std[8] out_mean[4] out_std[4].
+ * mpmx_mlp_forward: y4 = network(x8) in output units.
+ * mpmx_mlp_gradient: loss = mean over samples and outputs of the squared
+ * normalised error, and its gradient with respect to the parameters. */
+int mpmx_mlp_forward(const float* params, const float* norm24, const float* x8, int64_t n, float* y4);
+int mpmx_mlp_gradient(const float* params, const float* norm24, const float* x8, const float* t4, int64_t n,
+                      float* grad, float* loss);
 
 #ifdef __cplusplus
 }

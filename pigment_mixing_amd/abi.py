@@ -86,6 +86,7 @@ class Stats(ctypes.Structure):
         ("fallback_count", ctypes.c_int64), ("active_nodes", ctypes.c_int64),
         ("active_blocks", ctypes.c_int64), ("stray_count", ctypes.c_int64),
         ("device_error", ctypes.c_uint32), ("ms_last_step", ctypes.c_float),
+        ("learned_count", ctypes.c_int64), ("newton_iters", ctypes.c_int64),
     ]
 
 
@@ -99,6 +100,19 @@ class KernelTimes(ctypes.Structure):
 
 class Box(ctypes.Structure):
     _fields_ = [("lo", ctypes.c_float * 3), ("cells", ctypes.c_int * 3), ("latent", ctypes.c_float * 7)]
+
+
+class TrainConfig(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int64), ("batch", ctypes.c_int), ("lr", ctypes.c_float),
+                ("lr_final", ctypes.c_float), ("beta1", ctypes.c_float), ("beta2", ctypes.c_float),
+                ("eps", ctypes.c_float), ("seed", ctypes.c_uint64), ("init", ctypes.c_int),
+                ("report_every", ctypes.c_int)]
+
+
+class TrainReport(ctypes.Structure):
+    _fields_ = [("loss_first", ctypes.c_float), ("loss_last", ctypes.c_float), ("loss_full", ctypes.c_float),
+                ("steps", ctypes.c_int64), ("n_curve", ctypes.c_int64), ("ms", ctypes.c_double),
+                ("norm", ctypes.c_float * 24)]
 
 
 _LIB = None
@@ -136,6 +150,19 @@ def _sig(lib):
         ("mpmx_scene_boxes", ctypes.c_int,
          [ctypes.POINTER(Box), ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_uint64, c_float_p, c_float_p,
           ctypes.c_int64, c_i64_p]),
+        ("mpmx_closure_n_params", ctypes.c_int64, []),
+        ("mpmx_closure_set_params", ctypes.c_int, [H, c_float_p, ctypes.c_int64]),
+        ("mpmx_closure_get_params", ctypes.c_int, [H, c_float_p, ctypes.c_int64]),
+        ("mpmx_closure_set_norm", ctypes.c_int, [H, c_float_p, c_float_p, c_float_p, c_float_p]),
+        ("mpmx_harvest_begin", ctypes.c_int, [H, ctypes.c_int64]),
+        ("mpmx_harvest_end", ctypes.c_int, [H, c_float_p, ctypes.c_int64, c_i64_p]),
+        ("mpmx_train_config_default", None, [ctypes.POINTER(TrainConfig)]),
+        ("mpmx_closure_train", ctypes.c_int, [H, c_float_p, ctypes.c_int64, ctypes.POINTER(TrainConfig),
+                                              ctypes.POINTER(TrainReport), c_float_p, ctypes.c_int64]),
+        ("mpmx_closure_eval", ctypes.c_int, [H, c_float_p, ctypes.c_int64, c_float_p, c_float_p, c_u8_p]),
+        ("mpmx_mlp_forward", ctypes.c_int, [c_float_p, c_float_p, c_float_p, ctypes.c_int64, c_float_p]),
+        ("mpmx_mlp_gradient", ctypes.c_int,
+         [c_float_p, c_float_p, c_float_p, c_float_p, ctypes.c_int64, c_float_p, c_float_p]),
     ]
     for name, res, args in S:
         f = getattr(lib, name)
@@ -182,6 +209,44 @@ def scene_boxes(boxes, dx, per_axis=2, seed=42):
     if rc:
         raise RuntimeError(f"mpmx_scene_boxes: {rc}")
     return x, pig
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+def train_config(**kw):
+    tc = TrainConfig()
+    load().mpmx_train_config_default(ctypes.byref(tc))
+    for k, v in kw.items():
+        setattr(tc, k, v)
+    return tc
+
+
+def mlp_forward(params, norm24, x8):
+    """Device forward pass of the closure network: y4 = net(x8) in output units."""
+    lib = load()
+    p, n24, x = _f32(params), _f32(norm24), _f32(x8, (-1, 8))
+    y = np.zeros((x.shape[0], 4), np.float32)
+    rc = lib.mpmx_mlp_forward(fptr(p), fptr(n24), fptr(x), x.shape[0], fptr(y))
+    if rc:
+        raise MpmxError(rc, "mpmx_mlp_forward")
+    return y
+
+
+def mlp_gradient(params, norm24, x8, t4):
+    """Device loss and parameter gradient of the closure network."""
+    lib = load()
+    p, n24, x, t = _f32(params), _f32(norm24), _f32(x8, (-1, 8)), _f32(t4, (-1, 4))
+    g = np.zeros(p.size, np.float32)
+    loss = np.zeros(1, np.float32)
+    rc = lib.mpmx_mlp_gradient(fptr(p), fptr(n24), fptr(x), fptr(t), x.shape[0], fptr(g), fptr(loss))
+    if rc:
+        raise MpmxError(rc, "mpmx_mlp_gradient")
+    return float(loss[0]), g
 
 
 class MpmxError(RuntimeError):
@@ -293,3 +358,44 @@ class Handle:
         ks = KnifeState()
         self._chk(self.lib.mpmx_knife_state_at(self.h, 0, step, ctypes.byref(ks)), "knife_state")
         return ks
+
+    # ---- learned closure
+    def harvest_begin(self, capacity):
+        self._chk(self.lib.mpmx_harvest_begin(self.h, int(capacity)), "harvest_begin")
+
+    def harvest_end(self, max_records):
+        out = np.zeros((int(max_records), 12), np.float32)
+        n = ctypes.c_int64(0)
+        self._chk(self.lib.mpmx_harvest_end(self.h, fptr(out), int(max_records), ctypes.byref(n)), "harvest_end")
+        return out[:n.value]
+
+    def closure_train(self, records, curve_cap=256, **kw):
+        rec = _f32(records, (-1, 12))
+        tc = train_config(**kw)
+        rep = TrainReport()
+        curve = np.zeros(curve_cap, np.float32)
+        self._chk(self.lib.mpmx_closure_train(self.h, fptr(rec), rec.shape[0], ctypes.byref(tc), ctypes.byref(rep),
+                                              fptr(curve), curve_cap), "closure_train")
+        return rep, curve[:rep.n_curve].copy()
+
+    def closure_params(self):
+        n = self.lib.mpmx_closure_n_params()
+        p = np.zeros(n, np.float32)
+        self._chk(self.lib.mpmx_closure_get_params(self.h, fptr(p), n), "closure_get_params")
+        return p
+
+    def closure_eval(self, trial4):
+        t = _f32(trial4, (-1, 4))
+        n = t.shape[0]
+        lrn = np.zeros((n, 4), np.float32)
+        ex = np.zeros((n, 4), np.float32)
+        acc = np.zeros(n, np.uint8)
+        self._chk(self.lib.mpmx_closure_eval(self.h, fptr(t), n, fptr(lrn), fptr(ex), acc.ctypes.data_as(c_u8_p)),
+                  "closure_eval")
+        return lrn, ex, acc.astype(bool)
+
+    def closure_set(self, params, norm24):
+        p, n24 = _f32(params), _f32(norm24)
+        self._chk(self.lib.mpmx_closure_set_norm(self.h, fptr(n24[0:8]), fptr(n24[8:16]), fptr(n24[16:20]),
+                                                 fptr(n24[20:24])), "closure_set_norm")
+        self._chk(self.lib.mpmx_closure_set_params(self.h, fptr(p), p.size), "closure_set_params")

@@ -421,7 +421,31 @@ MPMX_HD void svd3(const float F[9], float U[9], float sig[3], float V[9]) {
 // In: h[3] = log(max(sigma,1e-4)), eps (eps_pl_dev).  Out: h (corrected), eps,
 // dgamma_rate (= delta_gamma/dt, plasticity.cpp:235).  Returns 1 if projected,
 // sets err bits on the reference's fatal conditions.
-MPMX_HD int vmvisc_principal(const SceneDev& sc, float h[3], float& eps, float& dg_rate, uint32_t& err) {
+// Perzyna consistency residual r = q (dt / (visc dgam + dt))^n - q_y(eps + dgam/d)
+// at equivalent stress q (plasticity.cpp:196-229), and its derivative along the
+// flow rule q = q_tr - f dgam.
+MPMX_HD float perzyna_residual(const SceneDev& sc, float q, float dgam, float eps_in) {
+  const float tm = sc.perzyna_visc * dgam + sc.dt;
+  const float tmp = sc.dt / tm;
+  const float tmp1 = sc.perzyna_exp == 1.0f ? tmp : powf(tmp, sc.perzyna_exp);
+  const float qy = sc.q_min + (sc.q_max - sc.q_min) * expf(-sc.xi * (eps_in + sc.inv_d_prefac * dgam));
+  return q * tmp1 - qy;
+}
+MPMX_HD float perzyna_residual_slope(const SceneDev& sc, float q_trial, float dgam, float eps_in) {
+  const float tm = sc.perzyna_visc * dgam + sc.dt;
+  const float tmp = sc.dt / tm;
+  const float tmp1 = sc.perzyna_exp == 1.0f ? tmp : powf(tmp, sc.perzyna_exp);
+  const float ex = expf(-sc.xi * (eps_in + sc.inv_d_prefac * dgam));
+  const float qyd = -sc.xi * sc.inv_d_prefac * (sc.q_max - sc.q_min) * ex;
+  return -sc.f_mu_prefac * tmp1 +
+         (q_trial - sc.f_mu_prefac * dgam) * sc.perzyna_exp *
+             (sc.perzyna_exp == 1.0f ? 1.0f : powf(tmp, sc.perzyna_exp - 1.0f)) * (-sc.perzyna_visc * sc.dt) /
+             (tm * tm) -
+         qyd;
+}
+
+MPMX_HD int vmvisc_principal(const SceneDev& sc, float h[3], float& eps, float& dg_rate, uint32_t& err,
+                             int* newton_iters = nullptr) {
   float htr = (h[0] + h[1]) + h[2];
   float m = htr / 3.0f;
   float dv[3] = {h[0] - m, h[1] - m, h[2] - m};
@@ -447,6 +471,7 @@ MPMX_HD int vmvisc_principal(const SceneDev& sc, float h[3], float& eps, float& 
     dgam = 0.01f * (q_trial - q_yield) / sc.f_mu_prefac;
     const int max_iter = 60;
     for (int it = 0; it < max_iter; ++it) {
+      if (newton_iters) ++*newton_iters;
       if (it == max_iter - 1) err |= ERR_NEWTON;
       float tm = sc.perzyna_visc * dgam + sc.dt;
       float tmp = sc.dt / tm;

@@ -5,6 +5,7 @@
 #include <string>
 #include <vector>
 #include "../kernels/mpm_kernels.hpp"
+#include "../../../include/mpmx.h"
 
 namespace mpmx {
 
@@ -21,5 +22,9 @@ inline int closure_attach(ClosureState& c, const DevPtrs&, hipStream_t, std::str
   return 0;
 }
 void closure_free(ClosureState& c);
+void closure_init_params(std::vector<float>& p, uint64_t seed);
+int closure_train(std::vector<float>& params, float* n24_io, const float* records12, int64_t n,
+                  const mpmx_train_config& tc, mpmx_train_report* rep, float* curve, int64_t curve_cap,
+                  hipStream_t s, std::string& err);
 
 }  // namespace mpmx

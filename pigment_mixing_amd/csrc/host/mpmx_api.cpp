@@ -294,9 +294,17 @@ int grid_grid(mpmx_handle* h) { return std::max(8, (h->n_cu * 4 + 7) / 8 * 8); }
 // P2G runs one-wavefront workgroups: 16 per CU
 int grid_p2g(mpmx_handle* h) { return std::max(8, (h->n_cu * 16 + 7) / 8 * 8); }
 
+// G2P with the exact return map, or with the learned closure (network in LDS:
+// two 256-thread workgroups per CU).
+bool learned(const mpmx_handle* h) { return h->scenes[0].cfg.closure != MPMX_CLOSURE_EXACT; }
+void enqueue_g2p(mpmx_handle* h, const DevPtrs& d) {
+  if (learned(h)) launch_g2p(d, std::max(8, (h->n_cu * 2 + 7) / 8 * 8), h->closure.d_blob, h->stream);
+  else launch_g2p(d, grid_fused(h), nullptr, h->stream);
+}
+
 // One step: G2P (A -> B, counts for the new sort), sort, P2G from B, grid update.
 void enqueue_step(mpmx_handle* h, const DevPtrs& d) {
-  launch_g2p(d, grid_fused(h), h->stream);
+  enqueue_g2p(h, d);
   launch_scan(d, h->nkeys, 1, h->stream);
   launch_cellscan(d, grid_grid(h), h->stream);
   launch_perm(d, h->np, h->stream);
@@ -445,7 +453,10 @@ int mpmx_create(const mpmx_config* cfg, mpmx_handle** out) {
   // MUSL's second scatter (musl.cpp:6-98) has no device kernel: refuse the
   // flag rather than run a USL step under a MUSL configuration.
   if (cfg->use_musl != 0) return MPMX_ERR_ARG;
-  if (cfg->closure != MPMX_CLOSURE_EXACT && cfg->closure != MPMX_CLOSURE_LEARNED) return MPMX_ERR_ARG;
+  if (cfg->closure != MPMX_CLOSURE_EXACT && cfg->closure != MPMX_CLOSURE_LEARNED &&
+      cfg->closure != MPMX_CLOSURE_LEARNED_REFINED)
+    return MPMX_ERR_ARG;
+  if (cfg->closure != MPMX_CLOSURE_EXACT && !(cfg->closure_tol > 0)) return MPMX_ERR_ARG;
   mpmx_handle* h = new mpmx_handle();
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) {
@@ -693,6 +704,8 @@ int mpmx_get_lattice(mpmx_handle* h, int* dims, float* origin, float* xs, float*
 int mpmx_step(mpmx_handle* h, int nsteps) {
   if (!h || nsteps < 0) return MPMX_ERR_ARG;
   if (!h->prepared) return fail(h, MPMX_ERR_STATE, "step: prepare first");
+  if (learned(h) && !h->closure.d_blob)
+    return fail(h, MPMX_ERR_STATE, "step: learned closure selected but no network parameters set");
   if (nsteps == 0) return MPMX_OK;
   int rc;
   auto t0 = std::chrono::steady_clock::now();
@@ -846,6 +859,8 @@ int mpmx_stats_get(mpmx_handle* h, mpmx_stats* o) {
     o->active_blocks = (int64_t)(h->stats_last[ST_NBLOCKS] / st);
     o->stray_count = (int64_t)h->stats_last[ST_STRAY];
     o->device_error = *h->h_err;
+    o->learned_count = (int64_t)h->h_stats[ST_LEARNED_LAST];
+    o->newton_iters = (int64_t)h->stats_last[ST_NEWTON];
   }
   o->ms_last_step = h->ms_last;
   return MPMX_OK;
@@ -883,6 +898,51 @@ int mpmx_closure_set_norm(mpmx_handle* h, const float* in_mean, const float* in_
   if (h->mlp_params.empty()) return MPMX_OK;
   return closure_upload(h->closure, h->mlp_params, h->in_mean, h->in_std, h->out_mean, h->out_std, h->stream,
                         h->err);
+}
+
+int mpmx_closure_train(mpmx_handle* h, const float* records12, int64_t n, const mpmx_train_config* tc,
+                       mpmx_train_report* report, float* curve, int64_t curve_cap) {
+  if (!h || !records12 || !tc || n <= 0 || curve_cap < 0 || (curve_cap > 0 && !curve)) return MPMX_ERR_ARG;
+  float n24[24];
+  const int rc = closure_train(h->mlp_params, n24, records12, n, *tc, report, curve, curve_cap, h->stream, h->err);
+  if (rc) return rc;
+  std::memcpy(h->in_mean, n24, 32);
+  std::memcpy(h->in_std, n24 + 8, 32);
+  std::memcpy(h->out_mean, n24 + 16, 16);
+  std::memcpy(h->out_std, n24 + 20, 16);
+  return closure_upload(h->closure, h->mlp_params, h->in_mean, h->in_std, h->out_mean, h->out_std, h->stream,
+                        h->err);
+}
+
+int mpmx_closure_eval(mpmx_handle* h, const float* trial4, int64_t n, float* learned4, float* exact4,
+                      uint8_t* accepted) {
+  if (!h || n < 0 || (n > 0 && (!trial4 || !learned4 || !exact4 || !accepted))) return MPMX_ERR_ARG;
+  if (!h->prepared) return fail(h, MPMX_ERR_STATE, "closure_eval: prepare first");
+  if (!learned(h) || !h->closure.d_blob)
+    return fail(h, MPMX_ERR_STATE, "closure_eval: needs a learned closure mode and network parameters");
+  if (n == 0) return MPMX_OK;
+  float *dt = nullptr, *dl = nullptr, *de = nullptr;
+  uint8_t* da = nullptr;
+  int rc = MPMX_OK;
+  if (hipMalloc(&dt, sizeof(float) * 4 * n) != hipSuccess || hipMalloc(&dl, sizeof(float) * 4 * n) != hipSuccess ||
+      hipMalloc(&de, sizeof(float) * 4 * n) != hipSuccess || hipMalloc(&da, n) != hipSuccess) {
+    rc = fail(h, MPMX_ERR_HIP, "closure_eval: hipMalloc failed");
+  } else if (hipMemcpyAsync(dt, trial4, sizeof(float) * 4 * n, hipMemcpyHostToDevice, h->stream) != hipSuccess) {
+    rc = fail(h, MPMX_ERR_HIP, "closure_eval: upload failed");
+  } else {
+    launch_closure_eval(h->d_scenes, h->closure.d_blob, dt, n, dl, de, da, h->n_cu * 2, h->stream);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(learned4, dl, sizeof(float) * 4 * n, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipMemcpyAsync(exact4, de, sizeof(float) * 4 * n, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipMemcpyAsync(accepted, da, n, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+      rc = fail(h, MPMX_ERR_HIP, "closure_eval: kernel or copy failed");
+  }
+  if (dt) (void)hipFree(dt);
+  if (dl) (void)hipFree(dl);
+  if (de) (void)hipFree(de);
+  if (da) (void)hipFree(da);
+  return rc;
 }
 
 int mpmx_harvest_begin(mpmx_handle* h, int64_t capacity) {
@@ -927,6 +987,8 @@ int mpmx_harvest_end(mpmx_handle* h, float* out12, int64_t max_records, int64_t*
 int mpmx_profile_steps(mpmx_handle* h, int nsteps, mpmx_kernel_times* out) {
   if (!h || !out || nsteps <= 0) return MPMX_ERR_ARG;
   if (!h->prepared) return fail(h, MPMX_ERR_STATE, "profile: prepare first");
+  if (learned(h) && !h->closure.d_blob)
+    return fail(h, MPMX_ERR_STATE, "profile: learned closure selected but no network parameters set");
   std::memset(out, 0, sizeof *out);
   int rc = read_stats(h);
   if (rc) return rc;
@@ -943,7 +1005,7 @@ int mpmx_profile_steps(mpmx_handle* h, int nsteps, mpmx_kernel_times* out) {
     for (int i = 0; i < chunk; ++i) {
       const DevPtrs d = current(h);
       HIPCHK(h, hipEventRecord(ev[0], h->stream));
-      launch_g2p(d, grid_fused(h), h->stream);
+      enqueue_g2p(h, d);
       HIPCHK(h, hipEventRecord(ev[1], h->stream));
       launch_scan(d, h->nkeys, 1, h->stream);
       launch_cellscan(d, grid_grid(h), h->stream);

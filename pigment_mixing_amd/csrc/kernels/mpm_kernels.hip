@@ -23,6 +23,9 @@
 //             boundary_collision.cpp:5-316), write the gather grid, clear accumulators.
 #include "../common/mpmx_device.hpp"
 #include "mpm_kernels.hpp"
+#include "mlp_device.hpp"
+
+#include <algorithm>
 
 namespace mpmx {
 
@@ -229,6 +232,9 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_down(DevPtrs d, int nkeys
       d.stats[ST_FALLBACK_LAST] = d.stats[ST_FALLBACK_CUR];
       d.stats[ST_FALLBACK] += d.stats[ST_FALLBACK_CUR];
       d.stats[ST_FALLBACK_CUR] = 0;
+      d.stats[ST_LEARNED_LAST] = d.stats[ST_LEARNED_CUR];
+      d.stats[ST_LEARNED] += d.stats[ST_LEARNED_CUR];
+      d.stats[ST_LEARNED_CUR] = 0;
     }
   }
 }
@@ -274,17 +280,109 @@ constexpr int GT3 = GT * GT * GT;  // 216 nodes x 16 floats = 13.8 KB
 // Read-only inputs come in as __restrict__ kernel arguments (scene table, node
 // coordinates, particle input side, permutation, grid values), so uniform reads
 // of them become scalar loads that the particle stores cannot clobber.
+//
+// LEARNED instantiates the learned return-map closure: the network (BF16
+// weights, mlp_device.hpp) sits in LDS, each wavefront owns a 64-row activation
+// buffer, and a wavefront with at least one plastic candidate evaluates the
+// network for its 64 lanes on MFMA.  A candidate's predicted state is accepted
+// when its Perzyna consistency residual, its trace change (x K) and its
+// flow-rule mismatch are all within closure_tol * q_y; otherwise the lane runs
+// the exact map (fallback, counted).
+struct G2PLearnedLds {
+  uint8_t net[MLP_BLOB_BYTES];
+  uint16_t act[4][64 * MLP_S];
+};
+
+// Learned return map for the principal Hencky strains h (in/out) of this lane.
+// Called by all 64 lanes of a wavefront together (MFMA).  `active`: the lane
+// holds a particle.  cand_out: its trial state is plastic (q_tr > q_y, not the
+// pressure cap).  Returns true when the predicted state was accepted and
+// applied to h, eps, dg_rate; false leaves them untouched (the caller runs the
+// exact map).
+__device__ bool learned_return_map(const SceneDev& sc, G2PLearnedLds& lrn, float h[3], float& eps, float& dg_rate,
+                                   bool active, bool& cand_out) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const float htr = (h[0] + h[1]) + h[2];
+  const float m = htr / 3.0f;
+  float nv[3] = {h[0] - m, h[1] - m, h[2] - m};
+  const float dn = sqrtf(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+  if (dn > 0.0f) { const float inv = 1.0f / dn; nv[0] *= inv; nv[1] *= inv; nv[2] *= inv; }
+  const float q_trial = sc.e_mu_prefac * dn;
+  const float q_yield = sc.q_min + (sc.q_max - sc.q_min) * expf(-sc.xi * eps);
+  const float p_trial = -sc.K * htr;
+  const bool cand = active && sc.closure != 0 && sc.plastic_model == 5 && !(p_trial < sc.p_min) && q_trial > q_yield;
+  cand_out = cand;
+  if (!__ballot(cand)) return false;
+  // ---- network inputs of all 64 lanes -> this wavefront's activation rows
+  const MlpNet net = mlp_net(lrn.net);
+  uint16_t* act = lrn.act[wave];
+  const float x8[8] = {h[0], h[1], h[2], dn, q_yield, q_yield * sc.perzyna_visc, sc.dt * sc.E, sc.nu};
+#pragma unroll
+  for (int c = 0; c < 16; ++c)
+    act[lane * MLP_S + c] = c < 8 ? f32_to_bf16((x8[c] - net.norm[c]) / net.norm[8 + c]) : (uint16_t)0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  float* out = reinterpret_cast<float*>(act);
+  mlp_wave_forward(net, act, out, MLP_S / 2, nullptr);
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  float y[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) y[c] = out[lane * (MLP_S / 2) + c] * net.norm[20 + c] + net.norm[16 + c];
+  if (!cand) return false;
+  // ---- predicted state
+  float dg = fmaxf(y[3], 0.0f);
+  float hp[3];
+  if (sc.closure == 2) {
+    // one Newton step on the consistency residual from the predicted increment,
+    // strains by the flow rule
+    const float r = perzyna_residual(sc, q_trial - sc.f_mu_prefac * dg, dg, eps);
+    const float rd = perzyna_residual_slope(sc, q_trial, dg, eps);
+    if (rd != 0.0f) dg = fmaxf(dg - r / rd, 0.0f);
+    const float fr = sc.inv_d_prefac * dg;
+    hp[0] = h[0] - fr * nv[0]; hp[1] = h[1] - fr * nv[1]; hp[2] = h[2] - fr * nv[2];
+  } else {
+    hp[0] = h[0] + y[0]; hp[1] = h[1] + y[1]; hp[2] = h[2] + y[2];
+  }
+  // ---- acceptance, each residual within tol * q_y:
+  //   yield consistency of (q(h'), dgamma); trace change x K; and the flow rule
+  //   h' = h - (dgamma / d) n (direction and magnitude) x e_mu
+  const float trp = (hp[0] + hp[1]) + hp[2];
+  const float mp = trp / 3.0f;
+  const float e0 = hp[0] - mp, e1 = hp[1] - mp, e2 = hp[2] - mp;
+  const float qp = sc.e_mu_prefac * sqrtf(e0 * e0 + e1 * e1 + e2 * e2);
+  const float lim = sc.closure_tol * q_yield;
+  const float ry = fabsf(perzyna_residual(sc, qp, dg, eps));
+  const float rv = sc.K * fabsf(trp - htr);
+  const float fac = sc.inv_d_prefac * dg;
+  const float f0 = hp[0] - (h[0] - fac * nv[0]), f1 = hp[1] - (h[1] - fac * nv[1]), f2 = hp[2] - (h[2] - fac * nv[2]);
+  const float rf = sc.e_mu_prefac * sqrtf(f0 * f0 + f1 * f1 + f2 * f2);
+  if (!(ry <= lim && rv <= lim && rf <= lim)) return false;
+  h[0] = hp[0]; h[1] = hp[1]; h[2] = hp[2];
+  eps = eps + sc.inv_d_prefac * dg;
+  dg_rate = dg / sc.dt;
+  return true;
+}
+
+template <bool LEARNED>
 __global__ void __launch_bounds__(256) k_g2p(DevPtrs d, const SceneDev* __restrict__ scenes,
                                              const float* __restrict__ axis, const float* __restrict__ A,
-                                             const int* __restrict__ perm, const float* __restrict__ gv) {
+                                             const int* __restrict__ perm, const float* __restrict__ gv,
+                                             const uint8_t* __restrict__ mlp_blob) {
   __shared__ float4 gtile[GT3 * 4];
   __shared__ float sax[3][GT];       // node coordinates B0 .. B0+5 per axis
-  __shared__ unsigned int plastic_wg;
+  __shared__ unsigned int plastic_wg, fallback_wg, learned_wg, newton_wg;
+  __shared__ __align__(16) uint8_t lrn_raw[LEARNED ? sizeof(G2PLearnedLds) : 16];
   const int step = d.counters[CNT_STEP];
   const int nblk = d.counters[CNT_PBLOCKS];
   float* __restrict__ Bf = d.pb;
   const int64_t cap = d.cap;
-  if (threadIdx.x == 0) plastic_wg = 0u;
+  if (threadIdx.x == 0) { plastic_wg = 0u; fallback_wg = 0u; learned_wg = 0u; newton_wg = 0u; }
+  G2PLearnedLds& lrn = *reinterpret_cast<G2PLearnedLds*>(lrn_raw);
+  if constexpr (LEARNED) {
+    for (int i = threadIdx.x; i < MLP_BLOB_BYTES / 4; i += blockDim.x)
+      reinterpret_cast<uint32_t*>(lrn.net)[i] = reinterpret_cast<const uint32_t*>(mlp_blob)[i];
+  }
   const XcdRange xr = xcd_range(nblk);
   for (int e = xr.e; e < xr.end; e += xr.stride) {
     const int key = __builtin_amdgcn_readfirstlane(d.pb_list[e]);
@@ -325,9 +423,14 @@ __global__ void __launch_bounds__(256) k_g2p(DevPtrs d, const SceneDev* __restri
     for (int it = 0; it < iters; ++it) {
       const int t = it * blockDim.x + threadIdx.x;
       const bool valid = t < cnt;
-      const int64_t j = start + t;
+      // Every lane computes (the learned closure needs whole wavefronts); a lane
+      // past the block's count recomputes the block's last particle and stores
+      // nothing.
+      const int64_t j = start + (valid ? t : cnt - 1);
       int newkey = d.dead_key, newcell = -1, plastic = 0;
-      if (valid) {
+      bool learned_flag = false, fallback_flag = false;
+      int newton = 0;
+      {
         const int src = perm[j];
         float x[3], v[3], F[9], pig[NPIG];
 #pragma unroll
@@ -435,9 +538,18 @@ __global__ void __launch_bounds__(256) k_g2p(DevPtrs d, const SceneDev* __restri
         float eps = eps_in, dgr = 0.0f;
         uint32_t err = 0;
         const float h_trial[3] = {h[0], h[1], h[2]};
-        if (sc.plastic_model == 5) plastic = vmvisc_principal(sc, h, eps, dgr, err);
-        if (err) atomicOr(&d.err[0], err);
-        if (plastic && d.harvest) {
+        bool learned_ok = false;
+        if constexpr (LEARNED) {
+          bool cand = false;
+          learned_ok = learned_return_map(sc, lrn, h, eps, dgr, valid, cand);
+          if (learned_ok) plastic = 1;
+          learned_flag = valid && learned_ok;
+          fallback_flag = valid && cand && !learned_ok;
+        }
+        if (!learned_ok && sc.plastic_model == 5) plastic = vmvisc_principal(sc, h, eps, dgr, err, &newton);
+        if (!valid) newton = 0;
+        if (err && valid) atomicOr(&d.err[0], err);
+        if (plastic && valid && d.harvest) {
           // record: h_trial[3], ||dev h_trial||, q_y, eta = q_y*perzyna_visc, dt*E, nu | h[3], dgamma
           const unsigned long long slot = atomicAdd(&d.stats[ST_HARVEST], 1ull);
           if ((int64_t)slot < d.harvest_cap) {
@@ -504,28 +616,86 @@ __global__ void __launch_bounds__(256) k_g2p(DevPtrs d, const SceneDev* __restri
           }
           if (!alive) { vn[0] = vn[1] = vn[2] = 0.0f; }
         }
-        if (!alive) { newkey = d.dead_key; newcell = -1; }
+        if (!alive || !valid) { newkey = d.dead_key; newcell = -1; }
         // ---------------- write the new state in slot j
+        if (valid) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) { Bf[(P_X + c) * cap + j] = xn[c]; Bf[(P_V + c) * cap + j] = vn[c]; }
+          for (int c = 0; c < 3; ++c) { Bf[(P_X + c) * cap + j] = xn[c]; Bf[(P_V + c) * cap + j] = vn[c]; }
 #pragma unroll
-        for (int c = 0; c < 9; ++c) { Bf[(P_C + c) * cap + j] = Bp[c]; Bf[(P_F + c) * cap + j] = Fn[c]; }
+          for (int c = 0; c < 9; ++c) { Bf[(P_C + c) * cap + j] = Bp[c]; Bf[(P_F + c) * cap + j] = Fn[c]; }
 #pragma unroll
-        for (int c = 0; c < NPIG; ++c) Bf[(P_PIG + c) * cap + j] = pig[c];
-        Bf[P_DIFF * cap + j] = diff_new;
-        Bf[P_EPS * cap + j] = eps;
-        Bf[P_DG * cap + j] = dgr;
+          for (int c = 0; c < NPIG; ++c) Bf[(P_PIG + c) * cap + j] = pig[c];
+          Bf[P_DIFF * cap + j] = diff_new;
+          Bf[P_EPS * cap + j] = eps;
+          Bf[P_DG * cap + j] = dgr;
 #pragma unroll
-        for (int c = 0; c < 6; ++c) Bf[(P_TAU + c) * cap + j] = tau[c];
-        d.pb_id[j] = id;
+          for (int c = 0; c < 6; ++c) Bf[(P_TAU + c) * cap + j] = tau[c];
+          d.pb_id[j] = id;
+        }
       }
       count_for_sort(d, valid, newkey, newcell, j);
       const uint64_t pm = __ballot(valid && plastic);
       if (__lane_id() == 0 && pm) atomicAdd(&plastic_wg, (unsigned)__popcll(pm));
+      if (newton) atomicAdd(&newton_wg, (unsigned)newton);
+      if constexpr (LEARNED) {
+        const uint64_t lm = __ballot(learned_flag);
+        const uint64_t fm = __ballot(fallback_flag);
+        if (__lane_id() == 0) {
+          if (lm) atomicAdd(&learned_wg, (unsigned)__popcll(lm));
+          if (fm) atomicAdd(&fallback_wg, (unsigned)__popcll(fm));
+        }
+      }
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0 && plastic_wg) atomicAdd(&d.stats[ST_PLASTIC_CUR], (unsigned long long)plastic_wg);
+  if (threadIdx.x == 0) {
+    if (plastic_wg) atomicAdd(&d.stats[ST_PLASTIC_CUR], (unsigned long long)plastic_wg);
+    if (newton_wg) atomicAdd(&d.stats[ST_NEWTON], (unsigned long long)newton_wg);
+    if (learned_wg) atomicAdd(&d.stats[ST_LEARNED_CUR], (unsigned long long)learned_wg);
+    if (fallback_wg) atomicAdd(&d.stats[ST_FALLBACK_CUR], (unsigned long long)fallback_wg);
+  }
+}
+
+// Learned and exact return maps side by side on given trial states (principal
+// Hencky strains h1 h2 h3 and eps_pl_dev per row), with scene 0's material: the
+// closure's accuracy measured in principal space, where FP32 rounding of F
+// does not hide it.  learned4 = (h', dgamma) when accepted, else NaN;
+// exact4 = (h', dgamma) of the exact map.
+__global__ void __launch_bounds__(256) k_closure_eval(const SceneDev* __restrict__ scenes,
+                                                      const uint8_t* __restrict__ mlp_blob,
+                                                      const float* __restrict__ trial4, int64_t n,
+                                                      float* __restrict__ learned4, float* __restrict__ exact4,
+                                                      uint8_t* __restrict__ accepted) {
+  __shared__ __align__(16) G2PLearnedLds lrn;
+  for (int i = threadIdx.x; i < MLP_BLOB_BYTES / 4; i += blockDim.x)
+    reinterpret_cast<uint32_t*>(lrn.net)[i] = reinterpret_cast<const uint32_t*>(mlp_blob)[i];
+  __syncthreads();
+  const SceneDev& sc = scenes[0];
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = base + threadIdx.x;
+    const bool valid = i < n;
+    const int64_t r = valid ? i : n - 1;
+    float h[3] = {trial4[4 * r], trial4[4 * r + 1], trial4[4 * r + 2]};
+    float eps = trial4[4 * r + 3], dgr = 0.0f;
+    bool cand = false;
+    const bool ok = learned_return_map(sc, lrn, h, eps, dgr, valid, cand);
+    float he[3] = {trial4[4 * r], trial4[4 * r + 1], trial4[4 * r + 2]};
+    float epse = trial4[4 * r + 3], dge = 0.0f;
+    uint32_t err = 0;
+    const int pl = sc.plastic_model == 5 ? vmvisc_principal(sc, he, epse, dge, err) : 0;
+    if (valid) {
+      const float nan = __uint_as_float(0x7FC00000u);
+      learned4[4 * i] = ok ? h[0] : nan;
+      learned4[4 * i + 1] = ok ? h[1] : nan;
+      learned4[4 * i + 2] = ok ? h[2] : nan;
+      learned4[4 * i + 3] = ok ? dgr * sc.dt : nan;
+      exact4[4 * i] = he[0];
+      exact4[4 * i + 1] = he[1];
+      exact4[4 * i + 2] = he[2];
+      exact4[4 * i + 3] = pl ? dge * sc.dt : 0.0f;
+      accepted[i] = ok ? 1 : 0;
+    }
+  }
 }
 
 // Dead particles keep their state: copy them to the output side and rank them.
@@ -959,10 +1129,23 @@ void launch_cellscan(const DevPtrs& d, int grid, hipStream_t st) {
 void launch_perm(const DevPtrs& d, int64_t n, hipStream_t st) {
   if (n > 0) hipLaunchKernelGGL(k_perm, dim3(blocks_for(n, 256)), dim3(256), 0, st, d, n);
 }
-void launch_g2p(const DevPtrs& d, int grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_g2p, dim3(grid), dim3(256), 0, st, d, (const SceneDev*)d.scenes, (const float*)d.axis_coords,
-                     (const float*)d.pa, (const int*)d.perm, (const float*)d.gv);
+void launch_g2p(const DevPtrs& d, int grid, const void* mlp_blob, hipStream_t st) {
+  if (mlp_blob)
+    hipLaunchKernelGGL(k_g2p<true>, dim3(grid), dim3(256), 0, st, d, (const SceneDev*)d.scenes,
+                       (const float*)d.axis_coords, (const float*)d.pa, (const int*)d.perm, (const float*)d.gv,
+                       (const uint8_t*)mlp_blob);
+  else
+    hipLaunchKernelGGL(k_g2p<false>, dim3(grid), dim3(256), 0, st, d, (const SceneDev*)d.scenes,
+                       (const float*)d.axis_coords, (const float*)d.pa, (const int*)d.perm, (const float*)d.gv,
+                       (const uint8_t*)nullptr);
   hipLaunchKernelGGL(k_copy_dead, dim3(64), dim3(256), 0, st, d);
+}
+void launch_closure_eval(const SceneDev* scenes, const void* mlp_blob, const float* trial4, int64_t n,
+                         float* learned4, float* exact4, uint8_t* accepted, int max_grid, hipStream_t st) {
+  if (n <= 0) return;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, max_grid);
+  hipLaunchKernelGGL(k_closure_eval, dim3((unsigned)blocks), dim3(256), 0, st, scenes, (const uint8_t*)mlp_blob,
+                     trial4, n, learned4, exact4, accepted);
 }
 void launch_p2g(const DevPtrs& d, int grid, int src_is_b, hipStream_t st) {
   hipLaunchKernelGGL(k_p2g, dim3(grid), dim3(64), 0, st, d, (const SceneDev*)d.scenes, (const float*)d.axis_coords,

@@ -27,7 +27,11 @@ enum {
 // uint64 cumulative statistics
 enum {
   ST_PLASTIC = 0, ST_ACTIVE_NODES = 1, ST_STRAY = 2, ST_FALLBACK = 3, ST_NBLOCKS = 4, ST_HARVEST = 5,
-  ST_PLASTIC_LAST = 6, ST_FALLBACK_LAST = 7, ST_PLASTIC_CUR = 8, ST_FALLBACK_CUR = 9, ST_N = 12
+  ST_PLASTIC_LAST = 6, ST_FALLBACK_LAST = 7, ST_PLASTIC_CUR = 8, ST_FALLBACK_CUR = 9,
+  ST_NEWTON = 10,       // Newton iterations of the exact map (cumulative)
+  ST_LEARNED = 11,      // learned-closure predictions accepted (cumulative)
+  ST_LEARNED_LAST = 12, ST_LEARNED_CUR = 13,
+  ST_N = 16
 };
 
 struct DevPtrs {
@@ -77,7 +81,9 @@ void launch_scan(const DevPtrs& d, int nkeys, int advance, hipStream_t st);
 int scan_chunks(int nkeys);
 void launch_cellscan(const DevPtrs& d, int grid, hipStream_t st);
 void launch_perm(const DevPtrs& d, int64_t n, hipStream_t st);
-void launch_g2p(const DevPtrs& d, int grid, hipStream_t st);
+void launch_g2p(const DevPtrs& d, int grid, const void* mlp_blob, hipStream_t st);
+void launch_closure_eval(const SceneDev* scenes, const void* mlp_blob, const float* trial4, int64_t n,
+                         float* learned4, float* exact4, uint8_t* accepted, int max_grid, hipStream_t st);
 void launch_p2g(const DevPtrs& d, int grid, int src_is_b, hipStream_t st);
 void launch_grid(const DevPtrs& d, int grid, hipStream_t st);
 void launch_unsort(const DevPtrs& d, int64_t n, float* out, int first, int ncomp, hipStream_t st);

@@ -1,0 +1,240 @@
+"""GPU checks of the learned return-map closure (device kernels through the
+C-ABI) against the CPU oracle of the network (oracle/mlp_oracle.py).
+
+Tolerances: the device and the oracle round to BF16 at the same points and
+accumulate in FP32 in different orders, so a rounding decision can flip by one
+BF16 step (2^-8 relative) on a few activations.
+  forward : |y_gpu - y_oracle| <= 2e-2 * std(y_oracle) per output (normalised units)
+  gradient: ||g_gpu - g_oracle|| <= 2e-2 ||g_oracle||, loss within 1e-3 relative
+A wrong MFMA operand or accumulator layout gives O(1) errors."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from pigment_mixing_amd import abi, scenes
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import mlp_oracle as mo  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+def _norm24(rng):
+    return np.concatenate([rng.normal(0, 0.5, 8), rng.uniform(0.5, 2.0, 8),
+                           rng.normal(0, 0.5, 4), rng.uniform(0.5, 2.0, 4)]).astype(np.float32)
+
+
+@pytest.mark.parametrize("n", [1, 64, 1000])
+def test_mlp_forward_matches_oracle(n):
+    rng = np.random.default_rng(11 + n)
+    p = mo.init_params(7)
+    n24 = _norm24(rng)
+    x = (rng.normal(0, 1, (n, 8)) * n24[8:16] + n24[0:8]).astype(np.float32)
+    log(f"[closure] forward n={n}")
+    y = abi.mlp_forward(p, n24, x)
+    yo, _ = mo.forward(p, n24, x, True)
+    scale = n24[20:24] * max(1e-3, float(np.std((yo - n24[16:20]) / n24[20:24]))) if n > 1 else n24[20:24]
+    err = np.abs(y - yo) / scale
+    log(f"[closure] forward n={n}: max err/std {err.max():.2e}")
+    assert err.max() <= 2e-2
+
+
+def test_mlp_gradient_matches_oracle():
+    rng = np.random.default_rng(21)
+    p = mo.init_params(8)
+    n24 = _norm24(rng)
+    n = 777      # not a multiple of 64: exercises the masked tail
+    x = (rng.normal(0, 1, (n, 8)) * n24[8:16] + n24[0:8]).astype(np.float32)
+    t = (rng.normal(0, 1, (n, 4)) * n24[20:24] + n24[16:20]).astype(np.float32)
+    log("[closure] gradient n=777")
+    loss, g = abi.mlp_gradient(p, n24, x, t)
+    lo, go = mo.loss_and_grad(p, n24, x, t, True)
+    rel = np.linalg.norm(g - go) / np.linalg.norm(go)
+    log(f"[closure] loss gpu={loss:.6e} oracle={lo:.6e}; grad rel err {rel:.2e}")
+    assert abs(loss - lo) <= 1e-3 * lo
+    assert rel <= 2e-2
+    # every layer's weight and bias block carries gradient
+    k = 0
+    for i in range(5):
+        nin, nout = mo.DIMS[i], mo.DIMS[i + 1]
+        for blk in (g[k:k + nout * nin], g[k + nout * nin:k + nout * nin + nout]):
+            assert np.abs(blk).max() > 0
+        k += nout * nin + nout
+
+
+def plastic_scene():
+    s = scenes.paint_2blob(cells=(6, 4, 6), dims=(24, 16, 24))
+    s["cfg"].q_max = s["cfg"].q_min = 5.0
+    s["cfg"].perzyna_visc = 10.0 / 5.0
+    s["knife"].p0[0] = 6 * scenes.DX - 0.010
+    s["knife"].half[0] = 0.010
+    s["knife"].half[2] = 0.006
+    return s
+
+
+def test_training_on_harvested_pairs_reduces_loss():
+    s = plastic_scene()
+    h = scenes.make_handle(s)
+    h.harvest_begin(2_000_000)
+    h.step(150)
+    rec = h.harvest_end(2_000_000)
+    log(f"[closure] harvested {rec.shape[0]} records")
+    assert rec.shape[0] > 10000
+    # harvested pairs obey the exact map's invariants: trace kept, q_y positive
+    tr_in = rec[:, 0:3].sum(1)
+    tr_out = rec[:, 8:11].sum(1)
+    assert np.abs(tr_in - tr_out).max() <= 1e-6 + 1e-4 * np.abs(tr_in).max()
+    assert (rec[:, 4] > 0).all() and (rec[:, 11] >= 0).all()
+    rep, curve = h.closure_train(rec, steps=1500, batch=4096, lr=2e-3, lr_final=1e-4, seed=5, report_every=100)
+    log(f"[closure] train: first {rep.loss_first:.3e} last {rep.loss_last:.3e} full {rep.loss_full:.3e} "
+        f"({rep.ms:.0f} ms, {rep.steps} steps)")
+    assert rep.loss_full < 0.02 * rep.loss_first
+    assert rep.loss_full < 1e-2
+    # the installed network reproduces the training targets through the forward kernel
+    p = h.closure_params()
+    norm = np.array(rep.norm, np.float32)
+    y = abi.mlp_forward(p, norm, rec[:4096, :8])
+    dh = rec[:4096, 8:11] - rec[:4096, 0:3]
+    scale = np.abs(dh).max()
+    err = np.abs(y[:, :3] - dh).max() / scale
+    log(f"[closure] max |dh_pred - dh| / max|dh| = {err:.3e}")
+    assert err < 0.1
+
+
+# ------------------------------------------------------------------ closure in the step
+MU = 1e5 / (2 * 1.3)
+LAM = 0.3 * 1e5 / (1.3 * 0.4)
+
+
+def kirchhoff(F):
+    """tau = U diag(2 mu ln s + lam sum ln s) U^T from F (numpy SVD, float64)."""
+    U, sv, _ = np.linalg.svd(F.astype(np.float64))
+    h = np.log(np.maximum(sv, 1e-4))
+    d = 2 * MU * h + LAM * h.sum(1, keepdims=True)
+    return np.einsum("nij,nj,nkj->nik", U, d, U)
+
+
+def trained_closure(scene, steps=150, train_steps=2000, seed=5):
+    h = scenes.make_handle(scene)
+    h.harvest_begin(4_000_000)
+    h.step(steps)
+    rec = h.harvest_end(4_000_000)
+    rep, _ = h.closure_train(rec, steps=train_steps, batch=4096, lr=2e-3, lr_final=5e-5, seed=seed,
+                             report_every=500)
+    log(f"[closure] {rec.shape[0]} records, loss {rep.loss_first:.3e} -> {rep.loss_full:.3e} in {rep.ms:.0f} ms")
+    state = h.download()
+    return h.closure_params(), np.array(rep.norm, np.float32), state, rec
+
+
+def handle_from_state(scene, st, closure=0, tol=1e-4, params=None, norm=None):
+    cfg = scene["cfg"]
+    cfg.closure = closure
+    cfg.closure_tol = tol
+    h = abi.Handle(cfg)
+    h.upload(st["x"], v=st["v"], F=st["F"], C=st["C"], pig=st["pig"], diff=st["diff"], eps=st["eps"],
+             active=st["active"])
+    h.set_objects(scene["plates"], scene["knife"])
+    h.prepare()
+    if closure:
+        h.closure_set(params, norm)
+    cfg.closure = 0
+    return h
+
+
+def principal_tau(h):
+    """Principal Kirchhoff stress 2 mu h + lam tr h (material of plastic_scene)."""
+    return 2 * MU * h + LAM * h.sum(1, keepdims=True)
+
+
+@pytest.mark.parametrize("mode,tol", [(1, 1e-4), (1, 1e-2), (2, 1e-4)])
+def test_learned_closure_accuracy_in_principal_space(mode, tol):
+    """Held-out harvested trial states through the device closure (the same
+    code the step runs) and the exact map: accepted predictions' principal
+    Kirchhoff stress within 1e-3 relative of the exact map (the BASELINE bar).
+    The raw BF16 network (mode 1) meets a 1e-4 yield-residual tolerance for
+    almost no particle (its outputs are accurate to ~1e-2); the refined mode
+    (one Newton step from the predicted increment) accepts nearly all."""
+    s = plastic_scene()
+    params, norm, st, rec = trained_closure(s)
+    hold = rec[-20000:]
+    trial = np.concatenate([hold[:, 0:3], np.zeros((hold.shape[0], 1), np.float32)], axis=1)
+    h = handle_from_state(s, st, closure=mode, tol=tol, params=params, norm=norm)
+    lrn, ex, acc = h.closure_eval(trial)
+    # the exact map on the device reproduces the harvested targets
+    np.testing.assert_allclose(ex[:, 0:3], hold[:, 8:11], rtol=0, atol=1e-7)
+    te = principal_tau(ex[:, 0:3].astype(np.float64))
+    rate = acc.mean()
+    if acc.any():
+        tl = principal_tau(lrn[acc, 0:3].astype(np.float64))
+        rel = np.linalg.norm(tl - te[acc], axis=1) / np.linalg.norm(te[acc], axis=1)
+        log(f"[closure] mode {mode} tol {tol:g}: accepted {rate:.4f}; stress rel err max {rel.max():.2e} "
+            f"median {np.median(rel):.2e}")
+        assert rel.max() <= 1e-3
+    else:
+        log(f"[closure] mode {mode} tol {tol:g}: accepted 0")
+    # raw network: accuracy of all predictions (accepted or not) as a reference figure
+    y = abi.mlp_forward(params, norm, hold[:, :8])
+    raw = principal_tau((hold[:, 0:3] + y[:, 0:3]).astype(np.float64))
+    rel_raw = np.linalg.norm(raw - te, axis=1) / np.linalg.norm(te, axis=1)
+    log(f"[closure] raw network stress rel err: median {np.median(rel_raw):.2e} p99 {np.percentile(rel_raw, 99):.2e}")
+    if mode == 2:
+        assert rate >= 0.9
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_learned_closure_in_step(mode):
+    """Exact and learned handles from the same plastic state for 100 steps:
+    every plastic particle is either accepted or sent to the exact map, and the
+    trajectories stay within the plastic-scene parity bars."""
+    s = plastic_scene()
+    params, norm, st, _ = trained_closure(s)
+    he = handle_from_state(s, st)
+    hl = handle_from_state(s, st, closure=mode, params=params, norm=norm)
+    he.step(1)
+    hl.step(1)
+    sl = hl.stats()
+    log(f"[closure] mode {mode} step 1: plastic {sl.plastic_count}, accepted {sl.learned_count}, "
+        f"fallback {sl.fallback_count}")
+    assert sl.plastic_count > 100
+    assert sl.learned_count + sl.fallback_count == sl.plastic_count
+    if mode == 2:
+        assert sl.learned_count >= 0.9 * sl.plastic_count
+    he.step(99)
+    hl.step(99)
+    a, b = he.download(), hl.download()
+    act = a["active"].astype(bool)
+    np.testing.assert_array_equal(a["active"], b["active"])
+    ex = np.linalg.norm(a["x"] - b["x"], axis=1)[act].max() / scenes.DX
+    eF = (np.linalg.norm((a["F"] - b["F"]).reshape(-1, 9), axis=1) / np.linalg.norm(a["F"].reshape(-1, 9), axis=1))[act].max()
+    log(f"[closure] mode {mode}: after 100 steps max|dx|/dx {ex:.2e} max rel dF {eF:.2e}; "
+        f"last call accepted {hl.stats().learned_count} fallback {hl.stats().fallback_count}")
+    assert ex <= 1e-3 and eF <= 1e-3
+
+
+def test_learned_closure_gain_on_newton_material():
+    """With a rate exponent of 2 the exact map iterates Newton (plasticity.cpp:196-229).
+    The refined closure starts from the network's increment and accepts after one
+    step: Newton iterations per step drop, with the stress bar kept."""
+    s = plastic_scene()
+    s["cfg"].perzyna_exp = 2.0
+    params, norm, st, _ = trained_closure(s)
+    he = handle_from_state(s, st)
+    hl = handle_from_state(s, st, closure=2, params=params, norm=norm)
+    he.step(20)
+    hl.step(20)
+    se, sl = he.stats(), hl.stats()
+    log(f"[closure] newton iterations over 20 steps: exact {se.newton_iters}, learned {sl.newton_iters}; "
+        f"accepted {sl.learned_count}, fallback {sl.fallback_count}")
+    assert se.newton_iters > 0
+    assert sl.newton_iters <= 0.5 * se.newton_iters
+    a, b = he.download(), hl.download()
+    act = a["active"].astype(bool)
+    eF = (np.linalg.norm((a["F"] - b["F"]).reshape(-1, 9), axis=1) / np.linalg.norm(a["F"].reshape(-1, 9), axis=1))[act].max()
+    log(f"[closure] newton material: max rel dF after 20 steps {eF:.2e}")
+    assert eF <= 1e-3
