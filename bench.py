@@ -125,6 +125,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--harvest-steps", type=int, default=40)
     ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched run")
+    ap.add_argument("--batch-steps", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (multi-GPU sharding is design-only); use --gpus 1",
@@ -189,6 +191,26 @@ def main():
           file=sys.stderr, flush=True)
     st_last = hl.stats()
 
+    # ---- 64 independent scenes in one handle (one graph per step pair), autotuned
+    batch = None
+    if not args.no_batch:
+        sl = scenes.batch_scenes(64)
+        nb = sum(sc["x"].shape[0] for sc in sl)
+        hb = scenes.make_batch_handle(sl)
+        tune = hb.autotune(4)
+        wall_b, st_b = timed_run(hb, args.batch_steps, 5)
+        kt_b = hb.profile(10)
+        hb.close()
+        batch = {"workload": "batch_64_scenes", "scenes": 64, "particles": nb,
+                 "value": st_b.n_active * args.batch_steps / wall_b, "unit": "particle-steps/s",
+                 "steps": args.batch_steps, "ms_per_step": wall_b / args.batch_steps * 1e3,
+                 "autotune": {"wg_per_cu": {"p2g": tune.wg_p2g, "g2p": tune.wg_g2p, "grid": tune.wg_grid},
+                              "trials": tune.trials, "tuning_steps": int(tune.steps)},
+                 "roofline": kernel_report(kt_b)}
+        print(f"[bench] batch: {nb} particles in 64 scenes, {batch['value']:.4g} particle-steps/s "
+              f"({batch['ms_per_step']:.3f} ms/step); tuned wg/CU p2g={tune.wg_p2g} g2p={tune.wg_g2p} "
+              f"grid={tune.wg_grid}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(scene, steps=args.cpu_steps)
@@ -231,6 +253,7 @@ def main():
                          "ms": rep.ms, "loss_first": rep.loss_first, "loss_full": rep.loss_full},
         },
         "roofline": rf,
+        "batch": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -327,6 +327,35 @@ typedef struct {
 int mpmx_scene_boxes(const mpmx_box* boxes, int n_boxes, float dx, int per_axis, uint64_t seed,
                      float* x, float* pig, int64_t capacity, int64_t* n_out);
 
+/* ---------------- batched scenes ---------------- */
+
+/* A handle advancing n_scenes (<= 64) independent scenes together: every
+ * kernel of a step walks the blocks of all scenes, and one HIP graph holds
+ * two steps of the whole batch.  Scenes differ in material, lattice, objects
+ * and particles; they share dt and the closure mode.  Particles are staged per
+ * scene (mpmx_upload_scene) and laid out in scene order, so downloads return
+ * scene 0's particles first; mpmx_scene_range gives each scene's slice.
+ * Knife index k of the knife calls is the scene index.  No counterpart in the
+ * reference (one Simulation per process there). */
+int mpmx_create_batch(const mpmx_config* cfgs, int n_scenes, mpmx_handle** out);
+int mpmx_upload_scene(mpmx_handle* h, int scene, int64_t n, const float* x, const float* v, const float* F,
+                      const float* C, const float* pig, const float* diff, const float* eps,
+                      const uint8_t* active);
+int mpmx_set_objects_scene(mpmx_handle* h, int scene, const mpmx_plate* plates, int n_plates,
+                           const mpmx_knife* knives, int n_knives);
+int mpmx_scene_range(mpmx_handle* h, int scene, int64_t* first, int64_t* count);
+
+/* Launch-shape autotuner: times candidate workgroups-per-CU for P2G, G2P and
+ * the grid update (coordinate descent, steps_per_trial ordinary steps each)
+ * and keeps the fastest.  The steps it runs advance the simulation. */
+typedef struct {
+  int wg_p2g, wg_g2p, wg_grid;   /* chosen workgroups per CU                  */
+  float ms_p2g, ms_g2p, ms_grid; /* mean kernel ms at the chosen shapes       */
+  int trials;
+  int64_t steps;                 /* simulation steps spent tuning             */
+} mpmx_tune_report;
+int mpmx_autotune(mpmx_handle* h, int steps_per_trial, mpmx_tune_report* report);
+
 /* ---------------- learned return-map closure ---------------- */
 
 /* MLP 8 -> 64 -> 64 -> 64 -> 64 -> 4, SiLU, BF16 weights, FP32 accumulate.

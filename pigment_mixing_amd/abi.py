@@ -102,6 +102,12 @@ class Box(ctypes.Structure):
     _fields_ = [("lo", ctypes.c_float * 3), ("cells", ctypes.c_int * 3), ("latent", ctypes.c_float * 7)]
 
 
+class TuneReport(ctypes.Structure):
+    _fields_ = [("wg_p2g", ctypes.c_int), ("wg_g2p", ctypes.c_int), ("wg_grid", ctypes.c_int),
+                ("ms_p2g", ctypes.c_float), ("ms_g2p", ctypes.c_float), ("ms_grid", ctypes.c_float),
+                ("trials", ctypes.c_int), ("steps", ctypes.c_int64)]
+
+
 class TrainConfig(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int64), ("batch", ctypes.c_int), ("lr", ctypes.c_float),
                 ("lr_final", ctypes.c_float), ("beta1", ctypes.c_float), ("beta2", ctypes.c_float),
@@ -150,6 +156,14 @@ def _sig(lib):
         ("mpmx_scene_boxes", ctypes.c_int,
          [ctypes.POINTER(Box), ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_uint64, c_float_p, c_float_p,
           ctypes.c_int64, c_i64_p]),
+        ("mpmx_create_batch", ctypes.c_int, [ctypes.POINTER(Config), ctypes.c_int, ctypes.POINTER(H)]),
+        ("mpmx_upload_scene", ctypes.c_int,
+         [H, ctypes.c_int, ctypes.c_int64, c_float_p, c_float_p, c_float_p, c_float_p, c_float_p, c_float_p,
+          c_float_p, c_u8_p]),
+        ("mpmx_set_objects_scene", ctypes.c_int,
+         [H, ctypes.c_int, ctypes.POINTER(Plate), ctypes.c_int, ctypes.POINTER(Knife), ctypes.c_int]),
+        ("mpmx_scene_range", ctypes.c_int, [H, ctypes.c_int, c_i64_p, c_i64_p]),
+        ("mpmx_autotune", ctypes.c_int, [H, ctypes.c_int, ctypes.POINTER(TuneReport)]),
         ("mpmx_closure_n_params", ctypes.c_int64, []),
         ("mpmx_closure_set_params", ctypes.c_int, [H, c_float_p, ctypes.c_int64]),
         ("mpmx_closure_get_params", ctypes.c_int, [H, c_float_p, ctypes.c_int64]),
@@ -256,12 +270,20 @@ class MpmxError(RuntimeError):
 
 
 class Handle:
-    """One simulation on the current HIP device (thin wrapper over the C-ABI)."""
+    """One simulation (or a batch of scenes) on the current HIP device (thin
+    wrapper over the C-ABI).  Pass a list of configs for a batched handle."""
 
     def __init__(self, cfg):
         self.lib = load()
         self.h = ctypes.c_void_p()
-        self._chk(self.lib.mpmx_create(ctypes.byref(cfg), ctypes.byref(self.h)), "create")
+        if isinstance(cfg, (list, tuple)):
+            arr = (Config * len(cfg))(*cfg)
+            self._chk(self.lib.mpmx_create_batch(arr, len(cfg), ctypes.byref(self.h)), "create_batch")
+            self.n_scenes = len(cfg)
+            self._scene_n = [0] * len(cfg)
+        else:
+            self._chk(self.lib.mpmx_create(ctypes.byref(cfg), ctypes.byref(self.h)), "create")
+            self.n_scenes = 1
         self.cfg = cfg
         self.n = 0
 
@@ -290,6 +312,33 @@ class Handle:
         self._chk(self.lib.mpmx_upload_particles(self.h, self.n, fptr(x), fptr(v), fptr(F), fptr(C), fptr(pig),
                                                  fptr(diff), fptr(eps),
                                                  None if act is None else act.ctypes.data_as(c_u8_p)), "upload")
+
+    def upload_scene(self, scene, x, v=None, F=None, C=None, pig=None, diff=None, eps=None, active=None):
+        x = np.ascontiguousarray(x, np.float32)
+        conv = lambda a: None if a is None else np.ascontiguousarray(a, np.float32)
+        v, F, C, pig, diff, eps = map(conv, (v, F, C, pig, diff, eps))
+        act = None if active is None else np.ascontiguousarray(active, np.uint8)
+        self._chk(self.lib.mpmx_upload_scene(self.h, scene, x.shape[0], fptr(x), fptr(v), fptr(F), fptr(C),
+                                             fptr(pig), fptr(diff), fptr(eps),
+                                             None if act is None else act.ctypes.data_as(c_u8_p)), "upload_scene")
+        self._scene_n[scene] = x.shape[0]
+        self.n = sum(self._scene_n)
+
+    def set_objects_scene(self, scene, plates=(), knife=None):
+        pl = (Plate * max(1, len(plates)))(*plates) if plates else None
+        kn = ctypes.byref(knife) if knife is not None else None
+        self._chk(self.lib.mpmx_set_objects_scene(self.h, scene, pl, len(plates), kn, 1 if knife is not None else 0),
+                  "set_objects_scene")
+
+    def scene_range(self, scene):
+        a, b = ctypes.c_int64(0), ctypes.c_int64(0)
+        self._chk(self.lib.mpmx_scene_range(self.h, scene, ctypes.byref(a), ctypes.byref(b)), "scene_range")
+        return a.value, b.value
+
+    def autotune(self, steps_per_trial=4):
+        r = TuneReport()
+        self._chk(self.lib.mpmx_autotune(self.h, steps_per_trial, ctypes.byref(r)), "autotune")
+        return r
 
     def set_objects(self, plates=(), knife=None):
         P = (Plate * max(1, len(plates)))(*plates) if plates else None

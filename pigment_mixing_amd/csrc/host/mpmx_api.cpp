@@ -21,6 +21,7 @@ using namespace mpmx;
 namespace {
 
 constexpr int MAX_CHUNK = 1024;  // knife states uploaded per chunk of steps
+constexpr int MAX_SCENES = 64;   // scenes per batched handle
 
 struct Scene {
   mpmx_config cfg;
@@ -32,6 +33,8 @@ struct Scene {
   bool has_override = false;
   mpmx_knife_state override_state;
   int64_t p_begin = 0, p_count = 0;  // particle range in upload order
+  std::vector<float> comp;           // batch staging: P_NCOMP x p_count, component-major
+  std::vector<uint8_t> act;
   int dims[3] = {0, 0, 0};
   float origin[3] = {0, 0, 0};
   std::vector<float> xs, ys, zs;
@@ -87,6 +90,10 @@ struct mpmx_handle {
   float* d_harvest = nullptr;
   int64_t harvest_cap = 0;
   unsigned long long harvest_base = 0;
+  // batched handle (mpmx_create_batch): particles staged per scene
+  bool batch = false;
+  // launch shapes (workgroups per CU), set by mpmx_autotune
+  int wg_g2p = 4, wg_p2g = 16, wg_grid = 4;
 };
 
 namespace {
@@ -289,16 +296,17 @@ DevPtrs swapped(const DevPtrs& d) {
 
 // Persistent grids are a multiple of 8 workgroups: the block walkers split
 // their lists into one contiguous segment per XCD (8 on gfx950).
-int grid_fused(mpmx_handle* h) { return std::max(8, (h->n_cu * 4 + 7) / 8 * 8); }
-int grid_grid(mpmx_handle* h) { return std::max(8, (h->n_cu * 4 + 7) / 8 * 8); }
-// P2G runs one-wavefront workgroups: 16 per CU
-int grid_p2g(mpmx_handle* h) { return std::max(8, (h->n_cu * 16 + 7) / 8 * 8); }
+int round8(int v) { return std::max(8, (v + 7) / 8 * 8); }
+int grid_fused(mpmx_handle* h) { return round8(h->n_cu * h->wg_g2p); }
+int grid_grid(mpmx_handle* h) { return round8(h->n_cu * h->wg_grid); }
+// P2G runs one-wavefront workgroups
+int grid_p2g(mpmx_handle* h) { return round8(h->n_cu * h->wg_p2g); }
 
 // G2P with the exact return map, or with the learned closure (network in LDS:
 // two 256-thread workgroups per CU).
 bool learned(const mpmx_handle* h) { return h->scenes[0].cfg.closure != MPMX_CLOSURE_EXACT; }
 void enqueue_g2p(mpmx_handle* h, const DevPtrs& d) {
-  if (learned(h)) launch_g2p(d, std::max(8, (h->n_cu * 2 + 7) / 8 * 8), h->closure.d_blob, h->stream);
+  if (learned(h)) launch_g2p(d, round8(h->n_cu * std::min(h->wg_g2p, 2)), h->closure.d_blob, h->stream);
   else launch_g2p(d, grid_fused(h), nullptr, h->stream);
 }
 
@@ -445,18 +453,23 @@ int mpmx_config_load_json(mpmx_config* c, const char* sim_path, const char* pig_
   return rc;
 }
 
+namespace {
+int check_config(const mpmx_config* cfg);
+}
+
 int mpmx_create(const mpmx_config* cfg, mpmx_handle** out) {
-  if (!cfg || !out) return MPMX_ERR_ARG;
+  return mpmx_create_batch(cfg, 1, out);
+}
+
+int mpmx_create_batch(const mpmx_config* cfgs, int n_scenes, mpmx_handle** out) {
+  if (!cfgs || !out || n_scenes < 1 || n_scenes > MAX_SCENES) return MPMX_ERR_ARG;
   *out = nullptr;
-  if (!(cfg->dx > 0) || !(cfg->dt > 0) || !(cfg->ppc > 0) || !(cfg->rho > 0)) return MPMX_ERR_ARG;
-  if (cfg->plastic_model != MPMX_PLASTIC_NONE && cfg->plastic_model != MPMX_PLASTIC_VMVISC) return MPMX_ERR_ARG;
-  // MUSL's second scatter (musl.cpp:6-98) has no device kernel: refuse the
-  // flag rather than run a USL step under a MUSL configuration.
-  if (cfg->use_musl != 0) return MPMX_ERR_ARG;
-  if (cfg->closure != MPMX_CLOSURE_EXACT && cfg->closure != MPMX_CLOSURE_LEARNED &&
-      cfg->closure != MPMX_CLOSURE_LEARNED_REFINED)
-    return MPMX_ERR_ARG;
-  if (cfg->closure != MPMX_CLOSURE_EXACT && !(cfg->closure_tol > 0)) return MPMX_ERR_ARG;
+  for (int i = 0; i < n_scenes; ++i) {
+    if (check_config(&cfgs[i])) return MPMX_ERR_ARG;
+    // one clock and one G2P variant for the whole batch
+    if (cfgs[i].dt != cfgs[0].dt || cfgs[i].closure != cfgs[0].closure) return MPMX_ERR_ARG;
+  }
+  const mpmx_config* cfg = cfgs;
   mpmx_handle* h = new mpmx_handle();
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) {
@@ -470,8 +483,9 @@ int mpmx_create(const mpmx_config* cfg, mpmx_handle** out) {
     delete h;
     return MPMX_ERR_HIP;
   }
-  h->scenes.resize(1);
-  h->scenes[0].cfg = *cfg;
+  h->scenes.resize(n_scenes);
+  for (int i = 0; i < n_scenes; ++i) h->scenes[i].cfg = cfg[i];
+  h->batch = n_scenes > 1;
   *out = h;
   return MPMX_OK;
 }
@@ -491,29 +505,76 @@ void mpmx_destroy(mpmx_handle* h) {
 
 const char* mpmx_last_error(const mpmx_handle* h) { return h ? h->err.c_str() : "null handle"; }
 
+namespace {
+// Host SoA (per particle) -> component-major staging [P_NCOMP][n]
+void stage(std::vector<float>& comp, std::vector<uint8_t>& actv, int64_t n, const float* x, const float* v,
+           const float* F, const float* C, const float* pig, const float* diff, const float* eps,
+           const uint8_t* active) {
+  comp.assign((size_t)P_NCOMP * n, 0.0f);
+  actv.assign(n, 1);
+  for (int64_t p = 0; p < n; ++p) {
+    for (int c = 0; c < 3; ++c) {
+      comp[(P_X + c) * n + p] = x[3 * p + c];
+      comp[(P_V + c) * n + p] = v ? v[3 * p + c] : 0.0f;
+    }
+    for (int c = 0; c < 9; ++c) {
+      comp[(P_C + c) * n + p] = C ? C[9 * p + c] : 0.0f;
+      comp[(P_F + c) * n + p] = F ? F[9 * p + c] : ((c == 0 || c == 4 || c == 8) ? 1.0f : 0.0f);
+    }
+    for (int c = 0; c < NPIG; ++c) comp[(P_PIG + c) * n + p] = pig ? pig[7 * p + c] : 0.0f;
+    comp[P_DIFF * n + p] = diff ? diff[p] : 0.0f;
+    comp[P_EPS * n + p] = eps ? eps[p] : 0.0f;
+    if (active) actv[p] = active[p] ? 1 : 0;
+  }
+}
+
+// Batched handle: concatenate the scenes' staging in scene order.
+int gather_batch(mpmx_handle* h) {
+  int64_t n = 0;
+  for (Scene& s : h->scenes) {
+    if (s.p_count <= 0) return fail(h, MPMX_ERR_STATE, "prepare: every scene of a batch needs particles");
+    s.p_begin = n;
+    n += s.p_count;
+  }
+  if (n > (int64_t)0x7fffffff / 2) return fail(h, MPMX_ERR_CAPACITY, "prepare: too many particles");
+  h->np = n;
+  h->h_comp.assign((size_t)P_NCOMP * n, 0.0f);
+  h->h_active.resize(n);
+  h->h_pscene.resize(n);
+  for (int si = 0; si < (int)h->scenes.size(); ++si) {
+    const Scene& s = h->scenes[si];
+    for (int c = 0; c < P_NCOMP; ++c)
+      std::copy(s.comp.begin() + (size_t)c * s.p_count, s.comp.begin() + (size_t)(c + 1) * s.p_count,
+                h->h_comp.begin() + (size_t)c * n + s.p_begin);
+    std::copy(s.act.begin(), s.act.end(), h->h_active.begin() + s.p_begin);
+    std::fill(h->h_pscene.begin() + s.p_begin, h->h_pscene.begin() + s.p_begin + s.p_count, si);
+  }
+  return MPMX_OK;
+}
+
+int check_config(const mpmx_config* cfg) {
+  if (!(cfg->dx > 0) || !(cfg->dt > 0) || !(cfg->ppc > 0) || !(cfg->rho > 0)) return MPMX_ERR_ARG;
+  if (cfg->plastic_model != MPMX_PLASTIC_NONE && cfg->plastic_model != MPMX_PLASTIC_VMVISC) return MPMX_ERR_ARG;
+  // MUSL's second scatter (musl.cpp:6-98) has no device kernel: refuse the
+  // flag rather than run a USL step under a MUSL configuration.
+  if (cfg->use_musl != 0) return MPMX_ERR_ARG;
+  if (cfg->closure != MPMX_CLOSURE_EXACT && cfg->closure != MPMX_CLOSURE_LEARNED &&
+      cfg->closure != MPMX_CLOSURE_LEARNED_REFINED)
+    return MPMX_ERR_ARG;
+  if (cfg->closure != MPMX_CLOSURE_EXACT && !(cfg->closure_tol > 0)) return MPMX_ERR_ARG;
+  return MPMX_OK;
+}
+}  // namespace
+
 int mpmx_upload_particles(mpmx_handle* h, int64_t n, const float* x, const float* v, const float* F,
                           const float* C, const float* pig, const float* diff, const float* eps,
                           const uint8_t* active) {
   if (!h) return MPMX_ERR_ARG;
+  if (h->batch) return fail(h, MPMX_ERR_STATE, "upload: batched handle, use mpmx_upload_scene");
   if (n <= 0 || !x) return fail(h, MPMX_ERR_ARG, "upload: need n > 0 and positions");
   if (n > (int64_t)0x7fffffff / 2) return fail(h, MPMX_ERR_CAPACITY, "upload: too many particles");
   h->np = n;
-  h->h_comp.assign((size_t)P_NCOMP * n, 0.0f);
-  h->h_active.assign(n, 1);
-  for (int64_t p = 0; p < n; ++p) {
-    for (int c = 0; c < 3; ++c) {
-      h->h_comp[(P_X + c) * n + p] = x[3 * p + c];
-      h->h_comp[(P_V + c) * n + p] = v ? v[3 * p + c] : 0.0f;
-    }
-    for (int c = 0; c < 9; ++c) {
-      h->h_comp[(P_C + c) * n + p] = C ? C[9 * p + c] : 0.0f;
-      h->h_comp[(P_F + c) * n + p] = F ? F[9 * p + c] : ((c == 0 || c == 4 || c == 8) ? 1.0f : 0.0f);
-    }
-    for (int c = 0; c < NPIG; ++c) h->h_comp[(P_PIG + c) * n + p] = pig ? pig[7 * p + c] : 0.0f;
-    h->h_comp[P_DIFF * n + p] = diff ? diff[p] : 0.0f;
-    h->h_comp[P_EPS * n + p] = eps ? eps[p] : 0.0f;
-    if (active) h->h_active[p] = active[p] ? 1 : 0;
-  }
+  stage(h->h_comp, h->h_active, n, x, v, F, C, pig, diff, eps, active);
   h->h_pscene.assign(n, 0);
   h->scenes[0].p_begin = 0;
   h->scenes[0].p_count = n;
@@ -521,9 +582,43 @@ int mpmx_upload_particles(mpmx_handle* h, int64_t n, const float* x, const float
   return MPMX_OK;
 }
 
+int mpmx_upload_scene(mpmx_handle* h, int scene, int64_t n, const float* x, const float* v, const float* F,
+                      const float* C, const float* pig, const float* diff, const float* eps, const uint8_t* active) {
+  if (!h) return MPMX_ERR_ARG;
+  if (!h->batch) {
+    if (scene != 0) return fail(h, MPMX_ERR_ARG, "upload_scene: no such scene");
+    return mpmx_upload_particles(h, n, x, v, F, C, pig, diff, eps, active);
+  }
+  if (scene < 0 || scene >= (int)h->scenes.size()) return fail(h, MPMX_ERR_ARG, "upload_scene: no such scene");
+  if (n <= 0 || !x) return fail(h, MPMX_ERR_ARG, "upload_scene: need n > 0 and positions");
+  if (n > (int64_t)0x7fffffff / 2) return fail(h, MPMX_ERR_CAPACITY, "upload_scene: too many particles");
+  Scene& s = h->scenes[scene];
+  stage(s.comp, s.act, n, x, v, F, C, pig, diff, eps, active);
+  s.p_count = n;
+  h->np = 0;
+  h->prepared = false;
+  return MPMX_OK;
+}
+
+int mpmx_scene_range(mpmx_handle* h, int scene, int64_t* first, int64_t* count) {
+  if (!h || !first || !count || scene < 0 || scene >= (int)h->scenes.size()) return MPMX_ERR_ARG;
+  if (!h->prepared) return fail(h, MPMX_ERR_STATE, "scene_range: prepare first");
+  *first = h->scenes[scene].p_begin;
+  *count = h->scenes[scene].p_count;
+  return MPMX_OK;
+}
+
 int mpmx_set_objects(mpmx_handle* h, const mpmx_plate* plates, int n_plates, const mpmx_knife* knives,
                      int n_knives) {
   if (!h) return MPMX_ERR_ARG;
+  if (h->batch) return fail(h, MPMX_ERR_STATE, "set_objects: batched handle, use mpmx_set_objects_scene");
+  return mpmx_set_objects_scene(h, 0, plates, n_plates, knives, n_knives);
+}
+
+int mpmx_set_objects_scene(mpmx_handle* h, int scene, const mpmx_plate* plates, int n_plates,
+                           const mpmx_knife* knives, int n_knives) {
+  if (!h) return MPMX_ERR_ARG;
+  if (scene < 0 || scene >= (int)h->scenes.size()) return fail(h, MPMX_ERR_ARG, "set_objects: no such scene");
   if (n_plates < 0 || n_plates > MAX_PLATES || n_knives < 0 || n_knives > 1)
     return fail(h, MPMX_ERR_CAPACITY, "set_objects: at most 4 plates and 1 knife");
   for (int i = 0; i < n_plates; ++i)
@@ -531,41 +626,46 @@ int mpmx_set_objects(mpmx_handle* h, const mpmx_plate* plates, int n_plates, con
       return fail(h, MPMX_ERR_ARG, "set_objects: invalid plate bc/type");
   if (n_knives && (knives[0].bc < 0 || knives[0].bc > 2 || knives[0].shape < 0 || knives[0].shape > 1))
     return fail(h, MPMX_ERR_ARG, "set_objects: invalid knife bc/shape");
-  Scene& s = h->scenes[0];
+  Scene& s = h->scenes[scene];
   s.plates.assign(plates, plates + n_plates);
   s.has_knife = n_knives > 0;
   if (s.has_knife) s.knife = knives[0];
   if (h->prepared) {
     fill_scene_dev(s, s.dev.node_block_base, s.dev.axis_off);
-    HIPCHK(h, hipMemcpyAsync(h->d_scenes, &s.dev, sizeof(SceneDev), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d_scenes + scene, &s.dev, sizeof(SceneDev), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
   }
   return MPMX_OK;
 }
 
 int mpmx_set_knife_keyframes(mpmx_handle* h, int k, const float* frames7, int n_frames, float fps) {
-  if (!h || k != 0 || n_frames < 0 || (n_frames > 0 && !frames7) || !(fps > 0)) return MPMX_ERR_ARG;
-  Scene& s = h->scenes[0];
+  if (!h || k < 0 || k >= (int)h->scenes.size() || n_frames < 0 || (n_frames > 0 && !frames7) || !(fps > 0))
+    return MPMX_ERR_ARG;
+  Scene& s = h->scenes[k];
   s.frames.assign(frames7, frames7 + 7 * (size_t)n_frames);
   s.frame_fps = fps;
   return MPMX_OK;
 }
 
 int mpmx_set_knife_state(mpmx_handle* h, int k, const mpmx_knife_state* st) {
-  if (!h || k != 0 || !st) return MPMX_ERR_ARG;
-  h->scenes[0].has_override = true;
-  h->scenes[0].override_state = *st;
+  if (!h || k < 0 || k >= (int)h->scenes.size() || !st) return MPMX_ERR_ARG;
+  h->scenes[k].has_override = true;
+  h->scenes[k].override_state = *st;
   return MPMX_OK;
 }
 
 int mpmx_knife_state_at(mpmx_handle* h, int k, int64_t step, mpmx_knife_state* out) {
-  if (!h || k != 0 || !out) return MPMX_ERR_ARG;
-  *out = scene_knife_state(h->scenes[0], step);
+  if (!h || k < 0 || k >= (int)h->scenes.size() || !out) return MPMX_ERR_ARG;
+  *out = scene_knife_state(h->scenes[k], step);
   return MPMX_OK;
 }
 
 int mpmx_prepare(mpmx_handle* h) {
   if (!h) return MPMX_ERR_ARG;
+  if (h->batch) {
+    const int rc = gather_batch(h);
+    if (rc) return rc;
+  }
   if (h->np <= 0) return fail(h, MPMX_ERR_STATE, "prepare: upload particles first");
   free_device(h);
   HIPCHK(h, hipSetDevice(h->device));
@@ -1047,3 +1147,55 @@ int mpmx_profile_steps(mpmx_handle* h, int nsteps, mpmx_kernel_times* out) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------- launch-shape autotuner
+// Coordinate descent over workgroups per CU: P2G (one-wavefront workgroups),
+// then G2P (256-thread workgroups; the learned variant fits at most two per
+// CU), then the grid update.  Each candidate is timed with HIP events over
+// `steps_per_trial` ordinary steps (the simulation advances; the launch shape
+// changes no result beyond FP32 atomic order), and the fastest is kept.  The
+// step graph is rebuilt with the chosen shapes at the next mpmx_step.
+int mpmx_autotune(mpmx_handle* h, int steps_per_trial, mpmx_tune_report* rep) {
+  if (!h || steps_per_trial <= 0) return MPMX_ERR_ARG;
+  if (!h->prepared) return fail(h, MPMX_ERR_STATE, "autotune: prepare first");
+  const std::vector<int> cand_p2g = {8, 12, 16, 20, 24, 32};
+  const std::vector<int> cand_g2p = learned(h) ? std::vector<int>{1, 2} : std::vector<int>{2, 3, 4, 6, 8};
+  const std::vector<int> cand_grid = {2, 4, 8};
+  int trials = 0;
+  mpmx_kernel_times kt;
+  auto sweep = [&](int* knob, const std::vector<int>& cands, float mpmx_kernel_times::*field, float* best_ms) -> int {
+    int best = *knob;
+    double best_t = 1e30;
+    for (int c : cands) {
+      *knob = c;
+      int rc = mpmx_profile_steps(h, steps_per_trial, &kt);
+      if (rc) return rc;
+      ++trials;
+      if (kt.*field < best_t) {
+        best_t = kt.*field;
+        best = c;
+      }
+    }
+    *knob = best;
+    if (best_ms) *best_ms = (float)best_t;
+    return MPMX_OK;
+  };
+  float ms_p2g = 0, ms_g2p = 0, ms_grid = 0;
+  int rc;
+  if ((rc = sweep(&h->wg_p2g, cand_p2g, &mpmx_kernel_times::ms_p2g, &ms_p2g))) return rc;
+  if ((rc = sweep(&h->wg_g2p, cand_g2p, &mpmx_kernel_times::ms_g2p, &ms_g2p))) return rc;
+  if ((rc = sweep(&h->wg_grid, cand_grid, &mpmx_kernel_times::ms_grid, &ms_grid))) return rc;
+  if (h->graph_exec) { (void)hipGraphExecDestroy(h->graph_exec); h->graph_exec = nullptr; }
+  if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+  if (rep) {
+    rep->wg_p2g = h->wg_p2g;
+    rep->wg_g2p = h->wg_g2p;
+    rep->wg_grid = h->wg_grid;
+    rep->ms_p2g = ms_p2g;
+    rep->ms_g2p = ms_g2p;
+    rep->ms_grid = ms_grid;
+    rep->trials = trials;
+    rep->steps = (int64_t)trials * steps_per_trial;
+  }
+  return MPMX_OK;
+}

@@ -10,7 +10,12 @@ reference's own Mixbox).
                          translating at 0.1 m/s through the shared face, 2000 steps
   paint_4blob_256x64x256 four 64x8x64-cell blobs, 1,048,576 particles, slab knife
                          on a 40 mm circle at 0.15 m/s, 500 steps
-Material (both): E = 1e5 Pa, nu = 0.3, rho = 1500 kg/m^3, von Mises yield 50 Pa,
+  batch_scenes           64 independent scenes (seeds 42 + s): particle counts
+                         log-uniform in 20k-400k, 2-4 box blobs of random extents
+                         and palette colours, cubic lattices of 64^3-160^3 nodes,
+                         yield U(10, 200) Pa, viscosity U(1, 50) Pa s, slab knife
+                         translating at U(0.05, 0.3) m/s, 300 steps
+Material (first two): E = 1e5 Pa, nu = 0.3, rho = 1500 kg/m^3, von Mises yield 50 Pa,
 viscosity 10 Pa s (perzyna_visc = 10/50 = 0.2 s), g = -9.81, flip_ratio -0.5,
 pigment parameters of the reference's pigment_config.json, dt = 1e-5 s,
 dx = 3.125 mm, floor SlipStick mu = 1 at y = 0, knife SlipFree mu = 0.3.
@@ -115,6 +120,52 @@ def paint_4blob(cells=(64, 8, 64), dims=(256, 64, 256), steps=500, dx=DX, seed=4
     kn = slab_knife(abi.MOTION_CIRCULAR, (cx * dx, 0.5 * cy * dx, cz * dx), radius=0.040, speed=0.15)
     return dict(name=f"paint_4blob_{dims[0]}x{dims[1]}x{dims[2]}", cfg=cfg, x=x, pig=pig,
                 plates=[floor_plate()], knife=kn, steps=steps)
+
+
+def batch_scene(s, seed=42, np_range=(20_000, 400_000), dx=DX):
+    """Scene s of the batched workload (its own seed 42 + s)."""
+    rng = np.random.default_rng(seed + s)
+    n_target = math.exp(rng.uniform(math.log(np_range[0]), math.log(np_range[1])))
+    k = int(rng.integers(2, 5))                       # 2..4 blobs
+    share = rng.dirichlet(np.full(k, 4.0))            # uneven blob sizes
+    cols = math.ceil(math.sqrt(k))
+    cy = 8
+    while True:
+        sides = [max(4, int(round(math.sqrt(n_target * f / 8.0 / cy)))) for f in share]
+        extent = cols * max(sides)
+        if extent + 16 <= 160 or cy >= 32:
+            break
+        cy += 4
+    g = int(min(160, max(64, extent + 16)))
+    names = sorted(LATENTS)
+    specs, offs = [], 4
+    for b, side in enumerate(sides):
+        ci, cj = b % cols, b // cols
+        lo = (offs + ci * max(sides), 0, offs + cj * max(sides))
+        specs.append((lo, (side, cy, side), names[int(rng.integers(len(names)))]))
+    x, pig = abi.scene_boxes(_boxes(specs, dx), dx, 2, seed + s)
+    yield_pa = float(rng.uniform(10.0, 200.0))
+    visc = float(rng.uniform(1.0, 50.0))
+    speed = float(rng.uniform(0.05, 0.3))
+    cfg = material_config((g, g, g), yield_pa=yield_pa, viscosity=visc, dx=dx)
+    x_mid = (offs + 0.5 * max(sides)) * dx
+    kn = slab_knife(abi.MOTION_LINEAR, (x_mid - 0.015, 0.5 * cy * dx, (offs + 0.5 * max(sides)) * dx),
+                    velocity=(speed, 0.0, 0.0))
+    return dict(name=f"batch_{s}_{g}^3", cfg=cfg, x=x, pig=pig, plates=[floor_plate()], knife=kn, steps=300,
+                yield_pa=yield_pa, viscosity=visc, knife_speed=speed)
+
+
+def batch_scenes(n=64, seed=42):
+    return [batch_scene(s, seed) for s in range(n)]
+
+
+def make_batch_handle(scene_list):
+    h = abi.Handle([sc["cfg"] for sc in scene_list])
+    for i, sc in enumerate(scene_list):
+        h.upload_scene(i, sc["x"], pig=sc["pig"])
+        h.set_objects_scene(i, sc["plates"], sc["knife"])
+    h.prepare()
+    return h
 
 
 def make_handle(scene):
