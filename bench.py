@@ -74,7 +74,11 @@ def cpu_baseline(scene, steps=2, threads=None):
                       f"(reference cost structure), -O3 -ffast-math"}
 
 
-def timed_run(h, steps, warmup):
+def timed_run(h, steps, warmup, tune=True):
+    """Autotune the launch shapes (its steps are part of the untimed warm-up),
+    warm up, then time `steps` graph-launched steps."""
+    if tune:
+        h.autotune(2)
     h.step(warmup)
     h.sync()
     t0 = time.perf_counter()
@@ -198,7 +202,7 @@ def main():
         nb = sum(sc["x"].shape[0] for sc in sl)
         hb = scenes.make_batch_handle(sl)
         tune = hb.autotune(4)
-        wall_b, st_b = timed_run(hb, args.batch_steps, 5)
+        wall_b, st_b = timed_run(hb, args.batch_steps, 5, tune=False)
         kt_b = hb.profile(10)
         hb.close()
         batch = {"workload": "batch_64_scenes", "scenes": 64, "particles": nb,

@@ -724,6 +724,8 @@ int mpmx_prepare(mpmx_handle* h) {
   ALLOC(d.cell_cnt, int, (size_t)h->nkeys * BLK3);
   ALLOC(d.cell_off, int, (size_t)h->nkeys * BLK3);
   ALLOC(d.cell_num, int, (size_t)h->nkeys * BLK3);
+  ALLOC(d.pdesc, int4, h->nkeys);
+  ALLOC(d.list_pos, int, h->nkeys);
   ALLOC(d.count_next, int, h->nkeys);
   ALLOC(d.count_cur, int, h->nkeys);
   ALLOC(d.offsets, int, h->nkeys);

@@ -11,13 +11,13 @@
 //   k_scan_*  prefix sums over block counts -> block offsets + block list.
 //   k_cellscan per block: offsets of the 64 cells inside the block.
 //   k_perm    slot <- particle permutation: block-major, cell-major inside a block.
-//   k_p2g     lane = cell, wave = channel group: each lane walks its cell's
-//             particles and accumulates their 27-node contributions (mass,
+//   k_p2g     lane = cell, wave = (block, channel group): each lane walks its
+//             cell's particles and accumulates their 27-node contributions (mass,
 //             APIC momentum, Kirchhoff force, pigment latent, shear intensity;
-//             p2g.cpp:6-126, explicit_euler_update.cpp:12-55) in registers, then
-//             adds them to a 6x6x6-node LDS tile (lanes hold distinct cells, so no
-//             two lanes of one instruction hit the same node) and the tile to the
-//             grid accumulators with global atomics.
+//             p2g.cpp:6-126, explicit_euler_update.cpp:12-55) in registers; the
+//             lanes' stencils merge across lanes (DPP / shuffles, one axis at a
+//             time) into the block's 6x6x6 node tile with one owner per node, which
+//             adds it to the grid accumulators with a global atomic.
 //   k_grid    one wavefront per active 4x4x4 node block: normalise, gravity + force,
 //             boundary collision with plates and knife (explicit_euler_update.cpp:64-110,
 //             boundary_collision.cpp:5-316), write the gather grid, clear accumulators.
@@ -213,7 +213,12 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_down(DevPtrs d, int nkeys
     if (k < nkeys) {
       d.offsets[k] = (int)(run >> 32);
       d.count_cur[k] = (int)(items[j] >> 32);
-      if (items[j] & 1ull) d.pb_list[(int)(run & 0xffffffffull)] = k;
+      if (items[j] & 1ull) {
+        const int e = (int)(run & 0xffffffffull);
+        d.pb_list[e] = k;
+        d.list_pos[k] = e;
+        d.pdesc[e] = make_int4(k, (int)(run >> 32), (int)(items[j] >> 32), d.blk_scene[k]);
+      }
       d.count_next[k] = 0;
     }
     run += items[j];
@@ -255,8 +260,8 @@ __global__ void __launch_bounds__(256) k_cellscan(DevPtrs d) {
       const int t = __shfl_up(incl, off);
       if (lane >= off) incl += t;
     }
-    d.cell_off[idx] = incl - c;
-    d.cell_num[idx] = c;
+    d.cell_off[e * BLK3 + lane] = incl - c;
+    d.cell_num[e * BLK3 + lane] = c;
     d.cell_cnt[idx] = 0;
   }
 }
@@ -269,7 +274,7 @@ __global__ void k_perm(DevPtrs d, int64_t n) {
   if (j >= n) return;
   const int key = d.key_out[j];
   const int cell = d.cell_out[j];
-  const int dest = d.offsets[key] + (cell >= 0 ? d.cell_off[key * BLK3 + cell] : 0) + d.rank_out[j];
+  const int dest = d.offsets[key] + (cell >= 0 ? d.cell_off[d.list_pos[key] * BLK3 + cell] : 0) + d.rank_out[j];
   d.perm[dest] = (int)j;
 }
 
@@ -368,7 +373,7 @@ template <bool LEARNED>
 __global__ void __launch_bounds__(256) k_g2p(DevPtrs d, const SceneDev* __restrict__ scenes,
                                              const float* __restrict__ axis, const float* __restrict__ A,
                                              const int* __restrict__ perm, const float* __restrict__ gv,
-                                             const uint8_t* __restrict__ mlp_blob) {
+                                             const uint8_t* __restrict__ mlp_blob, const int4* __restrict__ pdesc) {
   __shared__ float4 gtile[GT3 * 4];
   __shared__ float sax[3][GT];       // node coordinates B0 .. B0+5 per axis
   __shared__ unsigned int plastic_wg, fallback_wg, learned_wg, newton_wg;
@@ -385,8 +390,9 @@ __global__ void __launch_bounds__(256) k_g2p(DevPtrs d, const SceneDev* __restri
   }
   const XcdRange xr = xcd_range(nblk);
   for (int e = xr.e; e < xr.end; e += xr.stride) {
-    const int key = __builtin_amdgcn_readfirstlane(d.pb_list[e]);
-    const int s = __builtin_amdgcn_readfirstlane(d.blk_scene[key]);
+    const int4 ds = pdesc[e];       // key, first slot, count, scene
+    const int key = ds.x;
+    const int s = ds.w;
     const SceneDev& sc = scenes[s];
     int bi, bj, bk;
     block_coords(sc, key, bi, bj, bk);
@@ -418,7 +424,7 @@ __global__ void __launch_bounds__(256) k_g2p(DevPtrs d, const SceneDev* __restri
       tf = 1.0f + t * (sc.boost_factor - 1.0f);
     }
     const float dynD = sc.D_max * tf;
-    const int start = d.offsets[key], cnt = d.count_cur[key];
+    const int start = ds.y, cnt = ds.z;
     const int iters = (cnt + blockDim.x - 1) / blockDim.x;
     for (int it = 0; it < iters; ++it) {
       const int t = it * blockDim.x + threadIdx.x;
@@ -721,24 +727,14 @@ __global__ void k_copy_dead(DevPtrs d) {
 //   0: mass (u64 fixed point)   1: APIC momentum     2: Kirchhoff force
 //   3: latent 0..2              4: latent 3..5       5: latent 6 + shear intensity
 // Each lane walks its cell's particles (cell-sorted, so the stencil base is the
-// cell for all of them), accumulates the 27 stencil nodes x 3 channels in
-// registers, then adds them to the wavefront's private 6x6x6-node LDS tile
-// (lanes of one instruction hold distinct cells: no two hit one node) and the
-// tile goes to the grid accumulators with global atomics.  Items are
+// cell for all of them) and accumulates the 27 stencil nodes x 3 channels in
+// registers.  The 64 lanes' stencils then merge into the block's 6x6x6 node tile
+// entirely in registers, one axis at a time (see p2g_merge), so that every tile
+// node ends with exactly one owning lane, which adds it to the grid
+// accumulators with one global atomic per channel.  No LDS is used.  Items are
 // independent, so there is no workgroup-wide phase: many small waves per CU
 // overlap their particle loads with other waves' arithmetic.
-constexpr int PT = 6;                 // tile edge (nodes) = 4 bins + 2
-constexpr int PT3 = PT * PT * PT;     // 216
 constexpr int P2G_GROUPS = 6;
-
-union P2GTile {
-  float acc[3][PT3];
-  unsigned long long mass[PT3];
-};
-
-__device__ __forceinline__ int tile_index(int ci, int cj, int ck, int n) {
-  return ((ci + n / 9) * PT + (cj + (n / 3) % 3)) * PT + (ck + n % 3);
-}
 
 // Per-item inputs shared by the channel-group bodies.
 struct P2GItem {
@@ -750,26 +746,57 @@ struct P2GItem {
   float inv_dx, apicDinv;
 };
 
+// Per-particle inputs of channel group G: position, then the group's payload.
+template <int G>
+struct P2GLoad {
+  static constexpr int NQ = G == 0 ? 0 : (G == 1 ? 12 : (G == 2 ? 6 : 3));
+  float x[3];
+  float q[NQ > 0 ? NQ : 1];
+};
+
+template <int G>
+__device__ __forceinline__ void p2g_load(const float* __restrict__ P, int64_t cap, int src, P2GLoad<G>& o) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) o.x[c] = P[(P_X + c) * cap + src];
+  if constexpr (G == 1) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o.q[c] = P[(P_V + c) * cap + src];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) o.q[3 + c] = P[(P_C + c) * cap + src];
+  } else if constexpr (G == 2) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) o.q[c] = P[(P_TAU + c) * cap + src];
+  } else if constexpr (G >= 3) {
+    constexpr int q0 = G == 3 ? P_PIG : (G == 4 ? P_PIG + 3 : P_PIG + 6);
+    o.q[0] = P[q0 * cap + src];
+    o.q[1] = P[(G == 5 ? P_DIFF : q0 + 1) * cap + src];
+    o.q[2] = G == 5 ? 0.0f : P[(q0 + 2) * cap + src];
+  }
+}
+
 // Accumulate this lane's cell into acc[27][3] for channel group G.  The loop
 // runs kmax (wave-uniform) times; a lane past its own count contributes exact
 // zeros through selects, so the accumulators never sit behind a branch.  Every
 // load is in bounds for every lane (an empty cell reads its block's first slot,
-// a finished lane re-reads its last particle).
+// a finished lane re-reads its last particle).  The next particle's inputs are
+// loaded while the current one is accumulated.
 template <int G>
 __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[27][3]) {
   const float* __restrict__ P = it.P;
   const int64_t cap = it.cap;
   const float inv_dx = it.inv_dx;
   const int klast = max(it.cnt - 1, 0);
-  int src = it.pp[0];
+  P2GLoad<G> cur;
+  p2g_load<G>(P, cap, it.pp[0], cur);
+  int src_next = it.pp[min(1, klast)];
   for (int k = 0; k < it.kmax; ++k) {
     const bool act = k < it.cnt;
-    const int cur = src;
-    src = it.pp[min(k + 1, klast)];   // next particle's slot, loaded ahead
-    const float x0r = P[(P_X + 0) * cap + cur], x1r = P[(P_X + 1) * cap + cur], x2r = P[(P_X + 2) * cap + cur];
-    const float x0 = act ? x0r : it.X[0][1];
-    const float x1 = act ? x1r : it.X[1][1];
-    const float x2 = act ? x2r : it.X[2][1];
+    P2GLoad<G> nxt;
+    p2g_load<G>(P, cap, src_next, nxt);      // next particle (clamped to the last one)
+    src_next = it.pp[min(k + 2, klast)];
+    const float x0 = act ? cur.x[0] : it.X[0][1];
+    const float x1 = act ? cur.x[1] : it.X[1][1];
+    const float x2 = act ? cur.x[2] : it.X[2][1];
     float N[9];
 #pragma unroll
     for (int o = 0; o < 3; ++o) {
@@ -791,15 +818,9 @@ __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[2
     } else if constexpr (G == 1) {
       float v[3], C[9];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const float t = P[(P_V + c) * cap + cur];
-        v[c] = act ? t : 0.0f;
-      }
+      for (int c = 0; c < 3; ++c) v[c] = act ? cur.q[c] : 0.0f;
 #pragma unroll
-      for (int c = 0; c < 9; ++c) {
-        const float t = P[(P_C + c) * cap + cur] * it.apicDinv;
-        C[c] = act ? t : 0.0f;
-      }
+      for (int c = 0; c < 9; ++c) C[c] = act ? cur.q[3 + c] * it.apicDinv : 0.0f;
 #pragma unroll
       for (int n = 0; n < 27; ++n) {
         float wgt = (N[n / 9] * N[3 + (n / 3) % 3]) * N[6 + n % 3];
@@ -818,10 +839,7 @@ __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[2
         dN[6 + o] = spline_dN(__fmul_rn(__fsub_rn(x2, it.X[2][o]), inv_dx)) * inv_dx;
       }
 #pragma unroll
-      for (int c = 0; c < 6; ++c) {
-        const float t = P[(P_TAU + c) * cap + cur];
-        tau[c] = act ? t : 0.0f;
-      }
+      for (int c = 0; c < 6; ++c) tau[c] = act ? cur.q[c] : 0.0f;
 #pragma unroll
       for (int n = 0; n < 27; ++n) {
         const int oi = n / 9, oj = (n / 3) % 3, ok = n % 3;
@@ -834,17 +852,9 @@ __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[2
       }
     } else {
       // latent / shear: three (group 5: two) weighted scalars
-      constexpr int q0 = G == 3 ? P_PIG : (G == 4 ? P_PIG + 3 : P_PIG + 6);
-      constexpr int q1 = G == 5 ? P_DIFF : q0 + 1;
-      const float ta = P[q0 * cap + cur];
-      const float tb = P[q1 * cap + cur];
-      const float qa = act ? ta : 0.0f;
-      const float qb = act ? tb : 0.0f;
-      float qc = 0.0f;
-      if constexpr (G != 5) {
-        const float tc = P[(q0 + 2) * cap + cur];
-        qc = act ? tc : 0.0f;
-      }
+      const float qa = act ? cur.q[0] : 0.0f;
+      const float qb = act ? cur.q[1] : 0.0f;
+      const float qc = (act && G != 5) ? cur.q[2] : 0.0f;
 #pragma unroll
       for (int n = 0; n < 27; ++n) {
         const float wgt = (N[n / 9] * N[3 + (n / 3) % 3]) * N[6 + n % 3];
@@ -853,43 +863,115 @@ __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[2
         if constexpr (G != 5) acc[n][2] += qc * wgt;
       }
     }
+    cur = nxt;
   }
 }
 
-// Lanes (distinct cells) add their 27 nodes into the wave's tile.
-template <int G>
-__device__ __forceinline__ void p2g_to_tile(P2GTile& tile, int ci, int cj, int ck, int cnt, const float (&acc)[27][3]) {
-  if (cnt <= 0) return;
-  if constexpr (G == 0) {
+// ---- cross-lane merge of the 27-node stencils (lane = ci*16 + cj*4 + ck)
+// Lane shifts: the value of the lane N below within the 16-lane row (DPP
+// row_shr, cells N apart in z for N = 1, 2, in y for N = 4, 8) or N lanes below
+// across rows (cells 1 or 2 apart in x for N = 16, 32).  All are wave-wide
+// convergent operations executed by every lane; lanes without a source are
+// masked by the caller.
+template <int N>
+__device__ __forceinline__ uint32_t lane_shr_u32(uint32_t v) {
+  if constexpr (N >= 16) return (uint32_t)__shfl_up((int)v, N);
+  else return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x110 + N, 0xF, 0xF, true);
+}
+template <int N>
+__device__ __forceinline__ float lane_shr(float v) { return __uint_as_float(lane_shr_u32<N>(__float_as_uint(v))); }
+template <int N>
+__device__ __forceinline__ unsigned long long lane_shr(unsigned long long v) {
+  const uint32_t lo = lane_shr_u32<N>((uint32_t)v), hi = lane_shr_u32<N>((uint32_t)(v >> 32));
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// One axis of the merge.  v[27] is indexed (a, b, c) -> 9a + 3b + c, the axis
+// being reduced has index stride S and offsets 0, 1, 2 (node = cell + offset
+// along it, cell coordinate `coord` in 0..3).  Afterwards, along that axis,
+// slot 0 holds node `coord` (all contributions: own offset 0, the cell below's
+// offset 1, the cell two below's offset 2), and slots 1, 2 hold nodes 4 and 5,
+// complete only on lanes with coord == 3.
+template <typename T, int S, int SH1, int SH2>
+__device__ __forceinline__ void merge_axis(T (&v)[27], int coord) {
 #pragma unroll
-    for (int n = 0; n < 27; ++n)
-      atomicAdd(&tile.mass[tile_index(ci, cj, ck, n)],
-                ((unsigned long long)__float_as_uint(acc[n][1]) << 32) | (unsigned long long)__float_as_uint(acc[n][0]));
-  } else {
-#pragma unroll
-    for (int n = 0; n < 27; ++n) {
-      const int tix = tile_index(ci, cj, ck, n);
-      atomicAdd(&tile.acc[0][tix], acc[n][0]);
-      atomicAdd(&tile.acc[1][tix], acc[n][1]);
-      if constexpr (G != 5) atomicAdd(&tile.acc[2][tix], acc[n][2]);
-    }
+  for (int i = 0; i < 27; ++i) {
+    if ((i / S) % 3 != 0) continue;
+    const T w0 = v[i], w1 = v[i + S], w2 = v[i + 2 * S];
+    const T a1 = lane_shr<SH1>(w1), a2 = lane_shr<SH2>(w2), b2 = lane_shr<SH1>(w2);
+    v[i] = w0 + (coord >= 1 ? a1 : T(0)) + (coord >= 2 ? a2 : T(0));
+    v[i + S] = w1 + (coord >= 1 ? b2 : T(0));
+    v[i + 2 * S] = w2;
   }
 }
 
+template <typename T>
+__device__ __forceinline__ void p2g_merge(T (&v)[27], int ci, int cj, int ck) {
+  merge_axis<T, 1, 1, 2>(v, ck);     // z: neighbouring lanes
+  merge_axis<T, 3, 4, 8>(v, cj);     // y: lanes 4 apart in the row
+  merge_axis<T, 9, 16, 32>(v, ci);   // x: lanes 16 apart
+}
+
+// Tile node (0..5 per axis) held in merged slot (xs, ys, zs) by lane (ci, cj,
+// ck): slot 0 -> the cell coordinate, slots 1, 2 -> 4, 5 (owned by coord == 3).
+__device__ __forceinline__ bool slot_node(int xs, int ys, int zs, int ci, int cj, int ck, int& tx, int& ty, int& tz) {
+  tx = xs == 0 ? ci : 3 + xs;
+  ty = ys == 0 ? cj : 3 + ys;
+  tz = zs == 0 ? ck : 3 + zs;
+  return (xs == 0 || ci == 3) && (ys == 0 || cj == 3) && (zs == 0 || ck == 3);
+}
+
 template <int G>
-__device__ __forceinline__ void p2g_group(const P2GItem& it, P2GTile& tile, int ci, int cj, int ck) {
+__device__ __forceinline__ void p2g_group(const P2GItem& it, const DevPtrs& d, const SceneDev& sc, int B0i, int B0j,
+                                          int B0k, int ci, int cj, int ck, unsigned& touched) {
   float acc[27][3];
 #pragma unroll
   for (int n = 0; n < 27; ++n) { acc[n][0] = acc[n][1] = acc[n][2] = 0.0f; }
   p2g_accumulate<G>(it, acc);
-  p2g_to_tile<G>(tile, ci, cj, ck, it.cnt, acc);
+  if constexpr (G == 0) {
+    unsigned long long m[27];
+#pragma unroll
+    for (int n = 0; n < 27; ++n)
+      m[n] = ((unsigned long long)__float_as_uint(acc[n][1]) << 32) | (unsigned long long)__float_as_uint(acc[n][0]);
+    p2g_merge(m, ci, cj, ck);
+#pragma unroll
+    for (int n = 0; n < 27; ++n) {
+      int tx, ty, tz;
+      const bool own = slot_node(n / 9, (n / 3) % 3, n % 3, ci, cj, ck, tx, ty, tz);
+      if (own && m[n] != 0ull) {
+        const int gi = B0i + tx, gj = B0j + ty, gk = B0k + tz;
+        atomicAdd(&d.massfx[(size_t)node_block_of(sc, gi, gj, gk) * BLK3 + node_local(gi, gj, gk)], m[n]);
+        touched |= 1u << ((((tx >> 2) * 2) + (ty >> 2)) * 2 + (tz >> 2));
+      }
+    }
+  } else {
+    // group 5's channels are latent 6 and shear (A_PIG + 6, A_SHR = A_PIG + 7)
+    constexpr int ch0 = G == 1 ? A_MOM : (G == 2 ? A_FRC : (G == 3 ? A_PIG : (G == 4 ? A_PIG + 3 : A_PIG + 6)));
+    constexpr int nch = G == 5 ? 2 : 3;
+#pragma unroll
+    for (int c = 0; c < nch; ++c) {
+      float v[27];
+#pragma unroll
+      for (int n = 0; n < 27; ++n) v[n] = acc[n][c];
+      p2g_merge(v, ci, cj, ck);
+#pragma unroll
+      for (int n = 0; n < 27; ++n) {
+        int tx, ty, tz;
+        const bool own = slot_node(n / 9, (n / 3) % 3, n % 3, ci, cj, ck, tx, ty, tz);
+        if (own && v[n] != 0.0f) {
+          const int gi = B0i + tx, gj = B0j + ty, gk = B0k + tz;
+          atomicAdd(d.acc + ((size_t)node_block_of(sc, gi, gj, gk) * ACC_CH + ch0 + c) * BLK3 + node_local(gi, gj, gk),
+                    v[n]);
+        }
+      }
+    }
+  }
 }
 
 __global__ void __launch_bounds__(64) k_p2g(DevPtrs d, const SceneDev* __restrict__ scenes,
                                             const float* __restrict__ axis, const float* __restrict__ P,
-                                            const int* __restrict__ perm) {
-  __shared__ P2GTile tile;
-  __shared__ unsigned int touched;
+                                            const int* __restrict__ perm, const int4* __restrict__ pdesc,
+                                            const int* __restrict__ cell_off, const int* __restrict__ cell_num) {
   const int nblk = d.counters[CNT_PBLOCKS];
   const int parity = d.counters[CNT_GSTEP] & 1;
   const int lane = threadIdx.x;
@@ -901,25 +983,24 @@ __global__ void __launch_bounds__(64) k_p2g(DevPtrs d, const SceneDev* __restric
   const int seg = (nblk + nx - 1) / nx;
   const int e0 = xcd * seg;
   const int nitems = max(0, min(nblk, e0 + seg) - e0) * P2G_GROUPS;
-  for (int t = lane; t < 3 * PT3; t += 64) (&tile.acc[0][0])[t] = 0.0f;
-  if (lane == 0) touched = 0u;
-  __syncthreads();
   for (int item = blockIdx.x / nx; item < nitems; item += per) {
     const int e = e0 + item / P2G_GROUPS;
     const int group = item % P2G_GROUPS;
-    const int key = __builtin_amdgcn_readfirstlane(d.pb_list[e]);
-    const SceneDev& sc = scenes[__builtin_amdgcn_readfirstlane(d.blk_scene[key])];
+    // descriptor (key, first slot, count, scene) and this lane's cell, all by list position
+    const int4 ds = pdesc[e];
+    const int key = ds.x;
+    const SceneDev& sc = scenes[ds.w];
+    P2GItem it;
+    it.cnt = cell_num[e * BLK3 + lane];
+    const int coff = cell_off[e * BLK3 + lane];
     int bi, bj, bk;
     block_coords(sc, key, bi, bj, bk);
     const int B0i = bi * 4, B0j = bj * 4, B0k = bk * 4;
-    P2GItem it;
     it.P = P;
     it.cap = d.cap;
     it.inv_dx = sc.one_over_dx;
     it.apicDinv = sc.apicDinv;
-    it.cnt = d.cell_num[key * BLK3 + lane];
-    const int coff = d.cell_off[key * BLK3 + lane];
-    it.pp = perm + d.offsets[key] + (it.cnt > 0 ? coff : 0);
+    it.pp = perm + ds.y + (it.cnt > 0 ? coff : 0);
     int kmax = it.cnt;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) kmax = max(kmax, __shfl_xor(kmax, off));
@@ -932,51 +1013,25 @@ __global__ void __launch_bounds__(64) k_p2g(DevPtrs d, const SceneDev* __restric
       it.X[1][o] = xs[sc.dims[0] + min(B0j + cj + o, sc.dims[1] - 1)];
       it.X[2][o] = xs[sc.dims[0] + sc.dims[1] + min(B0k + ck + o, sc.dims[2] - 1)];
     }
+    unsigned touched = 0u;
     switch (group) {
-      case 0: p2g_group<0>(it, tile, ci, cj, ck); break;
-      case 1: p2g_group<1>(it, tile, ci, cj, ck); break;
-      case 2: p2g_group<2>(it, tile, ci, cj, ck); break;
-      case 3: p2g_group<3>(it, tile, ci, cj, ck); break;
-      case 4: p2g_group<4>(it, tile, ci, cj, ck); break;
-      default: p2g_group<5>(it, tile, ci, cj, ck); break;
+      case 0: p2g_group<0>(it, d, sc, B0i, B0j, B0k, ci, cj, ck, touched); break;
+      case 1: p2g_group<1>(it, d, sc, B0i, B0j, B0k, ci, cj, ck, touched); break;
+      case 2: p2g_group<2>(it, d, sc, B0i, B0j, B0k, ci, cj, ck, touched); break;
+      case 3: p2g_group<3>(it, d, sc, B0i, B0j, B0k, ci, cj, ck, touched); break;
+      case 4: p2g_group<4>(it, d, sc, B0i, B0j, B0k, ci, cj, ck, touched); break;
+      default: p2g_group<5>(it, d, sc, B0i, B0j, B0k, ci, cj, ck, touched); break;
     }
-    __syncthreads();
-    // ---- tile -> grid accumulators (read and clear)
     if (group == 0) {
-      for (int t = lane; t < PT3; t += 64) {
-        const unsigned long long m = tile.mass[t];
-        if (m == 0ull) continue;
-        tile.mass[t] = 0ull;
-        const int ti = t / (PT * PT), tj = (t / PT) % PT, tk = t % PT;
-        const int gi = B0i + ti, gj = B0j + tj, gk = B0k + tk;
-        atomicAdd(&d.massfx[(size_t)node_block_of(sc, gi, gj, gk) * BLK3 + node_local(gi, gj, gk)], m);
-        atomicOr(&touched, 1u << ((((ti >> 2) * 2) + (tj >> 2)) * 2 + (tk >> 2)));
-      }
-      __syncthreads();
+      // node blocks receiving mass from this block: OR of the lanes' bits
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) touched |= (unsigned)__shfl_xor((int)touched, off);
       if (lane < 8 && (touched & (1u << lane))) {
         const int nb = sc.node_block_base + ((bi + (lane >> 2)) * sc.bdims[1] + (bj + ((lane >> 1) & 1))) * sc.bdims[2] +
                        (bk + (lane & 1));
         activate_node_block(d, nb, parity);
       }
-      __syncthreads();
-      if (lane == 0) touched = 0u;
-    } else {
-      // group 5's channels are latent 6 and shear (A_PIG + 6, A_SHR = A_PIG + 7)
-      const int ch0 = group == 1 ? A_MOM : (group == 2 ? A_FRC : (group == 3 ? A_PIG : (group == 4 ? A_PIG + 3 : A_PIG + 6)));
-      const int nch = group == 5 ? 2 : 3;
-      for (int t = lane; t < PT3; t += 64) {
-        const float a0 = tile.acc[0][t], a1 = tile.acc[1][t], a2 = tile.acc[2][t];
-        if (a0 == 0.0f && a1 == 0.0f && a2 == 0.0f) continue;
-        tile.acc[0][t] = 0.0f; tile.acc[1][t] = 0.0f; tile.acc[2][t] = 0.0f;
-        const int ti = t / (PT * PT), tj = (t / PT) % PT, tk = t % PT;
-        const int gi = B0i + ti, gj = B0j + tj, gk = B0k + tk;
-        float* g = d.acc + (size_t)node_block_of(sc, gi, gj, gk) * ACC_CH * BLK3 + node_local(gi, gj, gk);
-        if (a0 != 0.0f) atomicAdd(g + (ch0 + 0) * BLK3, a0);
-        if (a1 != 0.0f) atomicAdd(g + (ch0 + 1) * BLK3, a1);
-        if (nch == 3 && a2 != 0.0f) atomicAdd(g + (ch0 + 2) * BLK3, a2);
-      }
     }
-    __syncthreads();
   }
 }
 
@@ -1133,11 +1188,11 @@ void launch_g2p(const DevPtrs& d, int grid, const void* mlp_blob, hipStream_t st
   if (mlp_blob)
     hipLaunchKernelGGL(k_g2p<true>, dim3(grid), dim3(256), 0, st, d, (const SceneDev*)d.scenes,
                        (const float*)d.axis_coords, (const float*)d.pa, (const int*)d.perm, (const float*)d.gv,
-                       (const uint8_t*)mlp_blob);
+                       (const uint8_t*)mlp_blob, (const int4*)d.pdesc);
   else
     hipLaunchKernelGGL(k_g2p<false>, dim3(grid), dim3(256), 0, st, d, (const SceneDev*)d.scenes,
                        (const float*)d.axis_coords, (const float*)d.pa, (const int*)d.perm, (const float*)d.gv,
-                       (const uint8_t*)nullptr);
+                       (const uint8_t*)nullptr, (const int4*)d.pdesc);
   hipLaunchKernelGGL(k_copy_dead, dim3(64), dim3(256), 0, st, d);
 }
 void launch_closure_eval(const SceneDev* scenes, const void* mlp_blob, const float* trial4, int64_t n,
@@ -1149,7 +1204,8 @@ void launch_closure_eval(const SceneDev* scenes, const void* mlp_blob, const flo
 }
 void launch_p2g(const DevPtrs& d, int grid, int src_is_b, hipStream_t st) {
   hipLaunchKernelGGL(k_p2g, dim3(grid), dim3(64), 0, st, d, (const SceneDev*)d.scenes, (const float*)d.axis_coords,
-                     (const float*)(src_is_b ? d.pb : d.pa), (const int*)d.perm);
+                     (const float*)(src_is_b ? d.pb : d.pa), (const int*)d.perm, (const int4*)d.pdesc,
+                     (const int*)d.cell_off, (const int*)d.cell_num);
 }
 void launch_grid(const DevPtrs& d, int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_grid, dim3(grid), dim3(256), 0, st, d);

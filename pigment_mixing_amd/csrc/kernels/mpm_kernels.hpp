@@ -49,8 +49,10 @@ struct DevPtrs {
   int* cell_out;           // cell (0..63) of the particle in its block, -1 if dead
   int* perm;
   int* cell_cnt;           // per (block, cell) particle counts of the next sort
-  int* cell_off;           // per (block, cell) offsets inside the block
-  int* cell_num;           // per (block, cell) counts of the current sort
+  int* cell_off;           // per (listed block, cell) offsets inside the block, list order
+  int* cell_num;           // per (listed block, cell) counts of the current sort, list order
+  int4* pdesc;             // per listed block: key, first slot, particle count, scene
+  int* list_pos;           // block key -> position in pb_list (listed blocks only)
   int* count_next;
   int* count_cur;
   int* offsets;
