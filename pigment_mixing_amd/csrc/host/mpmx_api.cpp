@@ -725,6 +725,11 @@ int mpmx_prepare(mpmx_handle* h) {
   ALLOC(d.cell_off, int, (size_t)h->nkeys * BLK3);
   ALLOC(d.cell_num, int, (size_t)h->nkeys * BLK3);
   ALLOC(d.pdesc, int4, h->nkeys);
+  // merged P2G tiles: room for the listed blocks of dense paint (~355 particles
+  // per block); blocks beyond it scatter with atomics
+  d.tile_cap = (int)std::min<int64_t>(h->nkeys, std::max<int64_t>(1024, n / 128));
+  ALLOC(d.tile_acc, float, (size_t)d.tile_cap * ACC_CH * PT3);
+  ALLOC(d.tile_mass, unsigned long long, (size_t)d.tile_cap * PT3);
   ALLOC(d.list_pos, int, h->nkeys);
   ALLOC(d.count_next, int, h->nkeys);
   ALLOC(d.count_cur, int, h->nkeys);

@@ -280,7 +280,14 @@ __global__ void k_perm(DevPtrs d, int64_t n) {
 
 // ------------------------------------------------------------------ k_g2p
 constexpr int GT = 6;              // gather tile edge: stencils of bins [B0, B0+4) span nodes [B0, B0+6)
-constexpr int GT3 = GT * GT * GT;  // 216 nodes x 16 floats = 13.8 KB
+constexpr int GT3 = GT * GT * GT;  // 216 nodes
+// LDS row of one node: 16 floats padded to 5 float4 (80 B), so that the nodes
+// the cells of one wavefront read at once (node index differences 1, 2, 3, 6,
+// 7, 8, 9, 36, ...) start in distinct 4-bank groups.  The learned-closure
+// variant keeps 4 float4 (its network and activation buffers must leave room
+// for two workgroups per CU).
+template <bool LEARNED>
+constexpr int gt_row() { return LEARNED ? 4 : 5; }
 
 // Read-only inputs come in as __restrict__ kernel arguments (scene table, node
 // coordinates, particle input side, permutation, grid values), so uniform reads
@@ -374,7 +381,8 @@ __global__ void __launch_bounds__(256) k_g2p(DevPtrs d, const SceneDev* __restri
                                              const float* __restrict__ axis, const float* __restrict__ A,
                                              const int* __restrict__ perm, const float* __restrict__ gv,
                                              const uint8_t* __restrict__ mlp_blob, const int4* __restrict__ pdesc) {
-  __shared__ float4 gtile[GT3 * 4];
+  constexpr int GT_ROW = gt_row<LEARNED>();
+  __shared__ float4 gtile[GT3 * GT_ROW];
   __shared__ float sax[3][GT];       // node coordinates B0 .. B0+5 per axis
   __shared__ unsigned int plastic_wg, fallback_wg, learned_wg, newton_wg;
   __shared__ __align__(16) uint8_t lrn_raw[LEARNED ? sizeof(G2PLearnedLds) : 16];
@@ -406,7 +414,7 @@ __global__ void __launch_bounds__(256) k_g2p(DevPtrs d, const SceneDev* __restri
       const int gk = min(B0k + t % GT, sc.dims[2] - 1);
       const float4* g = reinterpret_cast<const float4*>(
           gv + ((size_t)node_block_of(sc, gi, gj, gk) * BLK3 + node_local(gi, gj, gk)) * GV_STRIDE);
-      gtile[q] = g[part];
+      gtile[t * GT_ROW + part] = g[part];
     }
     if (threadIdx.x < 3 * GT) {
       const int a = threadIdx.x / GT, o = threadIdx.x % GT;
@@ -487,7 +495,7 @@ __global__ void __launch_bounds__(256) k_g2p(DevPtrs d, const SceneDev* __restri
             const float g0 = (dij0 * w[2].N[ok]) * inv_dx;
             const float g1 = (dij1 * w[2].N[ok]) * inv_dx;
             const float g2 = (nij * w[2].dN[ok]) * inv_dx;
-            const float4* gn = &gtile[(trow + ok) * 4];
+            const float4* gn = &gtile[(trow + ok) * GT_ROW];
             const float4 n0 = gn[0], n1 = gn[1], n2 = gn[2], n3 = gn[3];
             const float gvv[3] = {n0.x, n0.y, n0.z};
             const float gfl[3] = {n0.w, n1.x, n1.y};
@@ -780,6 +788,14 @@ __device__ __forceinline__ void p2g_load(const float* __restrict__ P, int64_t ca
 // load is in bounds for every lane (an empty cell reads its block's first slot,
 // a finished lane re-reads its last particle).  The next particle's inputs are
 // loaded while the current one is accumulated.
+//
+// The path is VALU-bound, so the per-node arithmetic uses the separable
+// structure of the quadratic B-spline stencil: w = Nx Ny Nz with the 9 products
+// Ny Nz precomputed, the APIC term v + C (X_node - x_p) split into per-axis
+// parts (one add per axis per node), and the force gradients as products of
+// per-axis factors.  (Rounding differs from the product order of the reference
+// by O(ulp); the mass channel, which decides node activation, keeps the
+// reference's product order and its tiny-weight cut.)
 template <int G>
 __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[27][3]) {
   const float* __restrict__ P = it.P;
@@ -815,52 +831,78 @@ __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[2
         acc[n][0] = __uint_as_float((unsigned)m);
         acc[n][1] = __uint_as_float((unsigned)(m >> 32));
       }
-    } else if constexpr (G == 1) {
-      float v[3], C[9];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) v[c] = act ? cur.q[c] : 0.0f;
-#pragma unroll
-      for (int c = 0; c < 9; ++c) C[c] = act ? cur.q[3 + c] * it.apicDinv : 0.0f;
-#pragma unroll
-      for (int n = 0; n < 27; ++n) {
-        float wgt = (N[n / 9] * N[3 + (n / 3) % 3]) * N[6 + n % 3];
-        wgt = wgt > 1e-25f ? wgt : 0.0f;
-        const float pd0 = it.X[0][n / 9] - x0, pd1 = it.X[1][(n / 3) % 3] - x1, pd2 = it.X[2][n % 3] - x2;
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-          acc[n][r] += (v[r] + (C[3 * r] * pd0 + C[3 * r + 1] * pd1 + C[3 * r + 2] * pd2)) * wgt;
-      }
-    } else if constexpr (G == 2) {
-      float dN[9], tau[6];
-#pragma unroll
-      for (int o = 0; o < 3; ++o) {
-        dN[o] = spline_dN(__fmul_rn(__fsub_rn(x0, it.X[0][o]), inv_dx)) * inv_dx;
-        dN[3 + o] = spline_dN(__fmul_rn(__fsub_rn(x1, it.X[1][o]), inv_dx)) * inv_dx;
-        dN[6 + o] = spline_dN(__fmul_rn(__fsub_rn(x2, it.X[2][o]), inv_dx)) * inv_dx;
-      }
-#pragma unroll
-      for (int c = 0; c < 6; ++c) tau[c] = act ? cur.q[c] : 0.0f;
-#pragma unroll
-      for (int n = 0; n < 27; ++n) {
-        const int oi = n / 9, oj = (n / 3) % 3, ok = n % 3;
-        const float g0 = (dN[oi] * N[3 + oj]) * N[6 + ok];
-        const float g1 = (N[oi] * dN[3 + oj]) * N[6 + ok];
-        const float g2 = (N[oi] * N[3 + oj]) * dN[6 + ok];
-        acc[n][0] += tau[0] * g0 + tau[3] * g1 + tau[4] * g2;
-        acc[n][1] += tau[3] * g0 + tau[1] * g1 + tau[5] * g2;
-        acc[n][2] += tau[4] * g0 + tau[5] * g1 + tau[2] * g2;
-      }
     } else {
-      // latent / shear: three (group 5: two) weighted scalars
-      const float qa = act ? cur.q[0] : 0.0f;
-      const float qb = act ? cur.q[1] : 0.0f;
-      const float qc = (act && G != 5) ? cur.q[2] : 0.0f;
+      float NyNz[9];
 #pragma unroll
-      for (int n = 0; n < 27; ++n) {
-        const float wgt = (N[n / 9] * N[3 + (n / 3) % 3]) * N[6 + n % 3];
-        acc[n][0] += qa * wgt;
-        acc[n][1] += qb * wgt;
-        if constexpr (G != 5) acc[n][2] += qc * wgt;
+      for (int p = 0; p < 9; ++p) NyNz[p] = N[3 + p / 3] * N[6 + p % 3];
+      if constexpr (G == 1) {
+        // v + C (X - x) = [v + C_.x (X_oi - x)] + C_.y (Y_oj - y) + C_.z (Z_ok - z)
+        float ax[3][3], ay[3][3], az[3][3];   // [offset][component]
+#pragma unroll
+        for (int o = 0; o < 3; ++o) {
+          const float px = it.X[0][o] - x0, py = it.X[1][o] - x1, pz = it.X[2][o] - x2;
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            const float v = act ? cur.q[r] : 0.0f;
+            const float cx = act ? cur.q[3 + 3 * r] * it.apicDinv : 0.0f;
+            const float cy = act ? cur.q[3 + 3 * r + 1] * it.apicDinv : 0.0f;
+            const float cz = act ? cur.q[3 + 3 * r + 2] * it.apicDinv : 0.0f;
+            ax[o][r] = v + cx * px;
+            ay[o][r] = cy * py;
+            az[o][r] = cz * pz;
+          }
+        }
+#pragma unroll
+        for (int n = 0; n < 27; ++n) {
+          const int oi = n / 9, oj = (n / 3) % 3, ok = n % 3;
+          const float wgt = N[oi] * NyNz[n % 9];
+#pragma unroll
+          for (int r = 0; r < 3; ++r) acc[n][r] += ((ax[oi][r] + ay[oj][r]) + az[ok][r]) * wgt;
+        }
+      } else if constexpr (G == 2) {
+        // f_r = tau_r0 dNx Ny Nz + tau_r1 Nx dNy Nz + tau_r2 Nx Ny dNz (gradient / dx)
+        float dN[9];
+#pragma unroll
+        for (int o = 0; o < 3; ++o) {
+          dN[o] = spline_dN(__fmul_rn(__fsub_rn(x0, it.X[0][o]), inv_dx)) * inv_dx;
+          dN[3 + o] = spline_dN(__fmul_rn(__fsub_rn(x1, it.X[1][o]), inv_dx)) * inv_dx;
+          dN[6 + o] = spline_dN(__fmul_rn(__fsub_rn(x2, it.X[2][o]), inv_dx)) * inv_dx;
+        }
+        float tau[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) tau[c] = act ? cur.q[c] : 0.0f;
+        const float T[3][3] = {{tau[0], tau[3], tau[4]}, {tau[3], tau[1], tau[5]}, {tau[4], tau[5], tau[2]}};
+        float dyz[9], ydz[9];                   // dNy Nz, Ny dNz per (oj, ok)
+#pragma unroll
+        for (int p = 0; p < 9; ++p) {
+          dyz[p] = dN[3 + p / 3] * N[6 + p % 3];
+          ydz[p] = N[3 + p / 3] * dN[6 + p % 3];
+        }
+        float tx[3][3];                         // [oi][r]: tau_r0 dNx
+#pragma unroll
+        for (int o = 0; o < 3; ++o)
+#pragma unroll
+          for (int r = 0; r < 3; ++r) tx[o][r] = T[r][0] * dN[o];
+#pragma unroll
+        for (int n = 0; n < 27; ++n) {
+          const int oi = n / 9, p = n % 9;
+          const float a = NyNz[p], b = N[oi] * dyz[p], c = N[oi] * ydz[p];
+#pragma unroll
+          for (int r = 0; r < 3; ++r) acc[n][r] += tx[oi][r] * a + T[r][1] * b + T[r][2] * c;
+        }
+      } else {
+        // latent / shear: three (group 5: two) weighted scalars, q * Nx folded per offset
+        float qa = act ? cur.q[0] : 0.0f, qb = act ? cur.q[1] : 0.0f, qc = (act && G != 5) ? cur.q[2] : 0.0f;
+        float fa[3], fb[3], fc[3];
+#pragma unroll
+        for (int o = 0; o < 3; ++o) { fa[o] = qa * N[o]; fb[o] = qb * N[o]; fc[o] = qc * N[o]; }
+#pragma unroll
+        for (int n = 0; n < 27; ++n) {
+          const int oi = n / 9, p = n % 9;
+          acc[n][0] += fa[oi] * NyNz[p];
+          acc[n][1] += fb[oi] * NyNz[p];
+          if constexpr (G != 5) acc[n][2] += fc[oi] * NyNz[p];
+        }
       }
     }
     cur = nxt;
@@ -921,9 +963,13 @@ __device__ __forceinline__ bool slot_node(int xs, int ys, int zs, int ci, int cj
   return (xs == 0 || ci == 3) && (ys == 0 || cj == 3) && (zs == 0 || ck == 3);
 }
 
+// Merged tile of block e: listed blocks below tile_cap store it (every owned
+// slot, zeros included) for the grid kernel to gather; later blocks add it to
+// the grid accumulators with atomics.
 template <int G>
-__device__ __forceinline__ void p2g_group(const P2GItem& it, const DevPtrs& d, const SceneDev& sc, int B0i, int B0j,
-                                          int B0k, int ci, int cj, int ck, unsigned& touched) {
+__device__ __forceinline__ void p2g_group(const P2GItem& it, const DevPtrs& d, const SceneDev& sc, int e, int B0i,
+                                          int B0j, int B0k, int ci, int cj, int ck, unsigned& touched) {
+  const bool tiled = e < d.tile_cap;
   float acc[27][3];
 #pragma unroll
   for (int n = 0; n < 27; ++n) { acc[n][0] = acc[n][1] = acc[n][2] = 0.0f; }
@@ -938,9 +984,12 @@ __device__ __forceinline__ void p2g_group(const P2GItem& it, const DevPtrs& d, c
     for (int n = 0; n < 27; ++n) {
       int tx, ty, tz;
       const bool own = slot_node(n / 9, (n / 3) % 3, n % 3, ci, cj, ck, tx, ty, tz);
+      if (own && tiled) d.tile_mass[(size_t)e * PT3 + (tx * PT + ty) * PT + tz] = m[n];
       if (own && m[n] != 0ull) {
-        const int gi = B0i + tx, gj = B0j + ty, gk = B0k + tz;
-        atomicAdd(&d.massfx[(size_t)node_block_of(sc, gi, gj, gk) * BLK3 + node_local(gi, gj, gk)], m[n]);
+        if (!tiled) {
+          const int gi = B0i + tx, gj = B0j + ty, gk = B0k + tz;
+          atomicAdd(&d.massfx[(size_t)node_block_of(sc, gi, gj, gk) * BLK3 + node_local(gi, gj, gk)], m[n]);
+        }
         touched |= 1u << ((((tx >> 2) * 2) + (ty >> 2)) * 2 + (tz >> 2));
       }
     }
@@ -958,7 +1007,8 @@ __device__ __forceinline__ void p2g_group(const P2GItem& it, const DevPtrs& d, c
       for (int n = 0; n < 27; ++n) {
         int tx, ty, tz;
         const bool own = slot_node(n / 9, (n / 3) % 3, n % 3, ci, cj, ck, tx, ty, tz);
-        if (own && v[n] != 0.0f) {
+        if (own && tiled) d.tile_acc[((size_t)e * ACC_CH + ch0 + c) * PT3 + (tx * PT + ty) * PT + tz] = v[n];
+        if (own && !tiled && v[n] != 0.0f) {
           const int gi = B0i + tx, gj = B0j + ty, gk = B0k + tz;
           atomicAdd(d.acc + ((size_t)node_block_of(sc, gi, gj, gk) * ACC_CH + ch0 + c) * BLK3 + node_local(gi, gj, gk),
                     v[n]);
@@ -1015,12 +1065,12 @@ __global__ void __launch_bounds__(64) k_p2g(DevPtrs d, const SceneDev* __restric
     }
     unsigned touched = 0u;
     switch (group) {
-      case 0: p2g_group<0>(it, d, sc, B0i, B0j, B0k, ci, cj, ck, touched); break;
-      case 1: p2g_group<1>(it, d, sc, B0i, B0j, B0k, ci, cj, ck, touched); break;
-      case 2: p2g_group<2>(it, d, sc, B0i, B0j, B0k, ci, cj, ck, touched); break;
-      case 3: p2g_group<3>(it, d, sc, B0i, B0j, B0k, ci, cj, ck, touched); break;
-      case 4: p2g_group<4>(it, d, sc, B0i, B0j, B0k, ci, cj, ck, touched); break;
-      default: p2g_group<5>(it, d, sc, B0i, B0j, B0k, ci, cj, ck, touched); break;
+      case 0: p2g_group<0>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched); break;
+      case 1: p2g_group<1>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched); break;
+      case 2: p2g_group<2>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched); break;
+      case 3: p2g_group<3>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched); break;
+      case 4: p2g_group<4>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched); break;
+      default: p2g_group<5>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched); break;
     }
     if (group == 0) {
       // node blocks receiving mass from this block: OR of the lanes' bits
@@ -1055,9 +1105,31 @@ __global__ void __launch_bounds__(256) k_grid(DevPtrs d) {
     const KnifeStateDev& ks = d.kstate[(size_t)step * d.n_scenes + s];
     int bi, bj, bk;
     block_coords(sc, nb, bi, bj, bk);
-    const int gi = bi * 4 + (lane >> 4), gj = bj * 4 + ((lane >> 2) & 3), gk = bk * 4 + (lane & 3);
+    const int li = lane >> 4, lj = (lane >> 2) & 3, lk = lane & 3;
+    const int gi = bi * 4 + li, gj = bj * 4 + lj, gk = bk * 4 + lk;
     float* acc = d.acc + (size_t)nb * ACC_CH * BLK3;
-    const unsigned long long mfx = d.massfx[(size_t)nb * BLK3 + lane];
+    // this node's sums: atomically scattered part (blocks past tile_cap) plus,
+    // in a fixed order, the merged tiles of the up to 8 particle blocks whose
+    // stencils reach it (the block itself and its lower neighbours)
+    unsigned long long mfx = d.massfx[(size_t)nb * BLK3 + lane];
+    float a[ACC_CH];
+#pragma unroll
+    for (int c = 0; c < ACC_CH; ++c) a[c] = acc[c * BLK3 + lane];
+    for (int q = 0; q < 8; ++q) {
+      const int ox = q >> 2, oy = (q >> 1) & 1, oz = q & 1;
+      const int pi = bi - ox, pj = bj - oy, pk = bk - oz;
+      if (pi < 0 || pj < 0 || pk < 0) continue;
+      const int key = sc.node_block_base + (pi * sc.bdims[1] + pj) * sc.bdims[2] + pk;
+      if (d.count_cur[key] <= 0) continue;
+      const int e = d.list_pos[key];
+      if (e >= d.tile_cap) continue;
+      const int tx = li + 4 * ox, ty = lj + 4 * oy, tz = lk + 4 * oz;
+      if (tx >= PT || ty >= PT || tz >= PT) continue;
+      const int t = (tx * PT + ty) * PT + tz;
+      mfx += d.tile_mass[(size_t)e * PT3 + t];
+#pragma unroll
+      for (int c = 0; c < ACC_CH; ++c) a[c] += d.tile_acc[((size_t)e * ACC_CH + c) * PT3 + t];
+    }
     float out[GV_STRIDE];
 #pragma unroll
     for (int c = 0; c < GV_STRIDE; ++c) out[c] = 0.0f;
@@ -1065,11 +1137,11 @@ __global__ void __launch_bounds__(256) k_grid(DevPtrs d) {
       const float mw = (float)((double)mfx * MASS_FX_INV);  // sum of weights (= m_i / m_p)
       float vold[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) vold[c] = acc[(A_MOM + c) * BLK3 + lane] / mw;
+      for (int c = 0; c < 3; ++c) vold[c] = a[A_MOM + c] / mw;
       float pg[NPIG];
 #pragma unroll
-      for (int c = 0; c < NPIG; ++c) pg[c] = acc[(A_PIG + c) * BLK3 + lane] / mw;
-      const float shear = acc[A_SHR * BLK3 + lane] / mw;
+      for (int c = 0; c < NPIG; ++c) pg[c] = a[A_PIG + c] / mw;
+      const float shear = a[A_SHR] / mw;
       const float ps = ((pg[0] + pg[1]) + pg[2]) + pg[3];
       if (ps > 1e-6f) { pg[0] /= ps; pg[1] /= ps; pg[2] /= ps; pg[3] /= ps; }
       const float mass = mw * sc.particle_mass;
@@ -1077,7 +1149,7 @@ __global__ void __launch_bounds__(256) k_grid(DevPtrs d) {
       float vn[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const float f = acc[(A_FRC + c) * BLK3 + lane];
+        const float f = a[A_FRC + c];
         vn[c] = vold[c] + (((-dtpv) * f) / mass + sc.dt * sc.gravity[c]);
       }
       const float* xs = d.axis_coords + sc.axis_off;

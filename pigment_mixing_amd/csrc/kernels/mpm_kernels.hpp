@@ -52,6 +52,8 @@ struct DevPtrs {
   int* cell_off;           // per (listed block, cell) offsets inside the block, list order
   int* cell_num;           // per (listed block, cell) counts of the current sort, list order
   int4* pdesc;             // per listed block: key, first slot, particle count, scene
+  float* tile_acc;         // per listed block (< tile_cap): merged 6x6x6 node tile, [ACC_CH][216]
+  unsigned long long* tile_mass;  // per listed block (< tile_cap): [216] fixed-point mass
   int* list_pos;           // block key -> position in pb_list (listed blocks only)
   int* count_next;
   int* count_cur;
@@ -74,8 +76,10 @@ struct DevPtrs {
   int n_scenes;
   int nb_list_cap;
   int dead_key;
-  int pad;
+  int tile_cap;            // listed blocks with a tile; later ones scatter with atomics
 };
+constexpr int PT = 6;                  // scatter tile edge (nodes) = 4 bins + 2
+constexpr int PT3 = PT * PT * PT;      // 216
 
 void launch_init_tau(const DevPtrs& d, int64_t n, hipStream_t st);
 void launch_keys(const DevPtrs& d, int64_t n, hipStream_t st);
