@@ -113,8 +113,7 @@ typedef struct {
   float grid_reference_point[3]; /* >= 1e10 disables (simulation.hpp:52)              */
   float dt;                  /* fixed time step                                       */
   float fps;                 /* frame rate for frameFinished (simulation_units.cpp:235) */
-  int   use_musl;            /* simulation.hpp:50; must be 0 (USL): mpmx_create
-                                returns MPMX_ERR_ARG for 1, MUSL has no kernel yet  */
+  int   use_musl;            /* simulation.hpp:50; 0 USL, 1 MUSL (musl.cpp:6-98)    */
   int   use_particle_boundaries;           /* simulation.hpp:55-57                    */
   float particle_boundary_min[3], particle_boundary_max[3];
   int   closure;             /* mpmx_closure                                          */

@@ -305,9 +305,20 @@ int grid_p2g(mpmx_handle* h) { return round8(h->n_cu * h->wg_p2g); }
 // G2P with the exact return map, or with the learned closure (network in LDS:
 // two 256-thread workgroups per CU).
 bool learned(const mpmx_handle* h) { return h->scenes[0].cfg.closure != MPMX_CLOSURE_EXACT; }
+bool musl(const mpmx_handle* h) { return h->scenes[0].cfg.use_musl != 0; }
+// USL: one fused G2P.  MUSL (musl.cpp:6-98): first half, second scatter of
+// momentum + grid velocity, second half.
 void enqueue_g2p(mpmx_handle* h, const DevPtrs& d) {
-  if (learned(h)) launch_g2p(d, round8(h->n_cu * std::min(h->wg_g2p, 2)), h->closure.d_blob, h->stream);
-  else launch_g2p(d, grid_fused(h), nullptr, h->stream);
+  const int g_plain = grid_fused(h);
+  const int g_last = learned(h) ? round8(h->n_cu * std::min(h->wg_g2p, 2)) : g_plain;
+  const void* blob = learned(h) ? h->closure.d_blob : nullptr;
+  if (musl(h)) {
+    launch_g2p(d, g_plain, nullptr, 1, h->stream);
+    launch_musl(d, grid_p2g(h), grid_grid(h), h->stream);
+    launch_g2p(d, g_last, blob, 2, h->stream);
+  } else {
+    launch_g2p(d, g_last, blob, 0, h->stream);
+  }
 }
 
 // One step: G2P (A -> B, counts for the new sort), sort, P2G from B, grid update.
@@ -467,7 +478,8 @@ int mpmx_create_batch(const mpmx_config* cfgs, int n_scenes, mpmx_handle** out) 
   for (int i = 0; i < n_scenes; ++i) {
     if (check_config(&cfgs[i])) return MPMX_ERR_ARG;
     // one clock and one G2P variant for the whole batch
-    if (cfgs[i].dt != cfgs[0].dt || cfgs[i].closure != cfgs[0].closure) return MPMX_ERR_ARG;
+    if (cfgs[i].dt != cfgs[0].dt || cfgs[i].closure != cfgs[0].closure || cfgs[i].use_musl != cfgs[0].use_musl)
+      return MPMX_ERR_ARG;
   }
   const mpmx_config* cfg = cfgs;
   mpmx_handle* h = new mpmx_handle();
@@ -555,9 +567,7 @@ int gather_batch(mpmx_handle* h) {
 int check_config(const mpmx_config* cfg) {
   if (!(cfg->dx > 0) || !(cfg->dt > 0) || !(cfg->ppc > 0) || !(cfg->rho > 0)) return MPMX_ERR_ARG;
   if (cfg->plastic_model != MPMX_PLASTIC_NONE && cfg->plastic_model != MPMX_PLASTIC_VMVISC) return MPMX_ERR_ARG;
-  // MUSL's second scatter (musl.cpp:6-98) has no device kernel: refuse the
-  // flag rather than run a USL step under a MUSL configuration.
-  if (cfg->use_musl != 0) return MPMX_ERR_ARG;
+  if (cfg->use_musl != 0 && cfg->use_musl != 1) return MPMX_ERR_ARG;
   if (cfg->closure != MPMX_CLOSURE_EXACT && cfg->closure != MPMX_CLOSURE_LEARNED &&
       cfg->closure != MPMX_CLOSURE_LEARNED_REFINED)
     return MPMX_ERR_ARG;

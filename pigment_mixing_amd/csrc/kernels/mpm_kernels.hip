@@ -376,7 +376,14 @@ __device__ bool learned_return_map(const SceneDev& sc, G2PLearnedLds& lrn, float
   return true;
 }
 
-template <bool LEARNED>
+// PHASE 0: the fused USL step.  MUSL (musl.cpp:6-98) splits it around a second
+// scatter: PHASE 1 gathers from the updated grid (velocity, APIC B, FLIP,
+// latent, shear), mixes the latent and sets v (AFLIP), and parks the PIC
+// velocity in the output side's position rows; the MUSL scatter and grid
+// normalisation then rebuild the grid velocity from the new v and B; PHASE 2
+// gathers the velocity gradient from it and finishes the step (F update,
+// return map, stress, advection with the parked PIC velocity, sort keys).
+template <bool LEARNED, int PHASE>
 __global__ void __launch_bounds__(256) k_g2p(DevPtrs d, const SceneDev* __restrict__ scenes,
                                              const float* __restrict__ axis, const float* __restrict__ A,
                                              const int* __restrict__ perm, const float* __restrict__ gv,
@@ -446,14 +453,26 @@ __global__ void __launch_bounds__(256) k_g2p(DevPtrs d, const SceneDev* __restri
       int newton = 0;
       {
         const int src = perm[j];
-        float x[3], v[3], F[9], pig[NPIG];
+        float x[3], v[3], F[9], pig[NPIG], picB[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) { x[c] = A[(P_X + c) * cap + src]; v[c] = A[(P_V + c) * cap + src]; }
+        for (int c = 0; c < 3; ++c) x[c] = A[(P_X + c) * cap + src];
+        if constexpr (PHASE == 2) {
+          // v and the PIC velocity come from the first half (output side, slot j)
 #pragma unroll
-        for (int c = 0; c < 9; ++c) F[c] = A[(P_F + c) * cap + src];
+          for (int c = 0; c < 3; ++c) { v[c] = Bf[(P_V + c) * cap + j]; picB[c] = Bf[(P_X + c) * cap + j]; }
+        } else {
 #pragma unroll
-        for (int c = 0; c < NPIG; ++c) pig[c] = A[(P_PIG + c) * cap + src];
-        const float eps_in = A[P_EPS * cap + src];
+          for (int c = 0; c < 3; ++c) { v[c] = A[(P_V + c) * cap + src]; picB[c] = 0.0f; }
+        }
+        if constexpr (PHASE != 1) {
+#pragma unroll
+          for (int c = 0; c < 9; ++c) F[c] = A[(P_F + c) * cap + src];
+        }
+        if constexpr (PHASE != 2) {
+#pragma unroll
+          for (int c = 0; c < NPIG; ++c) pig[c] = A[(P_PIG + c) * cap + src];
+        }
+        const float eps_in = PHASE == 1 ? 0.0f : A[P_EPS * cap + src];
         const int id = d.pa_id[src];
         // ---------------- G2P gather (g2p.cpp:25-91) over the 27-node stencil
         int b[3];
@@ -502,23 +521,31 @@ __global__ void __launch_bounds__(256) k_g2p(DevPtrs d, const SceneDev* __restri
             const float gpg[NPIG] = {n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, n3.x};
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
-              pic[r] += gvv[r] * wgt;
-              Bp[3 * r + 0] += (gvv[r] * pd0) * wgt;
-              Bp[3 * r + 1] += (gvv[r] * pd1) * wgt;
-              Bp[3 * r + 2] += (gvv[r] * pd2) * wgt;
-              fl[r] += gfl[r] * wgt;
+              if constexpr (PHASE != 2) {
+                pic[r] += gvv[r] * wgt;
+                Bp[3 * r + 0] += (gvv[r] * pd0) * wgt;
+                Bp[3 * r + 1] += (gvv[r] * pd1) * wgt;
+                Bp[3 * r + 2] += (gvv[r] * pd2) * wgt;
+                fl[r] += gfl[r] * wgt;
+              }
               L[3 * r + 0] += gvv[r] * g0;
               L[3 * r + 1] += gvv[r] * g1;
               L[3 * r + 2] += gvv[r] * g2;
             }
+            if constexpr (PHASE != 2) {
 #pragma unroll
-            for (int c = 0; c < NPIG; ++c) gp[c] += gpg[c] * wgt;
-            gs += n3.y * wgt;
+              for (int c = 0; c < NPIG; ++c) gp[c] += gpg[c] * wgt;
+              gs += n3.y * wgt;
+            }
           }
         }
         if (sc.flip_ratio >= 0.0f) {
 #pragma unroll
           for (int c = 0; c < 9; ++c) Bp[c] = 0.0f;  // PIC/FLIP: Bmat unused
+        }
+        if constexpr (PHASE == 2) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) pic[c] = picB[c];
         }
         // ---------------- shear intensity and pigment mixing (g2p.cpp:94-130)
         float sn = 0.0f;
@@ -533,10 +560,35 @@ __global__ void __launch_bounds__(256) k_g2p(DevPtrs d, const SceneDev* __restri
         const float tt = fmaxf(0.0f, fminf(1.0f, (gs - sc.D_edge0) / (sc.D_edge1 - sc.D_edge0)));
         const float mixf = tt * tt * (3.0f - 2.0f * tt);
         const float smf = fminf(fmaxf(dynD * mixf * sc.dt, 0.0f), 1.0f);
+        if constexpr (PHASE != 2) {
 #pragma unroll
-        for (int c = 0; c < NPIG; ++c) pig[c] = (1.0f - smf) * pig[c] + smf * gp[c];
-        const float ps = ((pig[0] + pig[1]) + pig[2]) + pig[3];
-        if (ps > 1e-6f) { pig[0] /= ps; pig[1] /= ps; pig[2] /= ps; pig[3] /= ps; }
+          for (int c = 0; c < NPIG; ++c) pig[c] = (1.0f - smf) * pig[c] + smf * gp[c];
+          const float ps = ((pig[0] + pig[1]) + pig[2]) + pig[3];
+          if (ps > 1e-6f) { pig[0] /= ps; pig[1] /= ps; pig[2] /= ps; pig[3] /= ps; }
+        }
+        if constexpr (PHASE == 1) {
+          // first half of MUSL (musl.cpp:13-26): particle velocity update; park
+          // the PIC velocity for the advection of the second half
+          if (valid) {
+            const float a = -sc.flip_ratio;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              float vn;
+              if (sc.flip_ratio < -1.0f) vn = pic[c];
+              else if (sc.flip_ratio < 0.0f) vn = a * (v[c] + fl[c]) + (1.0f - a) * pic[c];
+              else vn = sc.flip_ratio * (v[c] + fl[c]) + (1.0f - sc.flip_ratio) * pic[c];
+              Bf[(P_V + c) * cap + j] = vn;
+              Bf[(P_X + c) * cap + j] = pic[c];
+            }
+#pragma unroll
+            for (int c = 0; c < 9; ++c) Bf[(P_C + c) * cap + j] = Bp[c];
+#pragma unroll
+            for (int c = 0; c < NPIG; ++c) Bf[(P_PIG + c) * cap + j] = pig[c];
+            Bf[P_DIFF * cap + j] = diff_new;
+            d.pb_id[j] = id;
+          }
+          continue;
+        }
         // ---------------- F update + return map (deformation_update.cpp:33-45)
         float Ft[9];
 #pragma unroll
@@ -636,10 +688,14 @@ __global__ void __launch_bounds__(256) k_g2p(DevPtrs d, const SceneDev* __restri
 #pragma unroll
           for (int c = 0; c < 3; ++c) { Bf[(P_X + c) * cap + j] = xn[c]; Bf[(P_V + c) * cap + j] = vn[c]; }
 #pragma unroll
-          for (int c = 0; c < 9; ++c) { Bf[(P_C + c) * cap + j] = Bp[c]; Bf[(P_F + c) * cap + j] = Fn[c]; }
+          for (int c = 0; c < 9; ++c) Bf[(P_F + c) * cap + j] = Fn[c];
+          if constexpr (PHASE == 0) {
 #pragma unroll
-          for (int c = 0; c < NPIG; ++c) Bf[(P_PIG + c) * cap + j] = pig[c];
-          Bf[P_DIFF * cap + j] = diff_new;
+            for (int c = 0; c < 9; ++c) Bf[(P_C + c) * cap + j] = Bp[c];
+#pragma unroll
+            for (int c = 0; c < NPIG; ++c) Bf[(P_PIG + c) * cap + j] = pig[c];
+            Bf[P_DIFF * cap + j] = diff_new;
+          }
           Bf[P_EPS * cap + j] = eps;
           Bf[P_DG * cap + j] = dgr;
 #pragma unroll
@@ -747,6 +803,8 @@ constexpr int P2G_GROUPS = 6;
 // Per-item inputs shared by the channel-group bodies.
 struct P2GItem {
   const float* P;      // particle side holding the sorted step's state
+  const float* PB;     // MUSL: the first half's output side (indexed by sort slot)
+  int jb;              // MUSL: sort slot of this cell's first particle
   int64_t cap;
   const int* pp;       // permutation entries of this lane's cell (in bounds)
   int cnt, kmax;       // particles of this lane's cell, deepest cell of the wave
@@ -755,18 +813,27 @@ struct P2GItem {
 };
 
 // Per-particle inputs of channel group G: position, then the group's payload.
+// Group 6 is MUSL's second scatter of momentum (musl.cpp:31-87): position from
+// the step's input side (through the permutation), v and B from the first
+// half's output (slot j of the sort).
 template <int G>
 struct P2GLoad {
-  static constexpr int NQ = G == 0 ? 0 : (G == 1 ? 12 : (G == 2 ? 6 : 3));
+  static constexpr int NQ = G == 0 ? 0 : (G == 1 || G == 6 ? 12 : (G == 2 ? 6 : 3));
   float x[3];
   float q[NQ > 0 ? NQ : 1];
 };
 
 template <int G>
-__device__ __forceinline__ void p2g_load(const float* __restrict__ P, int64_t cap, int src, P2GLoad<G>& o) {
+__device__ __forceinline__ void p2g_load(const float* __restrict__ P, const float* __restrict__ PB, int64_t cap,
+                                         int src, int j, P2GLoad<G>& o) {
 #pragma unroll
   for (int c = 0; c < 3; ++c) o.x[c] = P[(P_X + c) * cap + src];
-  if constexpr (G == 1) {
+  if constexpr (G == 6) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o.q[c] = PB[(P_V + c) * cap + j];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) o.q[3 + c] = PB[(P_C + c) * cap + j];
+  } else if constexpr (G == 1) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) o.q[c] = P[(P_V + c) * cap + src];
 #pragma unroll
@@ -803,12 +870,12 @@ __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[2
   const float inv_dx = it.inv_dx;
   const int klast = max(it.cnt - 1, 0);
   P2GLoad<G> cur;
-  p2g_load<G>(P, cap, it.pp[0], cur);
+  p2g_load<G>(P, it.PB, cap, it.pp[0], it.jb, cur);
   int src_next = it.pp[min(1, klast)];
   for (int k = 0; k < it.kmax; ++k) {
     const bool act = k < it.cnt;
     P2GLoad<G> nxt;
-    p2g_load<G>(P, cap, src_next, nxt);      // next particle (clamped to the last one)
+    p2g_load<G>(P, it.PB, cap, src_next, it.jb + min(k + 1, klast), nxt);   // next particle (clamped)
     src_next = it.pp[min(k + 2, klast)];
     const float x0 = act ? cur.x[0] : it.X[0][1];
     const float x1 = act ? cur.x[1] : it.X[1][1];
@@ -835,7 +902,7 @@ __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[2
       float NyNz[9];
 #pragma unroll
       for (int p = 0; p < 9; ++p) NyNz[p] = N[3 + p / 3] * N[6 + p % 3];
-      if constexpr (G == 1) {
+      if constexpr (G == 1 || G == 6) {
         // v + C (X - x) = [v + C_.x (X_oi - x)] + C_.y (Y_oj - y) + C_.z (Z_ok - z)
         float ax[3][3], ay[3][3], az[3][3];   // [offset][component]
 #pragma unroll
@@ -995,7 +1062,7 @@ __device__ __forceinline__ void p2g_group(const P2GItem& it, const DevPtrs& d, c
     }
   } else {
     // group 5's channels are latent 6 and shear (A_PIG + 6, A_SHR = A_PIG + 7)
-    constexpr int ch0 = G == 1 ? A_MOM : (G == 2 ? A_FRC : (G == 3 ? A_PIG : (G == 4 ? A_PIG + 3 : A_PIG + 6)));
+    constexpr int ch0 = (G == 1 || G == 6) ? A_MOM : (G == 2 ? A_FRC : (G == 3 ? A_PIG : (G == 4 ? A_PIG + 3 : A_PIG + 6)));
     constexpr int nch = G == 5 ? 2 : 3;
 #pragma unroll
     for (int c = 0; c < nch; ++c) {
@@ -1018,10 +1085,14 @@ __device__ __forceinline__ void p2g_group(const P2GItem& it, const DevPtrs& d, c
   }
 }
 
+// MUSL: only the momentum of group 6 (MUSL's second scatter), no activation.
+template <bool MUSL>
 __global__ void __launch_bounds__(64) k_p2g(DevPtrs d, const SceneDev* __restrict__ scenes,
                                             const float* __restrict__ axis, const float* __restrict__ P,
                                             const int* __restrict__ perm, const int4* __restrict__ pdesc,
-                                            const int* __restrict__ cell_off, const int* __restrict__ cell_num) {
+                                            const int* __restrict__ cell_off, const int* __restrict__ cell_num,
+                                            const float* __restrict__ PB) {
+  constexpr int NG = MUSL ? 1 : P2G_GROUPS;
   const int nblk = d.counters[CNT_PBLOCKS];
   const int parity = d.counters[CNT_GSTEP] & 1;
   const int lane = threadIdx.x;
@@ -1032,10 +1103,10 @@ __global__ void __launch_bounds__(64) k_p2g(DevPtrs d, const SceneDev* __restric
   const int per = gridDim.x / nx;
   const int seg = (nblk + nx - 1) / nx;
   const int e0 = xcd * seg;
-  const int nitems = max(0, min(nblk, e0 + seg) - e0) * P2G_GROUPS;
+  const int nitems = max(0, min(nblk, e0 + seg) - e0) * NG;
   for (int item = blockIdx.x / nx; item < nitems; item += per) {
-    const int e = e0 + item / P2G_GROUPS;
-    const int group = item % P2G_GROUPS;
+    const int e = e0 + item / NG;
+    const int group = MUSL ? 6 : item % NG;
     // descriptor (key, first slot, count, scene) and this lane's cell, all by list position
     const int4 ds = pdesc[e];
     const int key = ds.x;
@@ -1051,6 +1122,8 @@ __global__ void __launch_bounds__(64) k_p2g(DevPtrs d, const SceneDev* __restric
     it.inv_dx = sc.one_over_dx;
     it.apicDinv = sc.apicDinv;
     it.pp = perm + ds.y + (it.cnt > 0 ? coff : 0);
+    it.PB = PB;
+    it.jb = ds.y + (it.cnt > 0 ? coff : 0);
     int kmax = it.cnt;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) kmax = max(kmax, __shfl_xor(kmax, off));
@@ -1070,7 +1143,8 @@ __global__ void __launch_bounds__(64) k_p2g(DevPtrs d, const SceneDev* __restric
       case 2: p2g_group<2>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched); break;
       case 3: p2g_group<3>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched); break;
       case 4: p2g_group<4>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched); break;
-      default: p2g_group<5>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched); break;
+      case 5: p2g_group<5>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched); break;
+      default: p2g_group<6>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched); break;
     }
     if (group == 0) {
       // node blocks receiving mass from this block: OR of the lanes' bits
@@ -1179,6 +1253,56 @@ __global__ void __launch_bounds__(256) k_grid(DevPtrs d) {
   if (threadIdx.x == 0 && active_wg) atomicAdd(&d.stats[ST_ACTIVE_NODES], (unsigned long long)active_wg);
 }
 
+// ------------------------------------------------------------------ k_grid_musl
+// MUSL grid velocity (musl.cpp:90-96): the second scatter's momentum (merged
+// tiles of the listed blocks in a fixed order + the atomic remainder), divided
+// by m_i / m_p with m_i the step's node mass; no boundary condition.  Runs over
+// the node blocks the grid update of this step processed.
+__global__ void __launch_bounds__(256) k_grid_musl(DevPtrs d) {
+  const int parity = d.counters[CNT_GSTEP] & 1;
+  const int nblk = d.counters[CNT_NBLOCKS0 + parity];
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int nwaves = (gridDim.x * blockDim.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  for (int e = wave; e < nblk; e += nwaves) {
+    const int nb = d.nb_list[parity * d.nb_list_cap + e];
+    const SceneDev& sc = d.scenes[d.blk_scene[nb]];
+    int bi, bj, bk;
+    block_coords(sc, nb, bi, bj, bk);
+    const int li = lane >> 4, lj = (lane >> 2) & 3, lk = lane & 3;
+    float* acc = d.acc + (size_t)nb * ACC_CH * BLK3;
+    float mom[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) mom[c] = acc[(A_MOM + c) * BLK3 + lane];
+    for (int q = 0; q < 8; ++q) {
+      const int ox = q >> 2, oy = (q >> 1) & 1, oz = q & 1;
+      const int pi = bi - ox, pj = bj - oy, pk = bk - oz;
+      if (pi < 0 || pj < 0 || pk < 0) continue;
+      const int key = sc.node_block_base + (pi * sc.bdims[1] + pj) * sc.bdims[2] + pk;
+      if (d.count_cur[key] <= 0) continue;
+      const int te = d.list_pos[key];
+      if (te >= d.tile_cap) continue;
+      const int tx = li + 4 * ox, ty = lj + 4 * oy, tz = lk + 4 * oz;
+      if (tx >= PT || ty >= PT || tz >= PT) continue;
+      const int t = (tx * PT + ty) * PT + tz;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) mom[c] += d.tile_acc[((size_t)te * ACC_CH + A_MOM + c) * PT3 + t];
+    }
+    float* g = d.gv + ((size_t)nb * BLK3 + lane) * GV_STRIDE;
+    const float mi = g[G_MASS];
+    if (mi > 0.0f) {
+      const float r = mi / sc.particle_mass;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) g[G_V + c] = mom[c] / r;
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) g[G_V + c] = 0.0f;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[(A_MOM + c) * BLK3 + lane] = 0.0f;
+  }
+}
+
 // --------------------------------------------------------------- downloads
 __global__ void k_unsort(DevPtrs d, int64_t n, float* out, int ncomp_first, int ncomp) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1256,16 +1380,25 @@ void launch_cellscan(const DevPtrs& d, int grid, hipStream_t st) {
 void launch_perm(const DevPtrs& d, int64_t n, hipStream_t st) {
   if (n > 0) hipLaunchKernelGGL(k_perm, dim3(blocks_for(n, 256)), dim3(256), 0, st, d, n);
 }
-void launch_g2p(const DevPtrs& d, int grid, const void* mlp_blob, hipStream_t st) {
-  if (mlp_blob)
-    hipLaunchKernelGGL(k_g2p<true>, dim3(grid), dim3(256), 0, st, d, (const SceneDev*)d.scenes,
-                       (const float*)d.axis_coords, (const float*)d.pa, (const int*)d.perm, (const float*)d.gv,
-                       (const uint8_t*)mlp_blob, (const int4*)d.pdesc);
+void launch_g2p(const DevPtrs& d, int grid, const void* mlp_blob, int phase, hipStream_t st) {
+  const SceneDev* sc = d.scenes;
+  const float* ax = d.axis_coords;
+  const float* pa = d.pa;
+  const int* pm = d.perm;
+  const float* gv = d.gv;
+  const uint8_t* bl = (const uint8_t*)mlp_blob;
+  const int4* pd = d.pdesc;
+  if (phase == 1)
+    hipLaunchKernelGGL((k_g2p<false, 1>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, bl, pd);
+  else if (phase == 2 && mlp_blob)
+    hipLaunchKernelGGL((k_g2p<true, 2>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, bl, pd);
+  else if (phase == 2)
+    hipLaunchKernelGGL((k_g2p<false, 2>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, bl, pd);
+  else if (mlp_blob)
+    hipLaunchKernelGGL((k_g2p<true, 0>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, bl, pd);
   else
-    hipLaunchKernelGGL(k_g2p<false>, dim3(grid), dim3(256), 0, st, d, (const SceneDev*)d.scenes,
-                       (const float*)d.axis_coords, (const float*)d.pa, (const int*)d.perm, (const float*)d.gv,
-                       (const uint8_t*)nullptr, (const int4*)d.pdesc);
-  hipLaunchKernelGGL(k_copy_dead, dim3(64), dim3(256), 0, st, d);
+    hipLaunchKernelGGL((k_g2p<false, 0>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, bl, pd);
+  if (phase != 1) hipLaunchKernelGGL(k_copy_dead, dim3(64), dim3(256), 0, st, d);
 }
 void launch_closure_eval(const SceneDev* scenes, const void* mlp_blob, const float* trial4, int64_t n,
                          float* learned4, float* exact4, uint8_t* accepted, int max_grid, hipStream_t st) {
@@ -1275,9 +1408,15 @@ void launch_closure_eval(const SceneDev* scenes, const void* mlp_blob, const flo
                      trial4, n, learned4, exact4, accepted);
 }
 void launch_p2g(const DevPtrs& d, int grid, int src_is_b, hipStream_t st) {
-  hipLaunchKernelGGL(k_p2g, dim3(grid), dim3(64), 0, st, d, (const SceneDev*)d.scenes, (const float*)d.axis_coords,
-                     (const float*)(src_is_b ? d.pb : d.pa), (const int*)d.perm, (const int4*)d.pdesc,
-                     (const int*)d.cell_off, (const int*)d.cell_num);
+  hipLaunchKernelGGL(k_p2g<false>, dim3(grid), dim3(64), 0, st, d, (const SceneDev*)d.scenes,
+                     (const float*)d.axis_coords, (const float*)(src_is_b ? d.pb : d.pa), (const int*)d.perm,
+                     (const int4*)d.pdesc, (const int*)d.cell_off, (const int*)d.cell_num, (const float*)nullptr);
+}
+void launch_musl(const DevPtrs& d, int grid_p2g, int grid_grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_p2g<true>, dim3(grid_p2g), dim3(64), 0, st, d, (const SceneDev*)d.scenes,
+                     (const float*)d.axis_coords, (const float*)d.pa, (const int*)d.perm, (const int4*)d.pdesc,
+                     (const int*)d.cell_off, (const int*)d.cell_num, (const float*)d.pb);
+  hipLaunchKernelGGL(k_grid_musl, dim3(grid_grid), dim3(256), 0, st, d);
 }
 void launch_grid(const DevPtrs& d, int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_grid, dim3(grid), dim3(256), 0, st, d);

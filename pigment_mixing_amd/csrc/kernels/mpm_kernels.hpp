@@ -87,7 +87,10 @@ void launch_scan(const DevPtrs& d, int nkeys, int advance, hipStream_t st);
 int scan_chunks(int nkeys);
 void launch_cellscan(const DevPtrs& d, int grid, hipStream_t st);
 void launch_perm(const DevPtrs& d, int64_t n, hipStream_t st);
-void launch_g2p(const DevPtrs& d, int grid, const void* mlp_blob, hipStream_t st);
+// phase 0: fused USL G2P (+ dead-particle copy); MUSL: phase 1 (first half),
+// launch_musl (second scatter + grid velocity), phase 2 (second half + copy).
+void launch_g2p(const DevPtrs& d, int grid, const void* mlp_blob, int phase, hipStream_t st);
+void launch_musl(const DevPtrs& d, int grid_p2g, int grid_grid, hipStream_t st);
 void launch_closure_eval(const SceneDev* scenes, const void* mlp_blob, const float* trial4, int64_t n,
                          float* learned4, float* exact4, uint8_t* accepted, int max_grid, hipStream_t st);
 void launch_p2g(const DevPtrs& d, int grid, int src_is_b, hipStream_t st);

@@ -167,6 +167,22 @@ def test_parity_paint_2blob_100_steps():
     np.testing.assert_allclose(gg["mass"], go["mass"], rtol=1e-4, atol=1e-4 * mp)
 
 
+def test_parity_musl_small_knife_100_steps():
+    """MUSL (musl.cpp:6-98, use_musl = 1): G2P in two halves around the second
+    scatter of v + B, deformation from its grid velocity."""
+    s = small()
+    s["cfg"].use_musl = 1
+    h, o = run_both(s, 100)
+    check_particles(h.download(), o.download(), scenes.DX, "small/musl/100", v_tol=1e-3)
+
+
+def test_parity_musl_paint_2blob_100_steps():
+    s = scenes.paint_2blob()
+    s["cfg"].use_musl = 1
+    h, o = run_both(s, 100)
+    check_particles(h.download(), o.download(), scenes.DX, "paint_2blob/musl/100")
+
+
 def test_grid_mass_independent_of_particle_and_atomic_order():
     """Same particles uploaded in two different orders: the first scatter runs
     with different lane assignments and atomic orders, yet node masses are

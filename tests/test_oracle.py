@@ -309,3 +309,46 @@ def test_coulomb_friction_incline():
     ref = 0.5 * 9.81 * t * t * (math.sin(th) - mu * math.cos(th))
     got = (out["x"][:, 0] - x[:, 0]).mean()
     assert abs(got - ref) / ref < 2e-2, (got, ref)
+
+
+def test_musl_free_fall_translation_and_rigid_rotation():
+    """MUSL (musl.cpp:6-98) leaves affine velocity fields and free fall exact:
+    the second scatter of v + B (x_i - x_p) reproduces the same affine grid
+    field, so F stays I under translation, the rigid-rotation field keeps its
+    velocities, and free fall adds dt*g per step as under USL."""
+    dt = 1e-5
+    s = _mini_scene()
+    s["plates"] = []
+    s["cfg"].use_musl = 1
+    o = orc.from_scene(s)
+    x0 = s["x"].astype(np.float64)
+    o.step(1)
+    out = o.download()
+    np.testing.assert_allclose(out["v"][:, 1], -9.81 * dt, rtol=1e-4)
+    np.testing.assert_allclose(out["x"] - x0, np.broadcast_to([0, -9.81 * dt * dt, 0], x0.shape), atol=2e-9)
+    s = _mini_scene(g=0.0)
+    s["plates"] = []
+    s["cfg"].use_musl = 1
+    o = orc.Oracle(s["cfg"])
+    v = np.broadcast_to(np.array([0.3, 0.1, 0.2], np.float32), s["x"].shape).copy()
+    o.upload(s["x"], v=v, pig=s["pig"])
+    o.prepare()
+    o.step(2)
+    out = o.download()
+    np.testing.assert_allclose(out["v"], v, rtol=1e-5)
+    np.testing.assert_allclose(out["F"], np.broadcast_to(np.eye(3), out["F"].shape), atol=1e-6)
+    # MUSL changes the deformation rate relative to USL in a sheared flow
+    s = _mini_scene(g=0.0)
+    s["plates"] = []
+    res = []
+    for musl in (0, 1):
+        s["cfg"].use_musl = musl
+        o = orc.Oracle(s["cfg"])
+        xc = s["x"] - s["x"].mean(0)
+        vs = np.zeros_like(s["x"])
+        vs[:, 0] = 20.0 * xc[:, 1] ** 2      # non-affine: APIC cannot carry it exactly
+        o.upload(s["x"], v=vs.astype(np.float32), pig=s["pig"])
+        o.prepare()
+        o.step(2)
+        res.append(o.download()["F"])
+    assert np.abs(res[0] - res[1]).max() > 0
