@@ -57,6 +57,14 @@ def step_bytes(np_live, n_active):
     return 356.0 * np_live + 232.0 * n_active
 
 
+# Learned closure, per queued candidate: G2P writes a 48-byte record instead of
+# F, eps, dgamma and stress (68 bytes); k_closure reads the record and writes
+# those 68 bytes.  Network flops per candidate: 2 * (8*64 + 3*64*64 + 64*4).
+CLOSURE_BYTES = 48.0 + 68.0
+CLOSURE_FLOPS = 2.0 * (8 * 64 + 3 * 64 * 64 + 64 * 4)
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
 def cpu_baseline(scene, steps=2, threads=None):
     """Reference-structured CPU step (oracle/_build/liboracle_fast.so) on the same scene."""
     from tests import oracle_py as orc
@@ -92,10 +100,18 @@ def kernel_report(kt):
     n_act = kt.active_nodes_per_step
     kb = kernel_bytes(kt.particles_per_step, n_act)
     kms = {"k_g2p": kt.ms_g2p, "k_p2g": kt.ms_p2g, "k_grid": kt.ms_grid}
+    q = kt.queued_per_step if kt.ms_closure > 0 else 0.0
+    kb["k_g2p"] -= 20.0 * q
     kgbs = {k: kb[k] / (kms[k] * 1e-3) / 1e9 for k in kb}
     dom = max(kms, key=kms.get)
-    step_gbs = step_bytes(kt.particles_per_step, n_act) / (kt.ms_step * 1e-3) / 1e9
-    return {
+    step_gbs = (step_bytes(kt.particles_per_step, n_act) + 96.0 * q) / (kt.ms_step * 1e-3) / 1e9
+    closure = None
+    if kt.ms_closure > 0:
+        tfl = q * CLOSURE_FLOPS / (kt.ms_closure * 1e-3) / 1e12
+        closure = {"ms": kt.ms_closure, "candidates": q,
+                   "GB/s": q * CLOSURE_BYTES / (kt.ms_closure * 1e-3) / 1e9,
+                   "mfma_TFLOP/s": tfl, "mfma_frac_bf16_dense": tfl / MFMA_BF16_PEAK_TFLOPS}
+    rep = {
         "bound": "hbm",
         "kernel": dom,
         "achieved": kgbs[dom],
@@ -112,6 +128,9 @@ def kernel_report(kt):
         "step_frac": step_gbs / HBM_PEAK_GBS,
         "active_nodes": n_act,
     }
+    if closure:
+        rep["k_closure"] = closure
+    return rep
 
 
 def principal_tau(hp, E, nu):
@@ -190,7 +209,8 @@ def main():
           f"{err_max}", file=sys.stderr, flush=True)
     kt = hl.profile(args.steps)
     rf = kernel_report(kt)
-    print(f"[bench] learned kernels ms: g2p={kt.ms_g2p:.4f} sort={kt.ms_sort:.4f} p2g={kt.ms_p2g:.4f} "
+    print(f"[bench] learned kernels ms: g2p={kt.ms_g2p:.4f} closure={kt.ms_closure:.4f} "
+          f"({kt.queued_per_step:.0f} queued) sort={kt.ms_sort:.4f} p2g={kt.ms_p2g:.4f} "
           f"grid={kt.ms_grid:.4f} step={kt.ms_step:.4f} active_nodes={kt.active_nodes_per_step:.0f}",
           file=sys.stderr, flush=True)
     st_last = hl.stats()

@@ -299,6 +299,8 @@ typedef struct {
   int64_t steps;
   double particles_per_step; /* live particles averaged over the steps */
   double active_nodes_per_step;
+  float ms_closure;          /* learned closure over the queued candidates (k_closure), 0 without it */
+  double queued_per_step;    /* plastic candidates queued for the learned closure, averaged */
 } mpmx_kernel_times;
 
 /* Advance nsteps without the HIP graph, timing every kernel with events

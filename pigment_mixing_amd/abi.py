@@ -95,6 +95,7 @@ class KernelTimes(ctypes.Structure):
         ("ms_g2p", ctypes.c_float), ("ms_sort", ctypes.c_float), ("ms_p2g", ctypes.c_float),
         ("ms_grid", ctypes.c_float), ("ms_step", ctypes.c_float), ("steps", ctypes.c_int64),
         ("particles_per_step", ctypes.c_double), ("active_nodes_per_step", ctypes.c_double),
+        ("ms_closure", ctypes.c_float), ("queued_per_step", ctypes.c_double),
     ]
 
 

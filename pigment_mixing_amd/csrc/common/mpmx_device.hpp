@@ -347,11 +347,33 @@ MPMX_HD uint32_t boundary_collision(const SceneDev& sc, const KnifeStateDev& ks,
   return 0u;
 }
 
+// Hardware reciprocal and square root on the device (v_rcp_f32, v_sqrt_f32,
+// about 1 ulp); IEEE forms on the host.
+MPMX_HD float rcp_approx(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
+}
+MPMX_HD float sqrt_approx(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_amdgcn_sqrtf(x);
+#else
+  return sqrtf(x);
+#endif
+}
+
 // ---------------------------------------------------------------- 3x3 SVD
 // One-sided (Hestenes) Jacobi: rotate the columns of G = F V pairwise until they
 // are orthogonal; sigma_i = |g_i|, U_i = g_i / sigma_i.  Singular values >= 0,
 // sorted descending; U, V orthogonal (det may be -1), as Eigen's JacobiSVD
 // returns them.  Fixed sweep count (branch-uniform across a wavefront).
+// The rotation's tangent uses the hardware reciprocal and square root: a
+// slightly inexact angle only leaves a rounding-level off-diagonal that the
+// next rotation of the pair removes.  The cosine stays correctly rounded, so
+// (cs, sn = cs t) preserves the column norms -- the singular values near 1,
+// whose logarithms are the (small) Hencky strains, are that sensitive.
 MPMX_HD void svd3(const float F[9], float U[9], float sig[3], float V[9]) {
   float g[9];
 #pragma unroll
@@ -367,9 +389,9 @@ MPMX_HD void svd3(const float F[9], float U[9], float sig[3], float V[9]) {
       float b = g[q] * g[q] + g[3 + q] * g[3 + q] + g[6 + q] * g[6 + q];
       float c = g[p] * g[q] + g[3 + p] * g[3 + q] + g[6 + p] * g[6 + q];
       float cs = 1.0f, sn = 0.0f;
-      if (fabsf(c) > 1e-30f * sqrtf(a * b) && c != 0.0f) {
-        float zeta = (b - a) / (2.0f * c);
-        float t = copysignf(1.0f, zeta) / (fabsf(zeta) + sqrtf(1.0f + zeta * zeta));
+      if (fabsf(c) > 1e-30f * sqrt_approx(a * b) && c != 0.0f) {
+        float zeta = (b - a) * rcp_approx(2.0f * c);
+        float t = copysignf(rcp_approx(fabsf(zeta) + sqrt_approx(1.0f + zeta * zeta)), zeta);
         cs = 1.0f / sqrtf(1.0f + t * t);
         sn = cs * t;
       }

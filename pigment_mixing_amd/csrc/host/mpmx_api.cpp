@@ -301,24 +301,28 @@ int grid_fused(mpmx_handle* h) { return round8(h->n_cu * h->wg_g2p); }
 int grid_grid(mpmx_handle* h) { return round8(h->n_cu * h->wg_grid); }
 // P2G runs one-wavefront workgroups
 int grid_p2g(mpmx_handle* h) { return round8(h->n_cu * h->wg_p2g); }
+// the closure kernel (network in LDS, activations in registers) runs three
+// 256-thread workgroups per CU (its registers allow three waves per SIMD)
+int grid_closure(mpmx_handle* h) { return round8(h->n_cu * 3); }
 
-// G2P with the exact return map, or with the learned closure (network in LDS:
-// two 256-thread workgroups per CU).
+// G2P with the exact return map, or with the learned closure: G2P queues the
+// plastic candidates and k_closure maps them.
 bool learned(const mpmx_handle* h) { return h->scenes[0].cfg.closure != MPMX_CLOSURE_EXACT; }
 bool musl(const mpmx_handle* h) { return h->scenes[0].cfg.use_musl != 0; }
 // USL: one fused G2P.  MUSL (musl.cpp:6-98): first half, second scatter of
 // momentum + grid velocity, second half.
-void enqueue_g2p(mpmx_handle* h, const DevPtrs& d) {
-  const int g_plain = grid_fused(h);
-  const int g_last = learned(h) ? round8(h->n_cu * std::min(h->wg_g2p, 2)) : g_plain;
-  const void* blob = learned(h) ? h->closure.d_blob : nullptr;
+void enqueue_g2p(mpmx_handle* h, const DevPtrs& d, hipEvent_t before_closure = nullptr) {
+  const int g = grid_fused(h);
+  const int lrn = learned(h) ? 1 : 0;
   if (musl(h)) {
-    launch_g2p(d, g_plain, nullptr, 1, h->stream);
+    launch_g2p(d, g, 0, 1, h->stream);
     launch_musl(d, grid_p2g(h), grid_grid(h), h->stream);
-    launch_g2p(d, g_last, blob, 2, h->stream);
+    launch_g2p(d, g, lrn, 2, h->stream);
   } else {
-    launch_g2p(d, g_last, blob, 0, h->stream);
+    launch_g2p(d, g, lrn, 0, h->stream);
   }
+  if (before_closure) (void)hipEventRecord(before_closure, h->stream);
+  if (lrn) launch_closure(d, h->closure.d_blob, grid_closure(h), h->stream);
 }
 
 // One step: G2P (A -> B, counts for the new sort), sort, P2G from B, grid update.
@@ -741,6 +745,12 @@ int mpmx_prepare(mpmx_handle* h) {
   ALLOC(d.tile_acc, float, (size_t)d.tile_cap * ACC_CH * PT3);
   ALLOC(d.tile_mass, unsigned long long, (size_t)d.tile_cap * PT3);
   ALLOC(d.list_pos, int, h->nkeys);
+  if (learned(h)) {
+    // closure queue: every live particle can be a candidate in one step
+    ALLOC(d.cq, float4, (size_t)CQ_REC * n);
+    d.cq_cap = n;
+    ALLOC(d.cq_cnt, int, CQ_SEGS * CQ_CNT_STRIDE);
+  }
   ALLOC(d.count_next, int, h->nkeys);
   ALLOC(d.count_cur, int, h->nkeys);
   ALLOC(d.offsets, int, h->nkeys);
@@ -1110,11 +1120,12 @@ int mpmx_profile_steps(mpmx_handle* h, int nsteps, mpmx_kernel_times* out) {
   int rc = read_stats(h);
   if (rc) return rc;
   const unsigned long long nodes0 = h->h_stats[ST_ACTIVE_NODES];
+  const unsigned long long queued0 = h->h_stats[ST_QUEUED];
   hipEvent_t ev[6];
   for (auto& e : ev) HIPCHK(h, hipEventCreate(&e));
-  double t_f = 0, t_s = 0, t_p = 0, t_g = 0, t_all = 0, live = 0;
-  // t_f: k_g2p (+ dead copy), t_s: sort (scans, cell scan, permutation),
-  // t_p: k_p2g, t_g: k_grid
+  double t_f = 0, t_c = 0, t_s = 0, t_p = 0, t_g = 0, t_all = 0, live = 0;
+  // t_f: k_g2p (+ dead copy, MUSL halves), t_c: k_closure, t_s: sort (scans,
+  // cell scan, permutation), t_p: k_p2g, t_g: k_grid
   int done = 0;
   while (done < nsteps) {
     const int chunk = std::min(nsteps - done, MAX_CHUNK);
@@ -1122,7 +1133,7 @@ int mpmx_profile_steps(mpmx_handle* h, int nsteps, mpmx_kernel_times* out) {
     for (int i = 0; i < chunk; ++i) {
       const DevPtrs d = current(h);
       HIPCHK(h, hipEventRecord(ev[0], h->stream));
-      enqueue_g2p(h, d);
+      enqueue_g2p(h, d, ev[5]);
       HIPCHK(h, hipEventRecord(ev[1], h->stream));
       launch_scan(d, h->nkeys, 1, h->stream);
       launch_cellscan(d, grid_grid(h), h->stream);
@@ -1133,13 +1144,14 @@ int mpmx_profile_steps(mpmx_handle* h, int nsteps, mpmx_kernel_times* out) {
       launch_grid(d, grid_grid(h), h->stream);
       HIPCHK(h, hipEventRecord(ev[4], h->stream));
       HIPCHK(h, hipEventSynchronize(ev[4]));
-      float a, b, c, e, all;
-      HIPCHK(h, hipEventElapsedTime(&a, ev[0], ev[1]));
+      float a, b, c, e, all, cl;
+      HIPCHK(h, hipEventElapsedTime(&a, ev[0], ev[5]));
+      HIPCHK(h, hipEventElapsedTime(&cl, ev[5], ev[1]));
       HIPCHK(h, hipEventElapsedTime(&b, ev[1], ev[2]));
       HIPCHK(h, hipEventElapsedTime(&c, ev[2], ev[3]));
       HIPCHK(h, hipEventElapsedTime(&e, ev[3], ev[4]));
       HIPCHK(h, hipEventElapsedTime(&all, ev[0], ev[4]));
-      t_f += a; t_s += b; t_p += c; t_g += e; t_all += all;
+      t_f += a; t_c += cl; t_s += b; t_p += c; t_g += e; t_all += all;
       h->parity_swapped = !h->parity_swapped;
       h->time_f += h->scenes[0].cfg.dt;
       h->step += 1;
@@ -1152,12 +1164,14 @@ int mpmx_profile_steps(mpmx_handle* h, int nsteps, mpmx_kernel_times* out) {
   if ((rc = read_stats(h)) != 0) return rc;
   out->steps = nsteps;
   out->ms_g2p = (float)(t_f / nsteps);
+  out->ms_closure = learned(h) ? (float)(t_c / nsteps) : 0.0f;
   out->ms_sort = (float)(t_s / nsteps);
   out->ms_p2g = (float)(t_p / nsteps);
   out->ms_grid = (float)(t_g / nsteps);
   out->ms_step = (float)(t_all / nsteps);
   out->particles_per_step = live / nsteps;
   out->active_nodes_per_step = (double)(h->h_stats[ST_ACTIVE_NODES] - nodes0) / nsteps;
+  out->queued_per_step = (double)(h->h_stats[ST_QUEUED] - queued0) / nsteps;
   for (int i = 0; i < ST_N; ++i) h->stats_prev[i] = h->h_stats[i];
   h->time = h->time_f;
   return MPMX_OK;
@@ -1167,8 +1181,7 @@ int mpmx_profile_steps(mpmx_handle* h, int nsteps, mpmx_kernel_times* out) {
 
 // ---------------------------------------------------------- launch-shape autotuner
 // Coordinate descent over workgroups per CU: P2G (one-wavefront workgroups),
-// then G2P (256-thread workgroups; the learned variant fits at most two per
-// CU), then the grid update.  Each candidate is timed with HIP events over
+// then G2P (256-thread workgroups), then the grid update.  Each candidate is timed with HIP events over
 // `steps_per_trial` ordinary steps (the simulation advances; the launch shape
 // changes no result beyond FP32 atomic order), and the fastest is kept.  The
 // step graph is rebuilt with the chosen shapes at the next mpmx_step.
@@ -1176,7 +1189,7 @@ int mpmx_autotune(mpmx_handle* h, int steps_per_trial, mpmx_tune_report* rep) {
   if (!h || steps_per_trial <= 0) return MPMX_ERR_ARG;
   if (!h->prepared) return fail(h, MPMX_ERR_STATE, "autotune: prepare first");
   const std::vector<int> cand_p2g = {8, 12, 16, 20, 24, 32};
-  const std::vector<int> cand_g2p = learned(h) ? std::vector<int>{1, 2} : std::vector<int>{2, 3, 4, 6, 8};
+  const std::vector<int> cand_g2p = {2, 3, 4, 6, 8};
   const std::vector<int> cand_grid = {2, 4, 8};
   int trials = 0;
   mpmx_kernel_times kt;

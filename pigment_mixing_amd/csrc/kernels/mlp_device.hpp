@@ -14,6 +14,12 @@
 // Rounding points (the CPU oracle of the network rounds at the same places):
 // normalised inputs -> BF16; weights -> BF16; each hidden layer's
 // SiLU(Z + b) -> BF16; the output layer stays FP32.
+//
+// Inference (mlp_wave_infer) runs the transposed product Z^T = W A^T instead:
+// W is then the A operand (same loads), the activations the B operand, and the
+// D tile of output block ob / sample block sb holds, per lane, four consecutive
+// outputs of one sample -- exactly the B operand of k-block ob of the next
+// layer.  The activations therefore stay in registers from layer to layer.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -37,11 +43,10 @@ constexpr int MLP_NORM_OFF = MLP_B_OFF + (4 * 64 + 16) * 4;
 constexpr int MLP_BLOB_BYTES = MLP_NORM_OFF + 24 * 4;
 static_assert(MLP_W1_OFF % 16 == 0 && MLP_W4_OFF % 16 == 0 && MLP_B_OFF % 16 == 0, "blob alignment");
 
-// Per-wavefront activation buffer: 64 rows x MLP_S bf16 (8704 B).
-constexpr int MLP_ACT_BYTES = 64 * MLP_S * 2;
-
 typedef short mlp_bf16x4 __attribute__((ext_vector_type(4)));
 typedef float mlp_f32x4 __attribute__((ext_vector_type(4)));
+typedef float mlp_f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 mlp_bf16x2v __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
   // round to nearest even (inputs here are finite)
@@ -50,15 +55,26 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
   return (uint16_t)(u >> 16);
 }
 __device__ __forceinline__ float bf16_to_f32(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+// two floats -> packed BF16 pair (v_cvt_pk_bf16_f32: round to nearest even), a in the low half
+__device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
+  const mlp_f32x2 v = {a, b};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, mlp_bf16x2v));
+}
 
 __device__ __forceinline__ mlp_bf16x4 ld_bf16x4(const uint16_t* p) {
   const uint2 v = *reinterpret_cast<const uint2*>(p);
   return __builtin_bit_cast(mlp_bf16x4, v);
 }
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + expf(-z)); }
+// logistic sigmoid 1 / (1 + e^-z) from the hardware base-2 exponential and
+// reciprocal (v_exp_f32, v_rcp_f32; about 1 ulp each): e^-z = 2^(-z log2 e).
+// Saturates to 0 / 1 for large |z| (exp2 -> inf / 0).
+__device__ __forceinline__ float sigmoid_fast(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * -1.4426950408889634f));
+}
+__device__ __forceinline__ float silu(float z) { return z * sigmoid_fast(z); }
 __device__ __forceinline__ float silu_grad(float z) {
-  const float s = 1.0f / (1.0f + expf(-z));
+  const float s = sigmoid_fast(z);
   return s * (1.0f + z * (1.0f - s));
 }
 
@@ -85,64 +101,97 @@ __device__ __forceinline__ MlpNet mlp_net(const uint8_t* blob) {
   return n;
 }
 
-// One dense layer of the wavefront's 64 samples, in place in `act`:
-// act[row][0 .. 16 KB) (bf16, stride MLP_S) -> act[row][0 .. 16 NB).  KB k-blocks
-// in, NB n-blocks out, W with row stride WS.  HIDDEN: SiLU(Z + b) -> bf16 into
-// act; otherwise Z + b (FP32) goes to out4[row * ostride + 0..3] (the last
-// layer; out4 may alias act with ostride = MLP_S / 2: row r's outputs then
-// overwrite row r's own, already consumed, inputs).
-// Row block rb reads only rows 16 rb .. 16 rb + 15 and is written after all its
-// reads (a wavefront's LDS operations complete in order), so in place is safe.
-// Optionally stores the pre-activations Z (FP32, [64][64]) for training.
-template <int KB, int NB, bool HIDDEN>
-__device__ __forceinline__ void mlp_layer(uint16_t* act, const uint16_t* W, int WS, const float* bias,
-                                          float* zsave, float* out4, int ostride) {
+// ---------------------------------------------------------------- inference
+// One layer of Z^T = W A^T + b for the wavefront's 4 sample blocks of 16:
+// act[sb][kb] is the B fragment (bf16 x 4: samples sb*16 + lane%16, inputs
+// kb*16 + 4 (lane/16) .. +3).  Hidden layers overwrite act[sb][ob] with
+// BF16(SiLU(Z + b)); the output layer (OB = 1) leaves Z + b of outputs
+// 4 (lane/16) .. +3 in out[sb].
+template <int KB, int OB, bool HIDDEN>
+__device__ __forceinline__ void mlp_layer_t(const uint16_t* W, int WS, const float* bias, mlp_bf16x4 (&act)[4][4],
+                                            mlp_f32x4* out) {
   const int lane = threadIdx.x & 63;
   const int r16 = lane & 15, g = lane >> 4;
-#pragma unroll 1
-  for (int rb = 0; rb < 4; ++rb) {
-    mlp_f32x4 acc[NB];
+  mlp_bf16x4 wf[OB][KB];
 #pragma unroll
-    for (int nb = 0; nb < NB; ++nb) acc[nb] = mlp_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int ob = 0; ob < OB; ++ob)
 #pragma unroll
-    for (int kb = 0; kb < KB; ++kb) {
-      const mlp_bf16x4 a = ld_bf16x4(act + (rb * 16 + r16) * MLP_S + kb * 16 + 4 * g);
+    for (int kb = 0; kb < KB; ++kb) wf[ob][kb] = ld_bf16x4(W + (ob * 16 + r16) * WS + kb * 16 + 4 * g);
+  mlp_f32x4 bz[OB];
 #pragma unroll
-      for (int nb = 0; nb < NB; ++nb) {
-        const mlp_bf16x4 b = ld_bf16x4(W + (nb * 16 + r16) * WS + kb * 16 + 4 * g);
-        acc[nb] = mfma16(a, b, acc[nb]);
+  for (int ob = 0; ob < OB; ++ob) bz[ob] = *reinterpret_cast<const mlp_f32x4*>(bias + ob * 16 + 4 * g);
+#pragma unroll
+  for (int sb = 0; sb < 4; ++sb) {
+    mlp_f32x4 acc[OB];
+#pragma unroll
+    for (int ob = 0; ob < OB; ++ob) acc[ob] = bz[ob];   // Z = b + sum_k W A
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+      for (int ob = 0; ob < OB; ++ob) acc[ob] = mfma16(wf[ob][kb], act[sb][kb], acc[ob]);
+    if constexpr (HIDDEN) {
+#pragma unroll
+      for (int ob = 0; ob < OB; ++ob) {
+        const mlp_f32x4 z = acc[ob];
+        uint2 pk;
+        pk.x = pack_bf16x2(silu(z[0]), silu(z[1]));
+        pk.y = pack_bf16x2(silu(z[2]), silu(z[3]));
+        act[sb][ob] = __builtin_bit_cast(mlp_bf16x4, pk);
       }
-    }
-#pragma unroll
-    for (int nb = 0; nb < NB; ++nb) {
-      const int col = nb * 16 + r16;
-      const float bc = bias[col];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int row = rb * 16 + 4 * g + i;
-        const float z = acc[nb][i] + bc;
-        if constexpr (HIDDEN) {
-          if (zsave) zsave[row * 64 + col] = z;
-          act[row * MLP_S + col] = f32_to_bf16(silu(z));
-        } else {
-          if (col < MLP_OUT) out4[row * ostride + col] = z;
-        }
-      }
+    } else {
+      out[sb] = acc[0];
     }
   }
 }
 
-// Forward pass of the wavefront's 64 samples.  On entry act[row][0..15] holds
-// the normalised inputs (bf16, cols 8..15 zero).  Normalised outputs go to
-// out4[row * ostride + 0..3] (FP32).  zsave (optional, FP32 [4][64][64])
-// receives the hidden layers' pre-activations.
-__device__ __forceinline__ void mlp_wave_forward(const MlpNet& net, uint16_t* act, float* out4, int ostride,
-                                                 float* zsave) {
-  mlp_layer<1, 4, true>(act, net.W[0], MLP_S0, net.b[0], zsave, nullptr, 0);
-  mlp_layer<4, 4, true>(act, net.W[1], MLP_S, net.b[1], zsave ? zsave + 4096 : nullptr, nullptr, 0);
-  mlp_layer<4, 4, true>(act, net.W[2], MLP_S, net.b[2], zsave ? zsave + 8192 : nullptr, nullptr, 0);
-  mlp_layer<4, 4, true>(act, net.W[3], MLP_S, net.b[3], zsave ? zsave + 12288 : nullptr, nullptr, 0);
-  mlp_layer<4, 1, false>(act, net.W[4], MLP_S, net.b[4], nullptr, out4, ostride);
+// Forward pass of the wavefront's 64 samples, activations in registers.
+// stage: this wavefront's 64 x 16-byte LDS rows.  On entry row s holds sample
+// s's 8 normalised inputs as BF16 (lane s wrote it; the caller fences).  On
+// return y[0..3] are the normalised outputs of the lane's own sample (the rows
+// have been reused to hand them back).
+__device__ __forceinline__ void mlp_wave_infer(const MlpNet& net, uint4* stage, float y[4]) {
+  const int lane = threadIdx.x & 63;
+  const int r16 = lane & 15, g = lane >> 4;
+  mlp_bf16x4 act[4][4];
+#pragma unroll
+  for (int sb = 0; sb < 4; ++sb) {
+    const uint2 v = reinterpret_cast<const uint2*>(stage + sb * 16 + r16)[g & 1];
+    act[sb][0] = g < 2 ? __builtin_bit_cast(mlp_bf16x4, v) : mlp_bf16x4{0, 0, 0, 0};
+  }
+  mlp_layer_t<1, 4, true>(net.W[0], MLP_S0, net.b[0], act, nullptr);
+  mlp_layer_t<4, 4, true>(net.W[1], MLP_S, net.b[1], act, nullptr);
+  mlp_layer_t<4, 4, true>(net.W[2], MLP_S, net.b[2], act, nullptr);
+  mlp_layer_t<4, 4, true>(net.W[3], MLP_S, net.b[3], act, nullptr);
+  mlp_f32x4 o[4];
+  mlp_layer_t<4, 1, false>(net.W[4], MLP_S, net.b[4], act, o);
+  // lanes 0..15 hold the 4 outputs of samples sb*16 + lane: hand them back
+  // through the staging rows (all reads of the inputs are done)
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  if (g == 0) {
+#pragma unroll
+    for (int sb = 0; sb < 4; ++sb)
+      stage[sb * 16 + r16] = __builtin_bit_cast(uint4, o[sb]);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  const mlp_f32x4 r = __builtin_bit_cast(mlp_f32x4, stage[lane]);
+  y[0] = r[0]; y[1] = r[1]; y[2] = r[2]; y[3] = r[3];
+}
+
+// Normalised inputs of the lane's sample -> its BF16 staging row.
+__device__ __forceinline__ void mlp_stage_inputs(uint4* stage, const float xn[8]) {
+  const int lane = threadIdx.x & 63;
+  uint4 pk;
+  pk.x = pack_bf16x2(xn[0], xn[1]);
+  pk.y = pack_bf16x2(xn[2], xn[3]);
+  pk.z = pack_bf16x2(xn[4], xn[5]);
+  pk.w = pack_bf16x2(xn[6], xn[7]);
+  stage[lane] = pk;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
 }  // namespace mpmx

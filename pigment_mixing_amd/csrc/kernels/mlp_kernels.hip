@@ -62,27 +62,23 @@ __global__ void k_mlp_pack(const float* __restrict__ params, int n_params, uint8
 // raw outputs (denormalised).  Rows past n compute on zeros and are not stored.
 __global__ void __launch_bounds__(64) k_mlp_forward(const uint8_t* __restrict__ blob, const float* __restrict__ x8,
                                                     float* __restrict__ y4, int64_t n) {
-  __shared__ __align__(16) uint8_t net_lds[MLP_B_OFF + (4 * 64 + 16) * 4 + 24 * 4];
-  __shared__ __align__(16) uint16_t act[64 * MLP_S];
-  __shared__ float out4[64 * MLP_OUT];
+  __shared__ __align__(16) uint8_t net_lds[MLP_BLOB_BYTES];
+  __shared__ uint4 stage[64];
   for (int i = threadIdx.x; i < MLP_BLOB_BYTES / 4; i += 64)
     reinterpret_cast<uint32_t*>(net_lds)[i] = reinterpret_cast<const uint32_t*>(blob)[i];
   const MlpNet net = mlp_net(net_lds);
   const int lane = threadIdx.x;
   const int64_t row = (int64_t)blockIdx.x * 64 + lane;
   __syncthreads();
+  float xn[MLP_IN];
 #pragma unroll
-  for (int c = 0; c < 16; ++c) {
-    float v = 0.0f;
-    if (c < MLP_IN && row < n) v = (x8[row * MLP_IN + c] - net.norm[c]) / net.norm[8 + c];
-    act[lane * MLP_S + c] = f32_to_bf16(v);
-  }
-  __syncthreads();
-  mlp_wave_forward(net, act, out4, MLP_OUT, nullptr);
-  __syncthreads();
+  for (int c = 0; c < MLP_IN; ++c) xn[c] = row < n ? (x8[row * MLP_IN + c] - net.norm[c]) / net.norm[8 + c] : 0.0f;
+  mlp_stage_inputs(stage, xn);
+  float y[MLP_OUT];
+  mlp_wave_infer(net, stage, y);
   if (row < n) {
 #pragma unroll
-    for (int c = 0; c < MLP_OUT; ++c) y4[row * MLP_OUT + c] = out4[lane * MLP_OUT + c] * net.norm[20 + c] + net.norm[16 + c];
+    for (int c = 0; c < MLP_OUT; ++c) y4[row * MLP_OUT + c] = y[c] * net.norm[20 + c] + net.norm[16 + c];
   }
 }
 

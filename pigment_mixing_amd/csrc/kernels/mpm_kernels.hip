@@ -87,6 +87,7 @@ __device__ __forceinline__ void block_coords(const SceneDev& sc, int key, int& b
 // blocks share halo nodes in that XCD's L2.
 struct XcdRange {
   int e, end, stride;
+  int first;   // first block of this XCD's share
 };
 __device__ __forceinline__ XcdRange xcd_range(int nblk) {
   const int nx = 8;
@@ -94,6 +95,7 @@ __device__ __forceinline__ XcdRange xcd_range(int nblk) {
   const int per = gridDim.x / nx;
   const int seg = (nblk + nx - 1) / nx;
   XcdRange r;
+  r.first = xcd * seg;
   r.e = xcd * seg + (int)(blockIdx.x / nx);
   r.end = min(nblk, (xcd + 1) * seg);
   r.stride = per;
@@ -227,6 +229,15 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_down(DevPtrs d, int nkeys
   if (blockIdx.x == nchunks - 1 && threadIdx.x == SCAN_THREADS - 1) {
     d.counters[CNT_PBLOCKS] = (int)(run & 0xffffffffull);
     d.counters[CNT_LIVE] = d.offsets[nkeys - 1];
+    // the closure queue of the step that just ended is consumed
+    if (d.cq_cnt) {
+      unsigned long long q = 0;
+      for (int x = 0; x < CQ_SEGS; ++x) {
+        q += (unsigned long long)d.cq_cnt[x * CQ_CNT_STRIDE];
+        d.cq_cnt[x * CQ_CNT_STRIDE] = 0;
+      }
+      d.stats[ST_QUEUED] += q;
+    }
     if (advance) {
       d.counters[CNT_STEP] += 1;
       d.counters[CNT_GSTEP] += 1;
@@ -283,26 +294,25 @@ constexpr int GT = 6;              // gather tile edge: stencils of bins [B0, B0
 constexpr int GT3 = GT * GT * GT;  // 216 nodes
 // LDS row of one node: 16 floats padded to 5 float4 (80 B), so that the nodes
 // the cells of one wavefront read at once (node index differences 1, 2, 3, 6,
-// 7, 8, 9, 36, ...) start in distinct 4-bank groups.  The learned-closure
-// variant keeps 4 float4 (its network and activation buffers must leave room
-// for two workgroups per CU).
-template <bool LEARNED>
-constexpr int gt_row() { return LEARNED ? 4 : 5; }
+// 7, 8, 9, 36, ...) start in distinct 4-bank groups.
+constexpr int GT_ROW = 5;
 
 // Read-only inputs come in as __restrict__ kernel arguments (scene table, node
 // coordinates, particle input side, permutation, grid values), so uniform reads
 // of them become scalar loads that the particle stores cannot clobber.
 //
-// LEARNED instantiates the learned return-map closure: the network (BF16
-// weights, mlp_device.hpp) sits in LDS, each wavefront owns a 64-row activation
-// buffer, and a wavefront with at least one plastic candidate evaluates the
-// network for its 64 lanes on MFMA.  A candidate's predicted state is accepted
-// when its Perzyna consistency residual, its trace change (x K) and its
-// flow-rule mismatch are all within closure_tol * q_y; otherwise the lane runs
-// the exact map (fallback, counted).
+// The learned return-map closure runs in its own kernel (k_closure) over a
+// queue of the plastic candidates that G2P found: the network (BF16 weights,
+// mlp_device.hpp) sits in LDS, each wavefront owns 64 16-byte staging rows
+// and evaluates 64 queued candidates at once on MFMA (activations in
+// registers), so the gather kernel keeps
+// the exact variant's registers, LDS and occupancy.  A candidate's predicted
+// state is accepted when its Perzyna consistency residual, its trace change
+// (x K) and its flow-rule mismatch are all within closure_tol * q_y; otherwise
+// the lane runs the exact map (fallback, counted).
 struct G2PLearnedLds {
   uint8_t net[MLP_BLOB_BYTES];
-  uint16_t act[4][64 * MLP_S];
+  uint4 stage[4][64];
 };
 
 // Learned return map for the principal Hencky strains h (in/out) of this lane.
@@ -311,36 +321,47 @@ struct G2PLearnedLds {
 // pressure cap).  Returns true when the predicted state was accepted and
 // applied to h, eps, dg_rate; false leaves them untouched (the caller runs the
 // exact map).
+// Trial quantities of the return map (plasticity.cpp:163-170) from the principal
+// Hencky strains: trace, unit deviator direction, ||dev h||, q_trial, q_yield.
+struct TrialState {
+  float htr, nv[3], dn, q_trial, q_yield;
+  bool cand;   // plastic candidate of the closure: q_tr > q_y outside the pressure cap
+};
+__device__ __forceinline__ TrialState trial_state(const SceneDev& sc, const float h[3], float eps) {
+  TrialState t;
+  t.htr = (h[0] + h[1]) + h[2];
+  const float m = t.htr / 3.0f;
+  t.nv[0] = h[0] - m; t.nv[1] = h[1] - m; t.nv[2] = h[2] - m;
+  t.dn = sqrtf(t.nv[0] * t.nv[0] + t.nv[1] * t.nv[1] + t.nv[2] * t.nv[2]);
+  if (t.dn > 0.0f) { const float inv = 1.0f / t.dn; t.nv[0] *= inv; t.nv[1] *= inv; t.nv[2] *= inv; }
+  t.q_trial = sc.e_mu_prefac * t.dn;
+  t.q_yield = sc.q_min + (sc.q_max - sc.q_min) * expf(-sc.xi * eps);
+  const float p_trial = -sc.K * t.htr;
+  t.cand = sc.closure != 0 && sc.plastic_model == 5 && !(p_trial < sc.p_min) && t.q_trial > t.q_yield;
+  return t;
+}
+
 __device__ bool learned_return_map(const SceneDev& sc, G2PLearnedLds& lrn, float h[3], float& eps, float& dg_rate,
                                    bool active, bool& cand_out) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const float htr = (h[0] + h[1]) + h[2];
-  const float m = htr / 3.0f;
-  float nv[3] = {h[0] - m, h[1] - m, h[2] - m};
-  const float dn = sqrtf(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
-  if (dn > 0.0f) { const float inv = 1.0f / dn; nv[0] *= inv; nv[1] *= inv; nv[2] *= inv; }
-  const float q_trial = sc.e_mu_prefac * dn;
-  const float q_yield = sc.q_min + (sc.q_max - sc.q_min) * expf(-sc.xi * eps);
-  const float p_trial = -sc.K * htr;
-  const bool cand = active && sc.closure != 0 && sc.plastic_model == 5 && !(p_trial < sc.p_min) && q_trial > q_yield;
+  const TrialState ts = trial_state(sc, h, eps);
+  const float htr = ts.htr, dn = ts.dn, q_trial = ts.q_trial, q_yield = ts.q_yield;
+  const float nv[3] = {ts.nv[0], ts.nv[1], ts.nv[2]};
+  const bool cand = active && ts.cand;
   cand_out = cand;
   if (!__ballot(cand)) return false;
-  // ---- network inputs of all 64 lanes -> this wavefront's activation rows
+  // ---- network inputs of all 64 lanes -> this wavefront's staging rows
   const MlpNet net = mlp_net(lrn.net);
-  uint16_t* act = lrn.act[wave];
+  uint4* stage = lrn.stage[wave];
   const float x8[8] = {h[0], h[1], h[2], dn, q_yield, q_yield * sc.perzyna_visc, sc.dt * sc.E, sc.nu};
+  float xn[8];
 #pragma unroll
-  for (int c = 0; c < 16; ++c)
-    act[lane * MLP_S + c] = c < 8 ? f32_to_bf16((x8[c] - net.norm[c]) / net.norm[8 + c]) : (uint16_t)0;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  float* out = reinterpret_cast<float*>(act);
-  mlp_wave_forward(net, act, out, MLP_S / 2, nullptr);
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
+  for (int c = 0; c < 8; ++c) xn[c] = (x8[c] - net.norm[c]) / net.norm[8 + c];
+  mlp_stage_inputs(stage, xn);
   float y[4];
+  mlp_wave_infer(net, stage, y);
 #pragma unroll
-  for (int c = 0; c < 4; ++c) y[c] = out[lane * (MLP_S / 2) + c] * net.norm[20 + c] + net.norm[16 + c];
+  for (int c = 0; c < 4; ++c) y[c] = y[c] * net.norm[20 + c] + net.norm[16 + c];
   if (!cand) return false;
   // ---- predicted state
   float dg = fmaxf(y[3], 0.0f);
@@ -376,6 +397,37 @@ __device__ bool learned_return_map(const SceneDev& sc, G2PLearnedLds& lrn, float
   return true;
 }
 
+// Harvest record of one plastic particle: h_trial[3], ||dev h_trial||, q_y,
+// eta = q_y * perzyna_visc, dt * E, nu | corrected h[3], dgamma.
+__device__ __forceinline__ void harvest_pair(const DevPtrs& d, const SceneDev& sc, const float h_trial[3],
+                                             float eps_in, const float h[3], float dgr) {
+  const unsigned long long slot = atomicAdd(&d.stats[ST_HARVEST], 1ull);
+  if ((int64_t)slot >= d.harvest_cap) return;
+  float* r = d.harvest + 12 * slot;
+  const float m = ((h_trial[0] + h_trial[1]) + h_trial[2]) / 3.0f;
+  const float e0 = h_trial[0] - m, e1 = h_trial[1] - m, e2 = h_trial[2] - m;
+  const float qy = sc.q_min + (sc.q_max - sc.q_min) * expf(-sc.xi * eps_in);
+  r[0] = h_trial[0]; r[1] = h_trial[1]; r[2] = h_trial[2];
+  r[3] = sqrtf(e0 * e0 + e1 * e1 + e2 * e2);
+  r[4] = qy;
+  r[5] = qy * sc.perzyna_visc;
+  r[6] = sc.dt * sc.E;
+  r[7] = sc.nu;
+  r[8] = h[0]; r[9] = h[1]; r[10] = h[2];
+  r[11] = dgr * sc.dt;
+}
+
+// F = U exp(h) V^T from the corrected principal strains.
+__device__ __forceinline__ void f_from_principal(const float U[9], const float h[3], const float V[9], float Fn[9]) {
+  const float e0 = expf(h[0]), e1 = expf(h[1]), e2 = expf(h[2]);
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      Fn[3 * r + c] = (U[3 * r + 0] * e0) * V[3 * c + 0] + (U[3 * r + 1] * e1) * V[3 * c + 1] +
+                      (U[3 * r + 2] * e2) * V[3 * c + 2];
+}
+
 // PHASE 0: the fused USL step.  MUSL (musl.cpp:6-98) splits it around a second
 // scatter: PHASE 1 gathers from the updated grid (velocity, APIC B, FLIP,
 // latent, shear), mixes the latent and sets v (AFLIP), and parks the PIC
@@ -383,27 +435,33 @@ __device__ bool learned_return_map(const SceneDev& sc, G2PLearnedLds& lrn, float
 // normalisation then rebuild the grid velocity from the new v and B; PHASE 2
 // gathers the velocity gradient from it and finishes the step (F update,
 // return map, stress, advection with the parked PIC velocity, sort keys).
+// LEARNED (phases 0 and 2): plastic candidates of the closure are appended to
+// the closure queue with their trial F instead of being mapped here; k_closure
+// writes their F, eps, dgamma and stress.
+// Four wavefronts per SIMD (128 VGPRs; the fused phase spills a few dwords):
+// the gather's latency hiding is worth more than the spills (measured ~20%
+// faster than three wavefronts at 137 VGPRs).
 template <bool LEARNED, int PHASE>
-__global__ void __launch_bounds__(256) k_g2p(DevPtrs d, const SceneDev* __restrict__ scenes,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_g2p(DevPtrs d, const SceneDev* __restrict__ scenes,
                                              const float* __restrict__ axis, const float* __restrict__ A,
                                              const int* __restrict__ perm, const float* __restrict__ gv,
-                                             const uint8_t* __restrict__ mlp_blob, const int4* __restrict__ pdesc) {
-  constexpr int GT_ROW = gt_row<LEARNED>();
+                                             const int4* __restrict__ pdesc) {
   __shared__ float4 gtile[GT3 * GT_ROW];
   __shared__ float sax[3][GT];       // node coordinates B0 .. B0+5 per axis
-  __shared__ unsigned int plastic_wg, fallback_wg, learned_wg, newton_wg;
-  __shared__ __align__(16) uint8_t lrn_raw[LEARNED ? sizeof(G2PLearnedLds) : 16];
+  __shared__ unsigned int plastic_wg, newton_wg;
   const int step = d.counters[CNT_STEP];
   const int nblk = d.counters[CNT_PBLOCKS];
   float* __restrict__ Bf = d.pb;
   const int64_t cap = d.cap;
-  if (threadIdx.x == 0) { plastic_wg = 0u; fallback_wg = 0u; learned_wg = 0u; newton_wg = 0u; }
-  G2PLearnedLds& lrn = *reinterpret_cast<G2PLearnedLds*>(lrn_raw);
-  if constexpr (LEARNED) {
-    for (int i = threadIdx.x; i < MLP_BLOB_BYTES / 4; i += blockDim.x)
-      reinterpret_cast<uint32_t*>(lrn.net)[i] = reinterpret_cast<const uint32_t*>(mlp_blob)[i];
-  }
+  if (threadIdx.x == 0) { plastic_wg = 0u; newton_wg = 0u; }
   const XcdRange xr = xcd_range(nblk);
+  // this XCD's closure-queue segment (learned variant)
+  int64_t cq_base = 0;
+  int* cq_counter = nullptr;
+  if constexpr (LEARNED) {
+    cq_base = xr.first < nblk ? pdesc[xr.first].y : 0;
+    cq_counter = d.cq_cnt + CQ_CNT_STRIDE * (blockIdx.x % CQ_SEGS);
+  }
   for (int e = xr.e; e < xr.end; e += xr.stride) {
     const int4 ds = pdesc[e];       // key, first slot, count, scene
     const int key = ds.x;
@@ -444,12 +502,10 @@ __global__ void __launch_bounds__(256) k_g2p(DevPtrs d, const SceneDev* __restri
     for (int it = 0; it < iters; ++it) {
       const int t = it * blockDim.x + threadIdx.x;
       const bool valid = t < cnt;
-      // Every lane computes (the learned closure needs whole wavefronts); a lane
-      // past the block's count recomputes the block's last particle and stores
-      // nothing.
+      // Every lane computes; a lane past the block's count recomputes the block's
+      // last particle and stores nothing.
       const int64_t j = start + (valid ? t : cnt - 1);
       int newkey = d.dead_key, newcell = -1, plastic = 0;
-      bool learned_flag = false, fallback_flag = false;
       int newton = 0;
       {
         const int src = perm[j];
@@ -604,44 +660,33 @@ __global__ void __launch_bounds__(256) k_g2p(DevPtrs d, const SceneDev* __restri
         float eps = eps_in, dgr = 0.0f;
         uint32_t err = 0;
         const float h_trial[3] = {h[0], h[1], h[2]};
-        bool learned_ok = false;
+        bool deferred = false;
         if constexpr (LEARNED) {
-          bool cand = false;
-          learned_ok = learned_return_map(sc, lrn, h, eps, dgr, valid, cand);
-          if (learned_ok) plastic = 1;
-          learned_flag = valid && learned_ok;
-          fallback_flag = valid && cand && !learned_ok;
-        }
-        if (!learned_ok && sc.plastic_model == 5) plastic = vmvisc_principal(sc, h, eps, dgr, err, &newton);
-        if (!valid) newton = 0;
-        if (err && valid) atomicOr(&d.err[0], err);
-        if (plastic && valid && d.harvest) {
-          // record: h_trial[3], ||dev h_trial||, q_y, eta = q_y*perzyna_visc, dt*E, nu | h[3], dgamma
-          const unsigned long long slot = atomicAdd(&d.stats[ST_HARVEST], 1ull);
-          if ((int64_t)slot < d.harvest_cap) {
-            float* r = d.harvest + 12 * slot;
-            const float m = ((h_trial[0] + h_trial[1]) + h_trial[2]) / 3.0f;
-            const float e0 = h_trial[0] - m, e1 = h_trial[1] - m, e2 = h_trial[2] - m;
-            const float qy = sc.q_min + (sc.q_max - sc.q_min) * expf(-sc.xi * eps_in);
-            r[0] = h_trial[0]; r[1] = h_trial[1]; r[2] = h_trial[2];
-            r[3] = sqrtf(e0 * e0 + e1 * e1 + e2 * e2);
-            r[4] = qy;
-            r[5] = qy * sc.perzyna_visc;
-            r[6] = sc.dt * sc.E;
-            r[7] = sc.nu;
-            r[8] = h[0]; r[9] = h[1]; r[10] = h[2];
-            r[11] = dgr * sc.dt;
+          // append the wavefront's candidates to this XCD's queue segment (one
+          // atomic per wavefront; slots in lane order)
+          deferred = valid && trial_state(sc, h, eps_in).cand;
+          const uint64_t dm = __ballot(deferred);
+          if (dm) {
+            int base = 0;
+            if (__lane_id() == 0) base = atomicAdd(cq_counter, __popcll(dm));
+            base = __shfl(base, 0);
+            const int64_t slot = cq_base + base + __popcll(dm & ((1ull << __lane_id()) - 1ull));
+            deferred = deferred && slot < d.cq_cap;   // a full queue maps in place
+            if (deferred) {
+              float4* r = d.cq + CQ_REC * slot;
+              r[0] = make_float4(__int_as_float((int)j), __int_as_float(s), eps_in, Ft[0]);
+              r[1] = make_float4(Ft[1], Ft[2], Ft[3], Ft[4]);
+              r[2] = make_float4(Ft[5], Ft[6], Ft[7], Ft[8]);
+            }
           }
         }
+        if (!deferred && sc.plastic_model == 5) plastic = vmvisc_principal(sc, h, eps, dgr, err, &newton);
+        if (!valid) newton = 0;
+        if (err && valid) atomicOr(&d.err[0], err);
+        if (plastic && valid && d.harvest) harvest_pair(d, sc, h_trial, eps_in, h, dgr);
         float Fn[9];
         if (plastic) {
-          const float e0 = expf(h[0]), e1 = expf(h[1]), e2 = expf(h[2]);
-#pragma unroll
-          for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-              Fn[3 * r + c] = (U[3 * r + 0] * e0) * V[3 * c + 0] + (U[3 * r + 1] * e1) * V[3 * c + 1] +
-                              (U[3 * r + 2] * e2) * V[3 * c + 2];
+          f_from_principal(U, h, V, Fn);
         } else {
 #pragma unroll
           for (int c = 0; c < 9; ++c) Fn[c] = Ft[c];
@@ -687,8 +732,14 @@ __global__ void __launch_bounds__(256) k_g2p(DevPtrs d, const SceneDev* __restri
         if (valid) {
 #pragma unroll
           for (int c = 0; c < 3; ++c) { Bf[(P_X + c) * cap + j] = xn[c]; Bf[(P_V + c) * cap + j] = vn[c]; }
+          if (!deferred) {
 #pragma unroll
-          for (int c = 0; c < 9; ++c) Bf[(P_F + c) * cap + j] = Fn[c];
+            for (int c = 0; c < 9; ++c) Bf[(P_F + c) * cap + j] = Fn[c];
+            Bf[P_EPS * cap + j] = eps;
+            Bf[P_DG * cap + j] = dgr;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) Bf[(P_TAU + c) * cap + j] = tau[c];
+          }
           if constexpr (PHASE == 0) {
 #pragma unroll
             for (int c = 0; c < 9; ++c) Bf[(P_C + c) * cap + j] = Bp[c];
@@ -696,10 +747,6 @@ __global__ void __launch_bounds__(256) k_g2p(DevPtrs d, const SceneDev* __restri
             for (int c = 0; c < NPIG; ++c) Bf[(P_PIG + c) * cap + j] = pig[c];
             Bf[P_DIFF * cap + j] = diff_new;
           }
-          Bf[P_EPS * cap + j] = eps;
-          Bf[P_DG * cap + j] = dgr;
-#pragma unroll
-          for (int c = 0; c < 6; ++c) Bf[(P_TAU + c) * cap + j] = tau[c];
           d.pb_id[j] = id;
         }
       }
@@ -707,15 +754,98 @@ __global__ void __launch_bounds__(256) k_g2p(DevPtrs d, const SceneDev* __restri
       const uint64_t pm = __ballot(valid && plastic);
       if (__lane_id() == 0 && pm) atomicAdd(&plastic_wg, (unsigned)__popcll(pm));
       if (newton) atomicAdd(&newton_wg, (unsigned)newton);
-      if constexpr (LEARNED) {
-        const uint64_t lm = __ballot(learned_flag);
-        const uint64_t fm = __ballot(fallback_flag);
-        if (__lane_id() == 0) {
-          if (lm) atomicAdd(&learned_wg, (unsigned)__popcll(lm));
-          if (fm) atomicAdd(&fallback_wg, (unsigned)__popcll(fm));
-        }
-      }
     }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (plastic_wg) atomicAdd(&d.stats[ST_PLASTIC_CUR], (unsigned long long)plastic_wg);
+    if (newton_wg) atomicAdd(&d.stats[ST_NEWTON], (unsigned long long)newton_wg);
+  }
+}
+
+// The learned closure over the queued candidates (one record per lane, 64 per
+// wavefront on MFMA): SVD of the trial F (the same routine as G2P), learned map
+// with the exact map as per-lane fallback, then F = U exp(h') V^T, eps_pl_dev,
+// dgamma / dt and the Kirchhoff stress into the particle's output slot.  The
+// queue's order (wavefront atomics) does not change any result: every record
+// is mapped on its own (the network's rows are independent).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_closure(DevPtrs d, const SceneDev* __restrict__ scenes,
+                                                 const uint8_t* __restrict__ mlp_blob) {
+  __shared__ __align__(16) G2PLearnedLds lrn;
+  __shared__ unsigned int plastic_wg, fallback_wg, learned_wg, newton_wg;
+  // the XCD segments, concatenated: record q of the whole queue lies in segment
+  // x with pre[x] <= q < pre[x + 1], at slot start[x] + q - pre[x]
+  const int nblk = d.counters[CNT_PBLOCKS];
+  const int segb = (nblk + CQ_SEGS - 1) / CQ_SEGS;
+  int pre[CQ_SEGS + 1], start[CQ_SEGS];   // slots and counts are below the particle capacity (int)
+  pre[0] = 0;
+#pragma unroll
+  for (int x = 0; x < CQ_SEGS; ++x) {
+    const bool any = x * segb < nblk;
+    start[x] = any ? d.pdesc[x * segb].y : 0;
+    pre[x + 1] = pre[x] + (any ? d.cq_cnt[x * CQ_CNT_STRIDE] : 0);
+  }
+  const int n = pre[CQ_SEGS];
+  if ((int64_t)blockIdx.x * blockDim.x >= n) return;   // whole workgroup: nothing queued for it
+  if (threadIdx.x == 0) { plastic_wg = 0u; fallback_wg = 0u; learned_wg = 0u; newton_wg = 0u; }
+  for (int i = threadIdx.x; i < MLP_BLOB_BYTES / 4; i += blockDim.x)
+    reinterpret_cast<uint32_t*>(lrn.net)[i] = reinterpret_cast<const uint32_t*>(mlp_blob)[i];
+  __syncthreads();
+  float* __restrict__ Bf = d.pb;
+  const int64_t cap = d.cap;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
+    const int q0 = (int)base + (int)threadIdx.x;
+    const int q = q0 < n ? q0 : n - 1;
+    int slot = start[0] + q;   // segment by a select chain (pre, start stay uniform)
+#pragma unroll
+    for (int k = 1; k < CQ_SEGS; ++k)
+      if (q >= pre[k]) slot = start[k] + (q - pre[k]);
+    const bool valid = q0 < n && slot < d.cq_cap;      // G2P mapped in place past the capacity
+    const float4* r = d.cq + CQ_REC * min((int64_t)slot, d.cq_cap - 1);
+    const float4 r0 = r[0], r1 = r[1], r2 = r[2];
+    const int64_t j = __float_as_int(r0.x);
+    const SceneDev& sc = scenes[__float_as_int(r0.y)];
+    const float eps_in = r0.z;
+    const float Ft[9] = {r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w};
+    float U[9], sig[3], V[9];
+    svd3(Ft, U, sig, V);
+    float h[3] = {logf(fmaxf(fabsf(sig[0]), 1e-4f)), logf(fmaxf(fabsf(sig[1]), 1e-4f)),
+                  logf(fmaxf(fabsf(sig[2]), 1e-4f))};
+    const float h_trial[3] = {h[0], h[1], h[2]};
+    float eps = eps_in, dgr = 0.0f;
+    uint32_t err = 0;
+    int newton = 0, plastic = 0;
+    bool cand = false;
+    const bool ok = learned_return_map(sc, lrn, h, eps, dgr, valid, cand);
+    if (ok) plastic = 1;
+    else if (sc.plastic_model == 5) plastic = vmvisc_principal(sc, h, eps, dgr, err, &newton);
+    if (valid) {
+      if (err) atomicOr(&d.err[0], err);
+      if (plastic && d.harvest) harvest_pair(d, sc, h_trial, eps_in, h, dgr);
+      float Fn[9];
+      if (plastic) f_from_principal(U, h, V, Fn);
+      else {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) Fn[c] = Ft[c];
+      }
+      float tau[6];
+      kirchhoff_from_principal(sc, U, h, tau);
+#pragma unroll
+      for (int c = 0; c < 9; ++c) Bf[(P_F + c) * cap + j] = Fn[c];
+      Bf[P_EPS * cap + j] = eps;
+      Bf[P_DG * cap + j] = dgr;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) Bf[(P_TAU + c) * cap + j] = tau[c];
+    }
+    const uint64_t pm = __ballot(valid && plastic);
+    const uint64_t lm = __ballot(valid && ok);
+    const uint64_t fm = __ballot(valid && cand && !ok);
+    if (__lane_id() == 0) {
+      if (pm) atomicAdd(&plastic_wg, (unsigned)__popcll(pm));
+      if (lm) atomicAdd(&learned_wg, (unsigned)__popcll(lm));
+      if (fm) atomicAdd(&fallback_wg, (unsigned)__popcll(fm));
+    }
+    if (valid && newton) atomicAdd(&newton_wg, (unsigned)newton);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1380,25 +1510,27 @@ void launch_cellscan(const DevPtrs& d, int grid, hipStream_t st) {
 void launch_perm(const DevPtrs& d, int64_t n, hipStream_t st) {
   if (n > 0) hipLaunchKernelGGL(k_perm, dim3(blocks_for(n, 256)), dim3(256), 0, st, d, n);
 }
-void launch_g2p(const DevPtrs& d, int grid, const void* mlp_blob, int phase, hipStream_t st) {
+void launch_g2p(const DevPtrs& d, int grid, int learned, int phase, hipStream_t st) {
   const SceneDev* sc = d.scenes;
   const float* ax = d.axis_coords;
   const float* pa = d.pa;
   const int* pm = d.perm;
   const float* gv = d.gv;
-  const uint8_t* bl = (const uint8_t*)mlp_blob;
   const int4* pd = d.pdesc;
   if (phase == 1)
-    hipLaunchKernelGGL((k_g2p<false, 1>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, bl, pd);
-  else if (phase == 2 && mlp_blob)
-    hipLaunchKernelGGL((k_g2p<true, 2>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, bl, pd);
+    hipLaunchKernelGGL((k_g2p<false, 1>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, pd);
+  else if (phase == 2 && learned)
+    hipLaunchKernelGGL((k_g2p<true, 2>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, pd);
   else if (phase == 2)
-    hipLaunchKernelGGL((k_g2p<false, 2>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, bl, pd);
-  else if (mlp_blob)
-    hipLaunchKernelGGL((k_g2p<true, 0>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, bl, pd);
+    hipLaunchKernelGGL((k_g2p<false, 2>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, pd);
+  else if (learned)
+    hipLaunchKernelGGL((k_g2p<true, 0>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, pd);
   else
-    hipLaunchKernelGGL((k_g2p<false, 0>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, bl, pd);
+    hipLaunchKernelGGL((k_g2p<false, 0>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, pd);
   if (phase != 1) hipLaunchKernelGGL(k_copy_dead, dim3(64), dim3(256), 0, st, d);
+}
+void launch_closure(const DevPtrs& d, const void* mlp_blob, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_closure, dim3(grid), dim3(256), 0, st, d, d.scenes, (const uint8_t*)mlp_blob);
 }
 void launch_closure_eval(const SceneDev* scenes, const void* mlp_blob, const float* trial4, int64_t n,
                          float* learned4, float* exact4, uint8_t* accepted, int max_grid, hipStream_t st) {

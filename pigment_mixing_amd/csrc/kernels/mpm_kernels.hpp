@@ -31,6 +31,7 @@ enum {
   ST_NEWTON = 10,       // Newton iterations of the exact map (cumulative)
   ST_LEARNED = 11,      // learned-closure predictions accepted (cumulative)
   ST_LEARNED_LAST = 12, ST_LEARNED_CUR = 13,
+  ST_QUEUED = 14,       // closure-queue records (cumulative)
   ST_N = 16
 };
 
@@ -72,12 +73,21 @@ struct DevPtrs {
   uint32_t* err;
   float* harvest;          // (trial, corrected) records of 12 floats, or null
   int64_t harvest_cap;
+  float4* cq;              // closure queue: CQ_REC float4 per deferred plastic candidate, or null
+  int64_t cq_cap;          // records it holds (the particle capacity)
+  int* cq_cnt;             // records appended per XCD segment this step (CQ_SEGS, CQ_CNT_STRIDE apart)
   int64_t cap;
   int n_scenes;
   int nb_list_cap;
   int dead_key;
   int tile_cap;            // listed blocks with a tile; later ones scatter with atomics
 };
+constexpr int CQ_REC = 3;              // float4 per closure-queue record: {slot, scene, eps, F0} {F1..F4} {F5..F8}
+// The queue is split into one segment per XCD walker: segment x starts at the
+// first particle slot of XCD x's contiguous share of the block list and can
+// hold every particle of that share, so it never overflows.
+constexpr int CQ_SEGS = 8;
+constexpr int CQ_CNT_STRIDE = 32;      // ints: one 128-byte line per segment counter
 constexpr int PT = 6;                  // scatter tile edge (nodes) = 4 bins + 2
 constexpr int PT3 = PT * PT * PT;      // 216
 
@@ -89,8 +99,12 @@ void launch_cellscan(const DevPtrs& d, int grid, hipStream_t st);
 void launch_perm(const DevPtrs& d, int64_t n, hipStream_t st);
 // phase 0: fused USL G2P (+ dead-particle copy); MUSL: phase 1 (first half),
 // launch_musl (second scatter + grid velocity), phase 2 (second half + copy).
-void launch_g2p(const DevPtrs& d, int grid, const void* mlp_blob, int phase, hipStream_t st);
+// learned (phases 0, 2): queue the closure's candidates for launch_closure.
+void launch_g2p(const DevPtrs& d, int grid, int learned, int phase, hipStream_t st);
 void launch_musl(const DevPtrs& d, int grid_p2g, int grid_grid, hipStream_t st);
+// The learned closure over the candidates G2P queued (after each G2P of a step
+// with the learned return map).
+void launch_closure(const DevPtrs& d, const void* mlp_blob, int grid, hipStream_t st);
 void launch_closure_eval(const SceneDev* scenes, const void* mlp_blob, const float* trial4, int64_t n,
                          float* learned4, float* exact4, uint8_t* accepted, int max_grid, hipStream_t st);
 void launch_p2g(const DevPtrs& d, int grid, int src_is_b, hipStream_t st);
