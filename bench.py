@@ -96,7 +96,28 @@ def timed_run(h, steps, warmup, tune=True):
     return wall, h.stats()
 
 
-def kernel_report(kt):
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "hbm_traffic_paint_4blob_256x64x256.json")
+TRAFFIC_NAMES = {"k_g2p": ("k_g2p<false, 0>", "k_g2p<true, 0>"), "k_p2g": ("k_p2g<false>",) * 2,
+                 "k_grid": ("k_grid",) * 2}
+
+
+def measured_traffic(kernel, learned):
+    """HBM bytes per dispatch of `kernel` on the 1M-particle workload from the
+    committed FETCH_SIZE / WRITE_SIZE measurement (copy-calibrated), or None
+    when that measurement belongs to other kernel sources."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        from pigment_mixing_amd import build
+        if t["kernel_source_sha256"] != build.kernel_source_hash():
+            return None
+        return t["kernels"][TRAFFIC_NAMES[kernel][1 if learned else 0]]["bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def kernel_report(kt, learned=False, traffic=False):
     n_act = kt.active_nodes_per_step
     kb = kernel_bytes(kt.particles_per_step, n_act)
     kms = {"k_g2p": kt.ms_g2p, "k_p2g": kt.ms_p2g, "k_grid": kt.ms_grid}
@@ -118,7 +139,7 @@ def kernel_report(kt):
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": kgbs[dom] / HBM_PEAK_GBS,
-        "traffic": None,
+        "traffic": measured_traffic(dom, learned) if traffic else None,
         "algorithmic_bytes_per_launch": kb[dom],
         "kernel_ms": kms[dom],
         "kernels": {k: {"ms": kms[k], "GB/s": kgbs[k], "frac": kgbs[k] / HBM_PEAK_GBS,
@@ -168,7 +189,7 @@ def main():
     wall_p, st_p = timed_run(h, args.steps, args.warmup)
     value_p = st_p.n_active * args.steps / wall_p
     kt_p = h.profile(args.steps)
-    rf_p = kernel_report(kt_p)
+    rf_p = kernel_report(kt_p, learned=False, traffic=True)
     print(f"[bench] plain: {args.steps} steps in {wall_p:.3f}s -> {value_p:.4g} particle-steps/s; kernels ms "
           f"g2p={kt_p.ms_g2p:.4f} sort={kt_p.ms_sort:.4f} p2g={kt_p.ms_p2g:.4f} grid={kt_p.ms_grid:.4f}",
           file=sys.stderr, flush=True)
@@ -208,7 +229,7 @@ def main():
           f"{st.plastic_count} accepted {st.learned_count} fallback {st.fallback_count}; held-out stress err max "
           f"{err_max}", file=sys.stderr, flush=True)
     kt = hl.profile(args.steps)
-    rf = kernel_report(kt)
+    rf = kernel_report(kt, learned=True, traffic=True)
     print(f"[bench] learned kernels ms: g2p={kt.ms_g2p:.4f} closure={kt.ms_closure:.4f} "
           f"({kt.queued_per_step:.0f} queued) sort={kt.ms_sort:.4f} p2g={kt.ms_p2g:.4f} "
           f"grid={kt.ms_grid:.4f} step={kt.ms_step:.4f} active_nodes={kt.active_nodes_per_step:.0f}",

@@ -42,6 +42,22 @@ COMMON = [
 ]
 
 
+def kernel_source_hash():
+    """SHA-256 over the device sources (kernels/ and common/ headers, sorted by
+    path): identifies the kernel build that a stored counter measurement
+    belongs to."""
+    import hashlib
+    hs = hashlib.sha256()
+    for sub in ("kernels", "common"):
+        d = os.path.join(CSRC, sub)
+        for name in sorted(os.listdir(d)):
+            if name.endswith((".hip", ".hpp", ".h")):
+                hs.update(f"{sub}/{name}\n".encode())
+                with open(os.path.join(d, name), "rb") as f:
+                    hs.update(f.read())
+    return hs.hexdigest()
+
+
 def hipcc():
     for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
         if c and os.path.exists(c):

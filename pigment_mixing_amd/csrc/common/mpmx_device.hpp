@@ -114,8 +114,17 @@ MPMX_HD int node_local(int i, int j, int k) { return (((i & 3) << 2) | (j & 3)) 
 // to float; for |u| < 0.5, 0.75 - fl(u*u) is exact in double, and for
 // 0.5 <= |u| < 1.5, t = 1.5 - |u| is exact in float and 0.5*fl(t*t) equals the
 // rounded double 0.5*t*t, so the unfused float forms below are bit-identical.
-// The products must not be contracted into FMAs.
+// The products must not be contracted into FMAs: HIP's __fmul_rn / __fsub_rn
+// are plain operators here, so these helpers switch contraction off.
+//
+// u = (x - X_node) / dx as the reference rounds it (one product, no FMA with
+// what follows).
+MPMX_HD float spline_u(float x, float X, float inv_dx) {
+#pragma clang fp contract(off)
+  return (x - X) * inv_dx;
+}
 MPMX_HD float spline_N(float u) {
+#pragma clang fp contract(off)
   float a = fabsf(u);
 #ifdef __HIP_DEVICE_COMPILE__
   if (a < 0.5f) return __fsub_rn(0.75f, __fmul_rn(a, a));
@@ -127,30 +136,58 @@ MPMX_HD float spline_N(float u) {
   return 0.0f;
 }
 MPMX_HD float spline_dN(float u) {
+#pragma clang fp contract(off)
   float a = fabsf(u);
   if (a < 0.5f) return -2.0f * u;
   if (a < 1.5f) return u > 0.0f ? u - 1.5f : (u < 0.0f ? u + 1.5f : u);
   return 0.0f;
 }
 
+// The same weights for stencil offset O of the particle's base node, where u
+// lies in [0.5, 1.5) (O = 0), [-0.5, 0.5) (O = 1) or [-1.5, -0.5) (O = 2) up to
+// the node coordinates' rounding: the branch of spline_N / spline_dN is known,
+// so these are its unfused float forms with the same operations (bit-identical
+// inside the interval).  Offsets 0 and 2 keep the zero beyond |u| = 1.5; a |u|
+// a rounding past 0.5 at offset 1 changes the value by O(u - 0.5)^2, below
+// float resolution.
+template <int O>
+MPMX_HD float spline_N_at(float u) {
+#pragma clang fp contract(off)
+#ifdef __HIP_DEVICE_COMPILE__
+  if constexpr (O == 1) return __fsub_rn(0.75f, __fmul_rn(u, u));
+  const float t = O == 0 ? __fsub_rn(1.5f, u) : __fadd_rn(1.5f, u);
+  return t > 0.0f ? __fmul_rn(0.5f, __fmul_rn(t, t)) : 0.0f;
+#else
+  return spline_N(u);
+#endif
+}
+template <int O>
+MPMX_HD float spline_dN_at(float u) {
+#pragma clang fp contract(off)
+#ifdef __HIP_DEVICE_COMPILE__
+  if constexpr (O == 1) return -2.0f * u;
+  const float t = O == 0 ? u - 1.5f : u + 1.5f;
+  return (O == 0 ? t < 0.0f : t > 0.0f) ? t : 0.0f;
+#else
+  return spline_dN(u);
+#endif
+}
+
 // Per-axis weights of the 3-node stencil starting at node b.
 struct AxisW { float N[3]; float dN[3]; };
 
 MPMX_HD void axis_weights(float xp, const float* coords, int b, float inv_dx, AxisW& a) {
+  float u[3];
 #pragma unroll
-  for (int o = 0; o < 3; ++o) {
-#ifdef __HIP_DEVICE_COMPILE__
-    float u = __fmul_rn(__fsub_rn(xp, coords[b + o]), inv_dx);
-#else
-    float u = (xp - coords[b + o]) * inv_dx;
-#endif
-    a.N[o] = spline_N(u);
-    a.dN[o] = spline_dN(u);
-  }
+  for (int o = 0; o < 3; ++o) u[o] = spline_u(xp, coords[b + o], inv_dx);
+  a.N[0] = spline_N_at<0>(u[0]); a.dN[0] = spline_dN_at<0>(u[0]);
+  a.N[1] = spline_N_at<1>(u[1]); a.dN[1] = spline_dN_at<1>(u[1]);
+  a.N[2] = spline_N_at<2>(u[2]); a.dN[2] = spline_dN_at<2>(u[2]);
 }
 
 // stencil base node (the particle "bin"): floor((x - origin)/dx - 0.5)
 MPMX_HD int stencil_base(float xp, float origin, float inv_dx) {
+#pragma clang fp contract(off)
 #ifdef __HIP_DEVICE_COMPILE__
   float f = __fmul_rn(__fsub_rn(xp, origin), inv_dx);
   return (int)floorf(__fsub_rn(f, 0.5f));

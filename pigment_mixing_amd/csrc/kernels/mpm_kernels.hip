@@ -542,13 +542,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         axis_weights(x[1], sax[1], r1, sc.one_over_dx, w[1]);
         axis_weights(x[2], sax[2], r2, sc.one_over_dx, w[2]);
         float pic[3] = {0, 0, 0}, fl[3] = {0, 0, 0}, Bp[9], L[9], gp[NPIG], gs = 0.0f;
+        float R[3][3];   // [ok][r]: sum over (oi, oj) of Nx Ny v_r at z offset ok
 #pragma unroll
-        for (int c = 0; c < 9; ++c) { Bp[c] = 0.0f; L[c] = 0.0f; }
+        for (int c = 0; c < 9; ++c) { Bp[c] = 0.0f; L[c] = 0.0f; R[c / 3][c % 3] = 0.0f; }
 #pragma unroll
         for (int c = 0; c < NPIG; ++c) gp[c] = 0.0f;
         const float inv_dx = sc.one_over_dx;
-        // 9 stencil rows (oi, oj), each with its 3 nodes along z unrolled: keeps one
-        // row of gather loads and weights live instead of all 27 nodes.
+        // The stencil weights are separable, w = Nx Ny Nz, and so are the
+        // velocity moments: per row (oi, oj) the z-weighted sum A_r = sum_ok
+        // Nz v_r gives the row's share of pic, of the x and y columns of B
+        // (pd0, pd1 are constant along the row) and of L's x and y columns; the
+        // z columns come from R_ok = sum over rows of Nx Ny v_r.  The flip
+        // velocity, latent and shear use the full weight per node.
+        // 9 rows, each with its 3 nodes along z unrolled: one row of gather loads
+        // live at a time.
         const float zc[3] = {sax[2][r2] - x[2], sax[2][r2 + 1] - x[2], sax[2][r2 + 2] - x[2]};
 #pragma unroll 1
         for (int row = 0; row < 9; ++row) {
@@ -557,44 +564,59 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
           const float dNx = oi == 0 ? w[0].dN[0] : (oi == 1 ? w[0].dN[1] : w[0].dN[2]);
           const float Ny = oj == 0 ? w[1].N[0] : (oj == 1 ? w[1].N[1] : w[1].N[2]);
           const float dNy = oj == 0 ? w[1].dN[0] : (oj == 1 ? w[1].dN[1] : w[1].dN[2]);
-          const float pd0 = sax[0][r0 + oi] - x[0];
-          const float pd1 = sax[1][r1 + oj] - x[1];
           const float nij = Nx * Ny;
-          const float dij0 = dNx * Ny;
-          const float dij1 = Nx * dNy;
           const int trow = ((r0 + oi) * GT + (r1 + oj)) * GT + r2;
+          float A[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
           for (int ok = 0; ok < 3; ++ok) {
-            const float pd2 = zc[ok];
-            const float wgt = nij * w[2].N[ok];
-            const float g0 = (dij0 * w[2].N[ok]) * inv_dx;
-            const float g1 = (dij1 * w[2].N[ok]) * inv_dx;
-            const float g2 = (nij * w[2].dN[ok]) * inv_dx;
+            const float Nz = w[2].N[ok];
             const float4* gn = &gtile[(trow + ok) * GT_ROW];
-            const float4 n0 = gn[0], n1 = gn[1], n2 = gn[2], n3 = gn[3];
+            const float4 n0 = gn[0];
             const float gvv[3] = {n0.x, n0.y, n0.z};
-            const float gfl[3] = {n0.w, n1.x, n1.y};
-            const float gpg[NPIG] = {n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, n3.x};
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
-              if constexpr (PHASE != 2) {
-                pic[r] += gvv[r] * wgt;
-                Bp[3 * r + 0] += (gvv[r] * pd0) * wgt;
-                Bp[3 * r + 1] += (gvv[r] * pd1) * wgt;
-                Bp[3 * r + 2] += (gvv[r] * pd2) * wgt;
-                fl[r] += gfl[r] * wgt;
-              }
-              L[3 * r + 0] += gvv[r] * g0;
-              L[3 * r + 1] += gvv[r] * g1;
-              L[3 * r + 2] += gvv[r] * g2;
+              A[r] += Nz * gvv[r];
+              R[ok][r] += nij * gvv[r];
             }
             if constexpr (PHASE != 2) {
+              const float4 n1 = gn[1], n2 = gn[2], n3 = gn[3];
+              const float gfl[3] = {n0.w, n1.x, n1.y};
+              const float gpg[NPIG] = {n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, n3.x};
+              const float wgt = nij * Nz;
+#pragma unroll
+              for (int r = 0; r < 3; ++r) fl[r] += gfl[r] * wgt;
 #pragma unroll
               for (int c = 0; c < NPIG; ++c) gp[c] += gpg[c] * wgt;
               gs += n3.y * wgt;
             }
           }
+          const float dij0 = dNx * Ny, dij1 = Nx * dNy;
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            L[3 * r + 0] += dij0 * A[r];
+            L[3 * r + 1] += dij1 * A[r];
+          }
+          if constexpr (PHASE != 2) {
+            const float b0 = (sax[0][r0 + oi] - x[0]) * nij, b1 = (sax[1][r1 + oj] - x[1]) * nij;
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+              pic[r] += nij * A[r];
+              Bp[3 * r + 0] += b0 * A[r];
+              Bp[3 * r + 1] += b1 * A[r];
+            }
+          }
         }
+#pragma unroll
+        for (int ok = 0; ok < 3; ++ok) {
+          const float b2 = zc[ok] * w[2].N[ok], l2 = w[2].dN[ok];
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            if constexpr (PHASE != 2) Bp[3 * r + 2] += b2 * R[ok][r];
+            L[3 * r + 2] += l2 * R[ok][r];
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < 9; ++c) L[c] *= inv_dx;
         if (sc.flip_ratio >= 0.0f) {
 #pragma unroll
           for (int c = 0; c < 9; ++c) Bp[c] = 0.0f;  // PIC/FLIP: Bmat unused
@@ -1010,14 +1032,21 @@ __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[2
     const float x0 = act ? cur.x[0] : it.X[0][1];
     const float x1 = act ? cur.x[1] : it.X[1][1];
     const float x2 = act ? cur.x[2] : it.X[2][1];
-    float N[9];
+    float u[9], N[9];
 #pragma unroll
     for (int o = 0; o < 3; ++o) {
-      const float n0 = spline_N(__fmul_rn(__fsub_rn(x0, it.X[0][o]), inv_dx));
-      N[o] = act ? n0 : 0.0f;
-      N[3 + o] = spline_N(__fmul_rn(__fsub_rn(x1, it.X[1][o]), inv_dx));
-      N[6 + o] = spline_N(__fmul_rn(__fsub_rn(x2, it.X[2][o]), inv_dx));
+      u[o] = spline_u(x0, it.X[0][o], inv_dx);
+      u[3 + o] = spline_u(x1, it.X[1][o], inv_dx);
+      u[6 + o] = spline_u(x2, it.X[2][o], inv_dx);
     }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      N[3 * a] = spline_N_at<0>(u[3 * a]);
+      N[3 * a + 1] = spline_N_at<1>(u[3 * a + 1]);
+      N[3 * a + 2] = spline_N_at<2>(u[3 * a + 2]);
+    }
+#pragma unroll
+    for (int o = 0; o < 3; ++o) N[o] = act ? N[o] : 0.0f;
     if constexpr (G == 0) {
 #pragma unroll
       for (int n = 0; n < 27; ++n) {
@@ -1060,10 +1089,10 @@ __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[2
         // f_r = tau_r0 dNx Ny Nz + tau_r1 Nx dNy Nz + tau_r2 Nx Ny dNz (gradient / dx)
         float dN[9];
 #pragma unroll
-        for (int o = 0; o < 3; ++o) {
-          dN[o] = spline_dN(__fmul_rn(__fsub_rn(x0, it.X[0][o]), inv_dx)) * inv_dx;
-          dN[3 + o] = spline_dN(__fmul_rn(__fsub_rn(x1, it.X[1][o]), inv_dx)) * inv_dx;
-          dN[6 + o] = spline_dN(__fmul_rn(__fsub_rn(x2, it.X[2][o]), inv_dx)) * inv_dx;
+        for (int a = 0; a < 3; ++a) {
+          dN[3 * a] = spline_dN_at<0>(u[3 * a]) * inv_dx;
+          dN[3 * a + 1] = spline_dN_at<1>(u[3 * a + 1]) * inv_dx;
+          dN[3 * a + 2] = spline_dN_at<2>(u[3 * a + 2]) * inv_dx;
         }
         float tau[6];
 #pragma unroll
