@@ -56,8 +56,11 @@ typedef enum {
   MPMX_PLATE_RIGHT = 3, MPMX_PLATE_FRONT = 4, MPMX_PLATE_BACK = 5
 } mpmx_plate_type;
 
-/* Plastic models on the path (reference enum class PlasticModel, tools.hpp:57). */
-typedef enum { MPMX_PLASTIC_NONE = 0, MPMX_PLASTIC_VMVISC = 5 } mpmx_plastic_model;
+/* Plastic models on the path (reference enum class PlasticModel, tools.hpp:57):
+ * VMVisc, von Mises/Perzyna (plasticity.cpp:163-238), the benchmark material;
+ * DPVisc, Drucker-Prager/Perzyna with Pradhana's volume correction
+ * (plasticity.cpp:240-340), the paint app's material (simulation_units.cpp:127-129). */
+typedef enum { MPMX_PLASTIC_NONE = 0, MPMX_PLASTIC_VMVISC = 5, MPMX_PLASTIC_DPVISC = 6 } mpmx_plastic_model;
 
 /* Return-map closure used inside the step. */
 typedef enum {
@@ -119,6 +122,7 @@ typedef struct {
   int   closure;             /* mpmx_closure                                          */
   float closure_tol;         /* relative yield-residual tolerance for the fallback    */
   int   n_threads;           /* CPU threads (oracle/baseline only; ignored on GPU)    */
+  int   use_pradhana;        /* DPVisc volume correction (simulation.hpp:86, default 1) */
 } mpmx_config;
 
 /* Axis-aligned plate — ObjectPlate (matter/src/objects/object_plate.hpp:13-78). */
@@ -217,6 +221,14 @@ int mpmx_upload_particles(mpmx_handle* h, int64_t n, const float* x, const float
                           const float* diff, const float* eps_pl_dev,
                           const uint8_t* active);
 
+/* The volumetric plastic strains eps_pl_vol and eps_pl_vol_pradhana of a
+ * scene's particles (Particles, data_structures.hpp:30-39; zero when not
+ * given), for a restart from a saved state.  Call after the scene's particle
+ * upload and before mpmx_prepare; scene 0 for a single-scene handle.  Either
+ * pointer may be NULL (zeros). */
+int mpmx_upload_plastic_state(mpmx_handle* h, int scene, const float* eps_pl_vol,
+                              const float* eps_pl_vol_pradhana);
+
 /* Plates and knives (Simulation::plates / objects, simulation.hpp:129-131;
  * setupScene adds the floor and the spatula, simulation_units.cpp:117-122).
  * At most 8 plates and 1 knife. */
@@ -270,6 +282,13 @@ int mpmx_synchronize(mpmx_handle* h);
 int mpmx_download_particles(mpmx_handle* h, float* x, float* v, float* F, float* C,
                             float* pig, float* diff, float* eps_pl_dev,
                             float* delta_gamma, uint8_t* active);
+
+/* The volumetric plastic strains in the caller's original particle order:
+ * eps_pl_vol and eps_pl_vol_pradhana (Particles, data_structures.hpp:30-39),
+ * written by the DPVisc map (plasticity.cpp:268-275) and the capped von Mises
+ * branch (plasticity.cpp:172-180).  Zero for scenes whose model does not track
+ * them.  Either pointer may be NULL. */
+int mpmx_download_plastic_state(mpmx_handle* h, float* eps_pl_vol, float* eps_pl_vol_pradhana);
 
 /* Renderer format (MPMIntegrationSim::recountParticles,
  * MPMIntegrationSim.cpp:100-116): pos4 = (x,y,z,1), lat7, diff — host arrays in

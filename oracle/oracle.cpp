@@ -269,6 +269,7 @@ struct Oracle {
   std::vector<M3> F, Bm;
   std::vector<L7> pig;
   std::vector<float> diff, eps_pl_dev, dgamma;
+  std::vector<float> eps_pl_vol, eps_prad;  // eps_pl_vol, eps_pl_vol_pradhana (data_structures.hpp:30-39)
   std::vector<uint8_t> active;
   float particle_mass = 0, particle_volume = 0, dx = 0;
   float lambda = 0, mu = 0, K = 0, d_prefac = 0, e_mu_prefac = 0, f_mu_prefac = 0;
@@ -776,63 +777,156 @@ struct Oracle {
   }
 
  public:
-  // plasticity.cpp:163-238 (VMVisc) on an already computed SVD.  Returns 1 if projected.
-  int vmvisc(const M3& U, const V3& sig, const M3& V, float eps_dev_in, M3& Fout, float& eps_out,
-             float& dg_out, int& err) const {
-    V3 h;
+  // Plastic state of one particle carried by the return maps.
+  struct PState {
+    float eps_dev = 0, eps_vol = 0, prad = 0, dg = 0;  // eps_pl_dev, eps_pl_vol, eps_pl_vol_pradhana, delta_gamma
+  };
+
+  // Perzyna overstress increment (plasticity.cpp:187-229 and 276-320): the
+  // closed form for perzyna_exp == 1 (with xi == 0 where hardening applies),
+  // else Newton on r(dgam) = (q_tr - f dgam)(dt/(visc dgam + dt))^n - q_y(dgam).
+  // `harden` selects VMVisc's softening yield q_y(eps + dgam/d); DPVisc's q_y
+  // is the fixed q_yield.
+  float perzyna_dgamma(float q_trial, float q_yield, bool closed, bool harden, float eps_dev_in,
+                       int& err) const {
+    float dgam;
+    if (closed) {
+      dgam = (q_trial - q_yield) / (f_mu_prefac + q_yield * cfg.perzyna_visc / cfg.dt);
+      if (dgam < 0) err = 1;
+      return dgam;
+    }
+    dgam = 0.01f * (q_trial - q_yield) / f_mu_prefac;
+    const int max_iter = 60;
+    for (int it = 0; it < max_iter; ++it) {
+      if (it == max_iter - 1) err = 1;
+      float tm = cfg.perzyna_visc * dgam + cfg.dt;
+      float tmp = cfg.dt / tm;
+      float tmp1 = std::pow(tmp, cfg.perzyna_exp);
+      float qyd = 0.0f;
+      if (harden) {
+        q_yield = cfg.q_min + (cfg.q_max - cfg.q_min) * std::exp(-cfg.xi * (eps_dev_in + (1.0f / d_prefac) * dgam));
+        qyd = -cfg.xi / d_prefac * (cfg.q_max - cfg.q_min) * std::exp(-cfg.xi * (eps_dev_in + (1.0f / d_prefac) * dgam));
+      }
+      float res = (q_trial - f_mu_prefac * dgam) * tmp1 - q_yield;
+      if (std::fabs(res) < 1e-2f) break;
+      float rd = -f_mu_prefac * tmp1 + (q_trial - f_mu_prefac * dgam) * cfg.perzyna_exp *
+                                           std::pow(tmp, cfg.perzyna_exp - 1) * (-cfg.perzyna_visc * cfg.dt) / (tm * tm) - qyd;
+      if (std::fabs(rd) < 1e-14f) err = 1;
+      dgam -= res / rd;
+      if (dgam < 0) dgam = 1e-10f;
+    }
+    return dgam;
+  }
+
+  static M3 f_from_hencky(const M3& U, const V3& h, const M3& V) {
+    M3 D = zero33();
+    for (int d = 0; d < 3; ++d) D(d, d) = std::exp(h[d]);
+    return matmul(matmul(U, D), transpose(V));
+  }
+
+  // plasticity.cpp:17-27: principal Hencky strain, its trace and unit deviator.
+  static void hencky_split(const V3& sig, V3& h, float& htr, V3& dev, float& dn) {
     for (int d = 0; d < 3; ++d) h[d] = std::log(std::max(std::fabs(sig[d]), 1e-4f));
-    float htr = (h[0] + h[1]) + h[2];
-    V3 dev = v3(h[0] - htr / 3, h[1] - htr / 3, h[2] - htr / 3);
-    float dn = norm(dev);
+    htr = (h[0] + h[1]) + h[2];
+    dev = v3(h[0] - htr / 3, h[1] - htr / 3, h[2] - htr / 3);
+    dn = norm(dev);
     if (dn > 0) dev = scale(dev, 1.0f / dn);
+  }
+
+  // plasticity.cpp:163-238 (VMVisc) on an already computed SVD.  Sets `changed`
+  // when F is replaced; returns 1 when the reference counts a projection
+  // (plastic_count++: not for the capped-pressure branch, :172-180).
+  int vmvisc(const M3& U, const V3& sig, const M3& V, const PState& in, M3& Fout, PState& out, bool& changed,
+             int& err) const {
+    V3 h, dev;
+    float htr, dn;
+    hencky_split(sig, h, htr, dev, dn);
     float q_trial = e_mu_prefac * dn;
-    float q_yield = cfg.q_min + (cfg.q_max - cfg.q_min) * std::exp(-cfg.xi * eps_dev_in);
+    float q_yield = cfg.q_min + (cfg.q_max - cfg.q_min) * std::exp(-cfg.xi * in.eps_dev);
     float p_trial = -K * htr;
-    eps_out = eps_dev_in;
-    dg_out = 0.0f;
-    if (p_trial < cfg.p_min * std::exp(-cfg.xi * 0.0f)) {
-      // capped branch (inactive for p_min = -1e20); eps_pl_vol not tracked
+    out = in;
+    out.dg = 0.0f;
+    changed = false;
+    if (p_trial < cfg.p_min * std::exp(-cfg.xi * in.eps_vol)) {
+      // capped von Mises: F = U V^T
       float dgam = q_trial / f_mu_prefac;
+      float inst = -p_trial / K;
       Fout = matmul(U, transpose(V));
-      eps_out = (float)((double)eps_dev_in + (1.0 / (double)d_prefac) * (double)dgam);
-      dg_out = dgam / cfg.dt;
-      return 1;
+      changed = true;
+      out.eps_vol = in.eps_vol + inst;
+      out.eps_dev = (float)((double)in.eps_dev + (1.0 / (double)d_prefac) * (double)dgam);
+      out.dg = dgam / cfg.dt;
+      return 0;
     }
     if (q_trial > q_yield) {
-      float dgam;
-      if (cfg.perzyna_exp == 1 && cfg.xi == 0) {
-        dgam = (q_trial - q_yield) / (f_mu_prefac + q_yield * cfg.perzyna_visc / cfg.dt);
-        if (dgam < 0) err = 1;
-      } else {
-        dgam = 0.01f * (q_trial - q_yield) / f_mu_prefac;
-        const int max_iter = 60;
-        for (int it = 0; it < max_iter; ++it) {
-          if (it == max_iter - 1) err = 1;
-          float tm = cfg.perzyna_visc * dgam + cfg.dt;
-          float tmp = cfg.dt / tm;
-          float tmp1 = std::pow(tmp, cfg.perzyna_exp);
-          q_yield = cfg.q_min + (cfg.q_max - cfg.q_min) * std::exp(-cfg.xi * (eps_dev_in + (1.0f / d_prefac) * dgam));
-          float res = (q_trial - f_mu_prefac * dgam) * tmp1 - q_yield;
-          if (std::fabs(res) < 1e-2f) break;
-          float qyd = -cfg.xi / d_prefac * (cfg.q_max - cfg.q_min) * std::exp(-cfg.xi * (eps_dev_in + (1.0f / d_prefac) * dgam));
-          float rd = -f_mu_prefac * tmp1 + (q_trial - f_mu_prefac * dgam) * cfg.perzyna_exp *
-                                               std::pow(tmp, cfg.perzyna_exp - 1) * (-cfg.perzyna_visc * cfg.dt) / (tm * tm) - qyd;
-          if (std::fabs(rd) < 1e-14f) err = 1;
-          dgam -= res / rd;
-          if (dgam < 0) dgam = 1e-10f;
-        }
-      }
+      float dgam = perzyna_dgamma(q_trial, q_yield, cfg.perzyna_exp == 1 && cfg.xi == 0, true, in.eps_dev, err);
       // (1.0/d_prefac) is a double in the reference; Eigen converts the product
       // to the float scalar type before scaling the vector (plasticity.cpp:232-234).
       float fac = (float)((1.0 / (double)d_prefac) * (double)dgam);
-      V3 hn = sub(h, scale(dev, fac));
-      M3 D = zero33();
-      for (int d = 0; d < 3; ++d) D(d, d) = std::exp(hn[d]);
-      Fout = matmul(matmul(U, D), transpose(V));
-      eps_out = (float)((double)eps_dev_in + (1.0 / (double)d_prefac) * (double)dgam);
-      dg_out = dgam / cfg.dt;
+      Fout = f_from_hencky(U, sub(h, scale(dev, fac)), V);
+      changed = true;
+      out.eps_dev = (float)((double)in.eps_dev + (1.0 / (double)d_prefac) * (double)dgam);
+      out.dg = dgam / cfg.dt;
       return 1;
     }
+    return 0;
+  }
+
+  // plasticity.cpp:240-340 (DPVisc; use_mibf is off, as prepareSimulation sets
+  // it, simulation_units.cpp:185).  Pradhana's correction shifts the yield
+  // surface by the volume gained at the tip (p_shift = -K eps_pl_vol_pradhana),
+  // resets the accumulation on every state right of the shifted tip.
+  int dpvisc(const M3& U, const V3& sig, const M3& V, const PState& in, M3& Fout, PState& out, bool& changed,
+             int& err) const {
+    V3 h, dev;
+    float htr, dn;
+    hencky_split(sig, h, htr, dev, dn);
+    float p_trial = -K * htr;
+    float q_trial = e_mu_prefac * dn;
+    float p_tip = -cfg.q_cohesion / cfg.M;
+    float p_shift = 0.0f;
+    if (cfg.use_pradhana) p_shift = -K * in.prad;
+    float q_yield = cfg.M * (p_trial + p_shift) + cfg.q_cohesion;
+    out = in;
+    changed = false;
+    int count = 0;
+    if ((p_trial + p_shift) < p_tip) {
+      // left of the shifted tip: project to the cohesion-only tip
+      float dgam = d_prefac * dn;
+      float p_proj = p_tip;
+      float inst = (p_proj - p_trial) / K;
+      count = 1;
+      float hv = -p_proj / (K * 3);
+      Fout = f_from_hencky(U, v3(hv, hv, hv), V);
+      changed = true;
+      out.dg = dgam / cfg.dt;
+      out.eps_dev = (float)((double)in.eps_dev + (1.0 / (double)d_prefac) * (double)dgam);
+      out.eps_vol = in.eps_vol + inst;
+      if (cfg.use_pradhana) out.prad = in.prad + inst;
+    } else {
+      if (cfg.use_pradhana) out.prad = 0.0f;
+      out.dg = 0.0f;
+    }
+    if ((p_trial + p_shift) > p_tip && q_trial > q_yield) {
+      count = 1;
+      float dgam = perzyna_dgamma(q_trial, q_yield, cfg.perzyna_exp == 1, false, in.eps_dev, err);
+      float fac = (float)((1.0 / (double)d_prefac) * (double)dgam);
+      Fout = f_from_hencky(U, sub(h, scale(dev, fac)), V);
+      changed = true;
+      out.eps_dev = (float)((double)in.eps_dev + (1.0 / (double)d_prefac) * (double)dgam);
+      out.dg = dgam / cfg.dt;
+    }
+    return count;
+  }
+
+  // plasticity.cpp:10-27 dispatch on the model.
+  int return_map(const M3& U, const V3& sig, const M3& V, const PState& in, M3& Fout, PState& out, bool& changed,
+                 int& err) const {
+    if (cfg.plastic_model == MPMX_PLASTIC_VMVISC) return vmvisc(U, sig, V, in, Fout, out, changed, err);
+    if (cfg.plastic_model == MPMX_PLASTIC_DPVISC) return dpvisc(U, sig, V, in, Fout, out, changed, err);
+    out = in;
+    out.dg = 0.0f;
+    changed = false;
     return 0;
   }
 
@@ -862,15 +956,18 @@ struct Oracle {
       M3 Ft;
       for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) Ft(r, c) = F[p](r, c) + prod(r, c);
       F[p] = Ft;
-      if (cfg.plastic_model == MPMX_PLASTIC_VMVISC) {
+      if (cfg.plastic_model != MPMX_PLASTIC_NONE) {
         M3 U, V;
         V3 s;
         svd3_jacobi(Ft, U, s, V);
         M3 Fo;
-        float eo, dg;
+        PState in, out;
+        in.eps_dev = eps_pl_dev[p]; in.eps_vol = eps_pl_vol[p]; in.prad = eps_prad[p];
+        bool changed = false;
         int err = 0;
-        int pl = vmvisc(U, s, V, eps_pl_dev[p], Fo, eo, dg, err);
-        if (pl) { F[p] = Fo; eps_pl_dev[p] = eo; dgamma[p] = dg; ++pc; }
+        pc += return_map(U, s, V, in, Fo, out, changed, err);
+        if (changed) F[p] = Fo;
+        eps_pl_dev[p] = out.eps_dev; eps_pl_vol[p] = out.eps_vol; eps_prad[p] = out.prad; dgamma[p] = out.dg;
         err_any |= err;
       }
     }
@@ -936,6 +1033,9 @@ struct Oracle {
 // ======================================================================= C API
 extern "C" {
 
+int orc_return_map_state(const mpmx_config* cfg, int64_t n, const float* Ft9, const float* in3, float* F9,
+                         float* out4, int32_t* changed, int32_t* counted);
+
 void* orc_create(const mpmx_config* cfg) {
   Oracle* o = new Oracle();
   o->cfg = *cfg;
@@ -952,6 +1052,7 @@ int orc_upload(void* h, int64_t n, const float* x, const float* v, const float* 
   o->x.resize(n); o->v.resize(n); o->pic.assign(n, zero3()); o->flip.assign(n, zero3());
   o->F.resize(n); o->Bm.resize(n); o->pig.resize(n); o->diff.resize(n); o->eps_pl_dev.resize(n);
   o->dgamma.assign(n, 0.0f); o->active.resize(n);
+  o->eps_pl_vol.assign(n, 0.0f); o->eps_prad.assign(n, 0.0f);
   for (int64_t p = 0; p < n; ++p) {
     for (int d = 0; d < 3; ++d) {
       o->x[p][d] = x[3 * p + d];
@@ -1097,9 +1198,23 @@ float orc_knife_sdf(int shape, const float* half, float offset, const float* p) 
   return sdf_box(p, half);
 }
 
-// VMVisc return map on F_trial (n particles): writes F, eps, dgamma, plastic flag.
+// The configured return map (VMVisc or DPVisc) on F_trial (n particles):
+// writes F, eps_pl_dev, dgamma and a flag set where F was replaced.
 int orc_return_map(const mpmx_config* cfg, int64_t n, const float* Ft9, const float* eps_in, float* F9,
                    float* eps_out, float* dg_out, int32_t* plastic) {
+  std::vector<float> in3(3 * (size_t)n, 0.0f), out4(4 * (size_t)n);
+  for (int64_t p = 0; p < n; ++p) in3[3 * p] = eps_in ? eps_in[p] : 0.0f;
+  int rc = orc_return_map_state(cfg, n, Ft9, in3.data(), F9, out4.data(), plastic, nullptr);
+  for (int64_t p = 0; p < n; ++p) { eps_out[p] = out4[4 * p]; dg_out[p] = out4[4 * p + 3]; }
+  return rc;
+}
+
+// Same with the whole plastic state: in3 = (eps_pl_dev, eps_pl_vol,
+// eps_pl_vol_pradhana) per particle, out4 = the same three after the map and
+// delta_gamma.  changed[p]: F replaced; counted[p] (may be NULL): the
+// reference's plastic_count increment.
+int orc_return_map_state(const mpmx_config* cfg, int64_t n, const float* Ft9, const float* in3, float* F9,
+                         float* out4, int32_t* changed, int32_t* counted) {
   Oracle o;
   o.cfg = *cfg;
   o.dx = cfg->dx;
@@ -1110,17 +1225,39 @@ int orc_return_map(const mpmx_config* cfg, int64_t n, const float* Ft9, const fl
     V3 s;
     for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) A(r, c) = Ft9[9 * p + 3 * r + c];
     svd3_jacobi(A, U, s, V);
-    float eo, dg;
+    Oracle::PState in, out;
+    in.eps_dev = in3[3 * p]; in.eps_vol = in3[3 * p + 1]; in.prad = in3[3 * p + 2];
+    bool ch = false;
     int err = 0;
-    int pl = o.vmvisc(U, s, V, eps_in ? eps_in[p] : 0.0f, Fo, eo, dg, err);
-    if (!pl) Fo = A;
+    int cnt = o.return_map(U, s, V, in, Fo, out, ch, err);
+    if (!ch) Fo = A;
     for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) F9[9 * p + 3 * r + c] = Fo(r, c);
-    eps_out[p] = eo;
-    dg_out[p] = dg;
-    plastic[p] = pl;
+    out4[4 * p] = out.eps_dev; out4[4 * p + 1] = out.eps_vol; out4[4 * p + 2] = out.prad; out4[4 * p + 3] = out.dg;
+    changed[p] = ch ? 1 : 0;
+    if (counted) counted[p] = cnt;
     err_any |= err;
   }
   return err_any ? MPMX_ERR_DEVICE : 0;
+}
+
+// Set eps_pl_vol / eps_pl_vol_pradhana after orc_upload (NULL: zeros).
+int orc_upload_vol(void* h, const float* eps_vol, const float* prad) {
+  Oracle* o = (Oracle*)h;
+  for (int64_t p = 0; p < o->Np; ++p) {
+    o->eps_pl_vol[p] = eps_vol ? eps_vol[p] : 0.0f;
+    o->eps_prad[p] = prad ? prad[p] : 0.0f;
+  }
+  return 0;
+}
+
+// eps_pl_vol and eps_pl_vol_pradhana per particle (either may be NULL).
+int orc_download_vol(void* h, float* eps_vol, float* prad) {
+  Oracle* o = (Oracle*)h;
+  for (int64_t p = 0; p < o->Np; ++p) {
+    if (eps_vol) eps_vol[p] = o->eps_pl_vol[p];
+    if (prad) prad[p] = o->eps_prad[p];
+  }
+  return 0;
 }
 
 // Kirchhoff stress tau = P(F) F^T (HenckyPiola), for tests.

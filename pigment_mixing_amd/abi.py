@@ -19,7 +19,7 @@ c_i64_p = ctypes.POINTER(ctypes.c_int64)
 OK, ERR_ARG, ERR_HIP, ERR_STATE, ERR_DEVICE, ERR_CAPACITY, ERR_IO = range(7)
 BC_NOSLIP, BC_SLIPSTICK, BC_SLIPFREE = 0, 1, 2
 PLATE_TOP, PLATE_BOTTOM, PLATE_LEFT, PLATE_RIGHT, PLATE_FRONT, PLATE_BACK = range(6)
-PLASTIC_NONE, PLASTIC_VMVISC = 0, 5
+PLASTIC_NONE, PLASTIC_VMVISC, PLASTIC_DPVISC = 0, 5, 6
 CLOSURE_EXACT, CLOSURE_LEARNED = 0, 1
 KNIFE_SLAB, KNIFE_SPATULA = 0, 1
 MOTION_STATIC, MOTION_LINEAR, MOTION_CIRCULAR, MOTION_LINEAR_THEN_CIRCULAR, MOTION_KEYFRAMES = range(5)
@@ -51,6 +51,7 @@ class Config(ctypes.Structure):
         ("particle_boundary_max", ctypes.c_float * 3),
         ("closure", ctypes.c_int), ("closure_tol", ctypes.c_float),
         ("n_threads", ctypes.c_int),
+        ("use_pradhana", ctypes.c_int),
     ]
 
 
@@ -147,6 +148,8 @@ def _sig(lib):
         ("mpmx_synchronize", ctypes.c_int, [H]),
         ("mpmx_download_particles", ctypes.c_int,
          [H, c_float_p, c_float_p, c_float_p, c_float_p, c_float_p, c_float_p, c_float_p, c_float_p, c_u8_p]),
+        ("mpmx_download_plastic_state", ctypes.c_int, [H, c_float_p, c_float_p]),
+        ("mpmx_upload_plastic_state", ctypes.c_int, [H, ctypes.c_int, c_float_p, c_float_p]),
         ("mpmx_pack_render", ctypes.c_int, [H, c_float_p, c_float_p, c_float_p]),
         ("mpmx_pack_render_device", ctypes.c_int, [H, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
         ("mpmx_get_grid", ctypes.c_int, [H, c_float_p, c_float_p, c_float_p, c_float_p, c_float_p]),
@@ -203,6 +206,17 @@ def fptr(a):
 def default_config():
     c = Config()
     load().mpmx_config_default(ctypes.byref(c))
+    return c
+
+
+def load_config_json(sim_config_path=None, pigment_config_path=None, cfg=None):
+    """Config with sim_config.json / pigment_config.json applied as the
+    reference's setupScene reads them (mpmx_config_load_json)."""
+    c = default_config() if cfg is None else cfg
+    enc = lambda p: None if p is None else os.fsencode(p)
+    rc = load().mpmx_config_load_json(ctypes.byref(c), enc(sim_config_path), enc(pigment_config_path))
+    if rc:
+        raise MpmxError(rc, f"mpmx_config_load_json({sim_config_path}, {pigment_config_path})")
     return c
 
 
@@ -314,6 +328,11 @@ class Handle:
                                                  fptr(diff), fptr(eps),
                                                  None if act is None else act.ctypes.data_as(c_u8_p)), "upload")
 
+    def upload_plastic_state(self, eps_vol=None, prad=None, scene=0):
+        conv = lambda a: None if a is None else np.ascontiguousarray(a, np.float32)
+        self._chk(self.lib.mpmx_upload_plastic_state(self.h, scene, fptr(conv(eps_vol)), fptr(conv(prad))),
+                  "upload_plastic_state")
+
     def upload_scene(self, scene, x, v=None, F=None, C=None, pig=None, diff=None, eps=None, active=None):
         x = np.ascontiguousarray(x, np.float32)
         conv = lambda a: None if a is None else np.ascontiguousarray(a, np.float32)
@@ -346,6 +365,11 @@ class Handle:
         K = ctypes.pointer(knife) if knife is not None else None
         self._chk(self.lib.mpmx_set_objects(self.h, P, len(plates), K, 1 if knife is not None else 0), "set_objects")
 
+    def set_keyframes(self, frames7, fps=60.0, k=0):
+        """Knife keyframes {pos[3], quat_xyzw[4]} per frame (MOTION_KEYFRAMES)."""
+        f = _f32(frames7, (-1, 7))
+        self._chk(self.lib.mpmx_set_knife_keyframes(self.h, k, fptr(f), f.shape[0], fps), "set_keyframes")
+
     def prepare(self):
         self._chk(self.lib.mpmx_prepare(self.h), "prepare")
 
@@ -374,6 +398,10 @@ class Handle:
             self.h, fptr(out["x"]), fptr(out["v"]), fptr(out["F"]), fptr(out["C"]), fptr(out["pig"]),
             fptr(out["diff"]), fptr(out["eps"]), fptr(out["dgamma"]), out["active"].ctypes.data_as(c_u8_p)),
             "download")
+        out["eps_vol"] = np.zeros(n, np.float32)
+        out["prad"] = np.zeros(n, np.float32)
+        self._chk(self.lib.mpmx_download_plastic_state(self.h, fptr(out["eps_vol"]), fptr(out["prad"])),
+                  "download_plastic_state")
         return out
 
     def pack_render(self):

@@ -20,6 +20,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "mpmx.h"
 
 #ifndef MPMX_HD
 #define MPMX_HD __host__ __device__ __forceinline__
@@ -96,6 +97,10 @@ struct SceneDev {
   float gravity[3];
   float D_max, D_edge0, D_edge1, start_boost, end_boost, boost_factor;
   int plastic_model, use_musl, use_pb, closure;
+  // DPVisc (plasticity.cpp:240-340): M = tan(friction angle), cohesion, Pradhana
+  float M, q_cohesion, d_prefac;
+  int use_pradhana;
+  int track_vol;             // carry eps_pl_vol / eps_pl_vol_pradhana rows (DPVisc or a capped VMVisc)
   float pb_min[3], pb_max[3];
   float closure_tol;
   int n_plates;
@@ -503,51 +508,115 @@ MPMX_HD float perzyna_residual_slope(const SceneDev& sc, float q_trial, float dg
          qyd;
 }
 
-MPMX_HD int vmvisc_principal(const SceneDev& sc, float h[3], float& eps, float& dg_rate, uint32_t& err,
-                             int* newton_iters = nullptr) {
-  float htr = (h[0] + h[1]) + h[2];
+// Perzyna overstress increment dgam for trial stress q_trial against yield
+// q_yield (plasticity.cpp:187-229, 276-320): the closed form when `closed`, else
+// Newton on r(dgam) = (q_tr - f dgam)(dt/(visc dgam + dt))^n - q_y(dgam), with
+// VMVisc's softening q_y(eps + dgam/d) when `harden` (DPVisc's q_y is fixed).
+MPMX_HD float perzyna_dgamma(const SceneDev& sc, float q_trial, float q_yield, bool closed, bool harden, float eps,
+                             uint32_t& err, int* newton_iters) {
+  float dgam;
+  if (closed) {
+    dgam = (q_trial - q_yield) / (sc.f_mu_prefac + q_yield * sc.perzyna_visc / sc.dt);
+    if (dgam < 0.0f) err |= ERR_NEG_DGAMMA;
+    return dgam;
+  }
+  dgam = 0.01f * (q_trial - q_yield) / sc.f_mu_prefac;
+  const int max_iter = 60;
+  for (int it = 0; it < max_iter; ++it) {
+    if (newton_iters) ++*newton_iters;
+    if (it == max_iter - 1) err |= ERR_NEWTON;
+    float tm = sc.perzyna_visc * dgam + sc.dt;
+    float tmp = sc.dt / tm;
+    float tmp1 = powf(tmp, sc.perzyna_exp);
+    float qyd = 0.0f;
+    if (harden) {
+      float ex = expf(-sc.xi * (eps + sc.inv_d_prefac * dgam));
+      q_yield = sc.q_min + (sc.q_max - sc.q_min) * ex;
+      qyd = -sc.xi * sc.inv_d_prefac * (sc.q_max - sc.q_min) * ex;
+    }
+    float res = (q_trial - sc.f_mu_prefac * dgam) * tmp1 - q_yield;
+    if (fabsf(res) < 1e-2f) break;
+    float rd = -sc.f_mu_prefac * tmp1 +
+               (q_trial - sc.f_mu_prefac * dgam) * sc.perzyna_exp * powf(tmp, sc.perzyna_exp - 1.0f) *
+                   (-sc.perzyna_visc * sc.dt) / (tm * tm) - qyd;
+    if (fabsf(rd) < 1e-14f) err |= ERR_NEWTON;
+    dgam -= res / rd;
+    if (dgam < 0.0f) dgam = 1e-10f;
+  }
+  return dgam;
+}
+
+// Unit deviatoric direction of the principal Hencky strain h (plasticity.cpp:20-27).
+MPMX_HD void hencky_deviator(const float h[3], float& htr, float dv[3], float& dn) {
+  htr = (h[0] + h[1]) + h[2];
   float m = htr / 3.0f;
-  float dv[3] = {h[0] - m, h[1] - m, h[2] - m};
-  float dn = sqrtf(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+  dv[0] = h[0] - m; dv[1] = h[1] - m; dv[2] = h[2] - m;
+  dn = sqrtf(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
   if (dn > 0.0f) { float inv = 1.0f / dn; dv[0] *= inv; dv[1] *= inv; dv[2] *= inv; }
+}
+
+// VMVisc (plasticity.cpp:163-238).  Returns 1 when projected (counted as the
+// reference's plastic_count), 2 for the capped-pressure branch (F = U V^T,
+// not counted, :172-180), 0 when elastic.  eps_vol (eps_pl_vol, may be null:
+// taken as 0 and not updated) enters the cap p_min e^{-xi eps_vol}.
+MPMX_HD int vmvisc_principal(const SceneDev& sc, float h[3], float& eps, float& dg_rate, uint32_t& err,
+                             int* newton_iters = nullptr, float* eps_vol = nullptr) {
+  float htr, dv[3], dn;
+  hencky_deviator(h, htr, dv, dn);
   float q_trial = sc.e_mu_prefac * dn;
   float q_yield = sc.q_min + (sc.q_max - sc.q_min) * expf(-sc.xi * eps);
   float p_trial = -sc.K * htr;
   dg_rate = 0.0f;
-  if (p_trial < sc.p_min) {  // capped von Mises (plasticity.cpp:172-180): F = U V^T
+  const float cap = eps_vol ? sc.p_min * expf(-sc.xi * *eps_vol) : sc.p_min;
+  if (p_trial < cap) {
     float dgam = q_trial / sc.f_mu_prefac;
+    if (eps_vol) *eps_vol = *eps_vol + (-p_trial / sc.K);
     h[0] = h[1] = h[2] = 0.0f;
     eps = eps + sc.inv_d_prefac * dgam;
     dg_rate = dgam / sc.dt;
-    return 1;
+    return 2;
   }
   if (!(q_trial > q_yield)) return 0;
-  float dgam;
-  if (sc.perzyna_exp == 1.0f && sc.xi == 0.0f) {
-    dgam = (q_trial - q_yield) / (sc.f_mu_prefac + q_yield * sc.perzyna_visc / sc.dt);
-    if (dgam < 0.0f) err |= ERR_NEG_DGAMMA;
-  } else {
-    dgam = 0.01f * (q_trial - q_yield) / sc.f_mu_prefac;
-    const int max_iter = 60;
-    for (int it = 0; it < max_iter; ++it) {
-      if (newton_iters) ++*newton_iters;
-      if (it == max_iter - 1) err |= ERR_NEWTON;
-      float tm = sc.perzyna_visc * dgam + sc.dt;
-      float tmp = sc.dt / tm;
-      float tmp1 = powf(tmp, sc.perzyna_exp);
-      float ex = expf(-sc.xi * (eps + sc.inv_d_prefac * dgam));
-      q_yield = sc.q_min + (sc.q_max - sc.q_min) * ex;
-      float res = (q_trial - sc.f_mu_prefac * dgam) * tmp1 - q_yield;
-      if (fabsf(res) < 1e-2f) break;
-      float qyd = -sc.xi * sc.inv_d_prefac * (sc.q_max - sc.q_min) * ex;
-      float rd = -sc.f_mu_prefac * tmp1 +
-                 (q_trial - sc.f_mu_prefac * dgam) * sc.perzyna_exp * powf(tmp, sc.perzyna_exp - 1.0f) *
-                     (-sc.perzyna_visc * sc.dt) / (tm * tm) - qyd;
-      if (fabsf(rd) < 1e-14f) err |= ERR_NEWTON;
-      dgam -= res / rd;
-      if (dgam < 0.0f) dgam = 1e-10f;
-    }
+  float dgam = perzyna_dgamma(sc, q_trial, q_yield, sc.perzyna_exp == 1.0f && sc.xi == 0.0f, true, eps, err,
+                              newton_iters);
+  float fac = sc.inv_d_prefac * dgam;
+  h[0] -= fac * dv[0]; h[1] -= fac * dv[1]; h[2] -= fac * dv[2];
+  eps = eps + fac;
+  dg_rate = dgam / sc.dt;
+  return 1;
+}
+
+// DPVisc with Pradhana's correction (plasticity.cpp:240-340; use_mibf off as
+// the paint app sets it).  The trial pressure is shifted by the volume gained
+// at the tip, p_shift = -K eps_pl_vol_pradhana; left of the cohesion tip the
+// strain projects to the tip (isotropic h), right of it the Perzyna
+// overstress returns along the deviator and the accumulation resets.  Returns
+// 1 when projected (F = U e^h V^T with the returned h), else 0.
+MPMX_HD int dpvisc_principal(const SceneDev& sc, float h[3], float& eps, float& eps_vol, float& prad,
+                             float& dg_rate, uint32_t& err, int* newton_iters = nullptr) {
+  float htr, dv[3], dn;
+  hencky_deviator(h, htr, dv, dn);
+  const float p_trial = -sc.K * htr;
+  const float q_trial = sc.e_mu_prefac * dn;
+  const float p_tip = -sc.q_cohesion / sc.M;
+  const float p_shift = sc.use_pradhana ? -sc.K * prad : 0.0f;
+  const float ps = p_trial + p_shift;
+  const float q_yield = sc.M * ps + sc.q_cohesion;
+  dg_rate = 0.0f;
+  if (ps < p_tip) {
+    const float dgam = sc.d_prefac * dn;
+    const float inst = (p_tip - p_trial) / sc.K;
+    const float hv = -p_tip / (sc.K * 3.0f);
+    h[0] = h[1] = h[2] = hv;
+    eps = eps + sc.inv_d_prefac * dgam;
+    eps_vol = eps_vol + inst;
+    if (sc.use_pradhana) prad = prad + inst;
+    dg_rate = dgam / sc.dt;
+    return 1;
   }
+  if (sc.use_pradhana) prad = 0.0f;
+  if (!(ps > p_tip && q_trial > q_yield)) return 0;
+  float dgam = perzyna_dgamma(sc, q_trial, q_yield, sc.perzyna_exp == 1.0f, false, eps, err, newton_iters);
   float fac = sc.inv_d_prefac * dgam;
   h[0] -= fac * dv[0]; h[1] -= fac * dv[1]; h[2] -= fac * dv[2];
   eps = eps + fac;

@@ -98,6 +98,13 @@ struct mpmx_handle {
 
 namespace {
 
+// A scene carries the eps_pl_vol / eps_pl_vol_pradhana rows when its model reads
+// them: DPVisc (plasticity.cpp:240-340) or a von Mises map with an active
+// pressure cap (p_min > -1e19, plasticity.cpp:172-180).
+bool tracks_volume(const mpmx_config& c) {
+  return c.plastic_model == MPMX_PLASTIC_DPVISC || (c.plastic_model == MPMX_PLASTIC_VMVISC && c.p_min > -1e19f);
+}
+
 int fail(mpmx_handle* h, int code, const std::string& msg) {
   if (h) h->err = msg;
   return code;
@@ -214,6 +221,11 @@ void fill_scene_dev(Scene& s, int node_block_base, int axis_off) {
   d.end_boost = c.end_boost_time;
   d.boost_factor = c.boost_factor;
   d.plastic_model = c.plastic_model;
+  d.M = c.M;
+  d.q_cohesion = c.q_cohesion;
+  d.d_prefac = d_prefac;
+  d.use_pradhana = c.use_pradhana ? 1 : 0;
+  d.track_vol = tracks_volume(c) ? 1 : 0;
   d.use_musl = c.use_musl;
   d.use_pb = c.use_particle_boundaries;
   d.closure = c.closure;
@@ -414,6 +426,7 @@ void mpmx_config_default(mpmx_config* c) {
   c->closure = MPMX_CLOSURE_EXACT;
   c->closure_tol = 1e-4f;
   c->n_threads = 1;
+  c->use_pradhana = 1;
 }
 
 int mpmx_config_load_json(mpmx_config* c, const char* sim_path, const char* pig_path) {
@@ -444,6 +457,7 @@ int mpmx_config_load_json(mpmx_config* c, const char* sim_path, const char* pig_
       if (j.has("use_musl")) c->use_musl = (int)j.num("use_musl", 0.0f);
       if (j.has("gravity_y")) c->gravity[1] = j.num("gravity_y", c->gravity[1]);
       if (j.has("plastic_model")) c->plastic_model = (int)j.num("plastic_model", 0.0f);
+      if (j.has("use_pradhana")) c->use_pradhana = (int)j.num("use_pradhana", 1.0f);
     } else {
       // simulation_units.cpp:69-80
       c->cfl = 0.2f; c->flip_ratio = -0.8f; c->E = 5e5f; c->nu = 0.3f; c->rho = 1000.0f;
@@ -570,7 +584,12 @@ int gather_batch(mpmx_handle* h) {
 
 int check_config(const mpmx_config* cfg) {
   if (!(cfg->dx > 0) || !(cfg->dt > 0) || !(cfg->ppc > 0) || !(cfg->rho > 0)) return MPMX_ERR_ARG;
-  if (cfg->plastic_model != MPMX_PLASTIC_NONE && cfg->plastic_model != MPMX_PLASTIC_VMVISC) return MPMX_ERR_ARG;
+  if (cfg->plastic_model != MPMX_PLASTIC_NONE && cfg->plastic_model != MPMX_PLASTIC_VMVISC &&
+      cfg->plastic_model != MPMX_PLASTIC_DPVISC)
+    return MPMX_ERR_ARG;
+  if (cfg->plastic_model == MPMX_PLASTIC_DPVISC && !(cfg->M > 0)) return MPMX_ERR_ARG;
+  // the learned closure replaces the von Mises/Perzyna map without the pressure cap
+  if (cfg->closure != MPMX_CLOSURE_EXACT && tracks_volume(*cfg)) return MPMX_ERR_ARG;
   if (cfg->use_musl != 0 && cfg->use_musl != 1) return MPMX_ERR_ARG;
   if (cfg->closure != MPMX_CLOSURE_EXACT && cfg->closure != MPMX_CLOSURE_LEARNED &&
       cfg->closure != MPMX_CLOSURE_LEARNED_REFINED)
@@ -610,6 +629,28 @@ int mpmx_upload_scene(mpmx_handle* h, int scene, int64_t n, const float* x, cons
   stage(s.comp, s.act, n, x, v, F, C, pig, diff, eps, active);
   s.p_count = n;
   h->np = 0;
+  h->prepared = false;
+  return MPMX_OK;
+}
+
+int mpmx_upload_plastic_state(mpmx_handle* h, int scene, const float* eps_pl_vol, const float* eps_pl_vol_pradhana) {
+  if (!h) return MPMX_ERR_ARG;
+  if (scene < 0 || scene >= (int)h->scenes.size()) return fail(h, MPMX_ERR_ARG, "upload_plastic_state: no such scene");
+  std::vector<float>* comp;
+  int64_t n;
+  if (h->batch) {
+    comp = &h->scenes[scene].comp;
+    n = h->scenes[scene].p_count;
+  } else {
+    comp = &h->h_comp;
+    n = h->np;
+  }
+  if (n <= 0 || comp->size() != (size_t)P_NCOMP * n)
+    return fail(h, MPMX_ERR_STATE, "upload_plastic_state: upload the scene's particles first");
+  for (int64_t p = 0; p < n; ++p) {
+    (*comp)[(size_t)P_EPSV * n + p] = eps_pl_vol ? eps_pl_vol[p] : 0.0f;
+    (*comp)[(size_t)P_PRAD * n + p] = eps_pl_vol_pradhana ? eps_pl_vol_pradhana[p] : 0.0f;
+  }
   h->prepared = false;
   return MPMX_OK;
 }
@@ -719,6 +760,8 @@ int mpmx_prepare(mpmx_handle* h) {
   d.n_scenes = ns;
   d.dead_key = nb_total;
   d.nb_list_cap = nb_total;
+  d.track_vol = 0;
+  for (const Scene& sc : h->scenes) d.track_vol |= sc.dev.track_vol;
   int rc;
 #define ALLOC(ptr, type, count)                         \
   if ((rc = dalloc(h, (type**)&(ptr), (count))) != 0) return rc;
@@ -916,6 +959,28 @@ int mpmx_download_particles(mpmx_handle* h, float* x, float* v, float* F, float*
     if (hipMemcpyAsync(active, tmp, n, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
         hipStreamSynchronize(h->stream) != hipSuccess)
       rc = fail(h, MPMX_ERR_HIP, "download copy failed");
+  }
+  (void)hipFree(tmp);
+  return rc;
+}
+
+int mpmx_download_plastic_state(mpmx_handle* h, float* eps_pl_vol, float* eps_pl_vol_pradhana) {
+  if (!h) return MPMX_ERR_ARG;
+  if (!h->prepared) return fail(h, MPMX_ERR_STATE, "download: prepare first");
+  const int64_t n = h->np;
+  const DevPtrs d = current(h);
+  float* tmp = nullptr;
+  HIPCHK(h, hipMalloc(&tmp, sizeof(float) * n));
+  struct Item { float* out; int row; } items[] = {{eps_pl_vol, P_EPSV}, {eps_pl_vol_pradhana, P_PRAD}};
+  int rc = MPMX_OK;
+  for (const Item& it : items) {
+    if (!it.out) continue;
+    launch_unsort(d, n, tmp, it.row, 1, h->stream);
+    if (hipMemcpyAsync(it.out, tmp, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess) {
+      rc = fail(h, MPMX_ERR_HIP, "download copy failed");
+      break;
+    }
   }
   (void)hipFree(tmp);
   return rc;

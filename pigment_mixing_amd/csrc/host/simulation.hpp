@@ -29,7 +29,7 @@ typedef std::array<float, 7> Latent;
 
 enum class BC { NoSlip = MPMX_BC_NOSLIP, SlipStick = MPMX_BC_SLIPSTICK, SlipFree = MPMX_BC_SLIPFREE };
 enum class PlateType { top = 0, bottom = 1, left = 2, right = 3, front = 4, back = 5 };
-enum class PlasticModel { NoPlasticity = MPMX_PLASTIC_NONE, VMVisc = MPMX_PLASTIC_VMVISC };
+enum class PlasticModel { NoPlasticity = MPMX_PLASTIC_NONE, VMVisc = MPMX_PLASTIC_VMVISC, DPVisc = MPMX_PLASTIC_DPVISC };
 
 // Particles (data_structures.hpp:9-50): the fields the path and the renderer use.
 struct Particles {
@@ -37,6 +37,7 @@ struct Particles {
   std::vector<TV> x, v;
   std::vector<TM> F, Bmat;
   std::vector<T> eps_pl_dev, delta_gamma, diffusion_factor;
+  std::vector<T> eps_pl_vol, eps_pl_vol_pradhana;
   std::vector<Latent> pigments;
   void resize(size_t n) {
     active.assign(n, 1);
@@ -45,6 +46,8 @@ struct Particles {
     F.assign(n, TM{1, 0, 0, 0, 1, 0, 0, 0, 1});
     Bmat.assign(n, TM{});
     eps_pl_dev.assign(n, 0);
+    eps_pl_vol.assign(n, 0);
+    eps_pl_vol_pradhana.assign(n, 0);
     delta_gamma.assign(n, 0);
     diffusion_factor.assign(n, 0);
     pigments.assign(n, Latent{});
@@ -181,6 +184,8 @@ class Simulation {
     std::vector<uint8_t> act(Np);
     check(mpmx_download_particles(h_, x.data(), v.data(), F.data(), C.data(), pig.data(), diff.data(), eps.data(),
                                   dg.data(), act.data()),
+          "download");
+    check(mpmx_download_plastic_state(h_, particles.eps_pl_vol.data(), particles.eps_pl_vol_pradhana.data()),
           "download");
     for (unsigned p = 0; p < Np; ++p) {
       for (int d = 0; d < 3; ++d) {

@@ -441,7 +441,7 @@ __device__ __forceinline__ void f_from_principal(const float U[9], const float h
 // Four wavefronts per SIMD (128 VGPRs; the fused phase spills a few dwords):
 // the gather's latency hiding is worth more than the spills (measured ~20%
 // faster than three wavefronts at 137 VGPRs).
-template <bool LEARNED, int PHASE>
+template <bool LEARNED, int PHASE, bool VOL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_g2p(DevPtrs d, const SceneDev* __restrict__ scenes,
                                              const float* __restrict__ axis, const float* __restrict__ A,
                                              const int* __restrict__ perm, const float* __restrict__ gv,
@@ -702,10 +702,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             }
           }
         }
-        if (!deferred && sc.plastic_model == 5) plastic = vmvisc_principal(sc, h, eps, dgr, err, &newton);
+        // VOL: the eps_pl_vol / eps_pl_vol_pradhana rows, read and written for
+        // the scenes whose model tracks them (DPVisc, capped von Mises)
+        float epsv = 0.0f, prad = 0.0f;
+        if constexpr (VOL) {
+          if (sc.track_vol) { epsv = A[P_EPSV * cap + src]; prad = A[P_PRAD * cap + src]; }
+          if (sc.plastic_model == MPMX_PLASTIC_VMVISC)
+            plastic = vmvisc_principal(sc, h, eps, dgr, err, &newton, sc.track_vol ? &epsv : nullptr);
+          else if (sc.plastic_model == MPMX_PLASTIC_DPVISC)
+            plastic = dpvisc_principal(sc, h, eps, epsv, prad, dgr, err, &newton);
+        } else {
+          if (!deferred && sc.plastic_model == MPMX_PLASTIC_VMVISC)
+            plastic = vmvisc_principal(sc, h, eps, dgr, err, &newton);
+        }
         if (!valid) newton = 0;
         if (err && valid) atomicOr(&d.err[0], err);
-        if (plastic && valid && d.harvest) harvest_pair(d, sc, h_trial, eps_in, h, dgr);
+        if (plastic == 1 && valid && d.harvest) harvest_pair(d, sc, h_trial, eps_in, h, dgr);
         float Fn[9];
         if (plastic) {
           f_from_principal(U, h, V, Fn);
@@ -761,6 +773,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             Bf[P_DG * cap + j] = dgr;
 #pragma unroll
             for (int c = 0; c < 6; ++c) Bf[(P_TAU + c) * cap + j] = tau[c];
+            if constexpr (VOL) {
+              if (sc.track_vol) { Bf[P_EPSV * cap + j] = epsv; Bf[P_PRAD * cap + j] = prad; }
+            }
           }
           if constexpr (PHASE == 0) {
 #pragma unroll
@@ -773,7 +788,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         }
       }
       count_for_sort(d, valid, newkey, newcell, j);
-      const uint64_t pm = __ballot(valid && plastic);
+      const uint64_t pm = __ballot(valid && plastic == 1);
       if (__lane_id() == 0 && pm) atomicAdd(&plastic_wg, (unsigned)__popcll(pm));
       if (newton) atomicAdd(&newton_wg, (unsigned)newton);
     }
@@ -843,7 +858,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     else if (sc.plastic_model == 5) plastic = vmvisc_principal(sc, h, eps, dgr, err, &newton);
     if (valid) {
       if (err) atomicOr(&d.err[0], err);
-      if (plastic && d.harvest) harvest_pair(d, sc, h_trial, eps_in, h, dgr);
+      if (plastic == 1 && d.harvest) harvest_pair(d, sc, h_trial, eps_in, h, dgr);
       float Fn[9];
       if (plastic) f_from_principal(U, h, V, Fn);
       else {
@@ -859,7 +874,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
 #pragma unroll
       for (int c = 0; c < 6; ++c) Bf[(P_TAU + c) * cap + j] = tau[c];
     }
-    const uint64_t pm = __ballot(valid && plastic);
+    const uint64_t pm = __ballot(valid && plastic == 1);
     const uint64_t lm = __ballot(valid && ok);
     const uint64_t fm = __ballot(valid && cand && !ok);
     if (__lane_id() == 0) {
@@ -1546,16 +1561,21 @@ void launch_g2p(const DevPtrs& d, int grid, int learned, int phase, hipStream_t 
   const int* pm = d.perm;
   const float* gv = d.gv;
   const int4* pd = d.pdesc;
+  // the learned closure never runs with volume tracking (refused at create)
   if (phase == 1)
-    hipLaunchKernelGGL((k_g2p<false, 1>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, pd);
+    hipLaunchKernelGGL((k_g2p<false, 1, false>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, pd);
   else if (phase == 2 && learned)
-    hipLaunchKernelGGL((k_g2p<true, 2>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, pd);
+    hipLaunchKernelGGL((k_g2p<true, 2, false>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, pd);
+  else if (phase == 2 && d.track_vol)
+    hipLaunchKernelGGL((k_g2p<false, 2, true>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, pd);
   else if (phase == 2)
-    hipLaunchKernelGGL((k_g2p<false, 2>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, pd);
+    hipLaunchKernelGGL((k_g2p<false, 2, false>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, pd);
   else if (learned)
-    hipLaunchKernelGGL((k_g2p<true, 0>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, pd);
+    hipLaunchKernelGGL((k_g2p<true, 0, false>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, pd);
+  else if (d.track_vol)
+    hipLaunchKernelGGL((k_g2p<false, 0, true>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, pd);
   else
-    hipLaunchKernelGGL((k_g2p<false, 0>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, pd);
+    hipLaunchKernelGGL((k_g2p<false, 0, false>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, pd);
   if (phase != 1) hipLaunchKernelGGL(k_copy_dead, dim3(64), dim3(256), 0, st, d);
 }
 void launch_closure(const DevPtrs& d, const void* mlp_blob, int grid, hipStream_t st) {

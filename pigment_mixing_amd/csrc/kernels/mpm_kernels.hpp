@@ -10,7 +10,9 @@ namespace mpmx {
 enum {
   P_X = 0, P_V = 3, P_C = 6, P_F = 15, P_PIG = 24, P_DIFF = 31, P_EPS = 32, P_DG = 33,
   P_TAU = 34,  // symmetric Kirchhoff stress of the current F: xx, yy, zz, xy, xz, yz
-  P_NCOMP = 40
+  P_EPSV = 40,  // eps_pl_vol (data_structures.hpp:30), carried where a model tracks it
+  P_PRAD = 41,  // eps_pl_vol_pradhana (data_structures.hpp:39), DPVisc's volume correction
+  P_NCOMP = 42
 };
 constexpr int GV_STRIDE = 16;  // floats per node in the gather grid (v3 flip3 pig7 shear mass pad)
 
@@ -81,6 +83,7 @@ struct DevPtrs {
   int nb_list_cap;
   int dead_key;
   int tile_cap;            // listed blocks with a tile; later ones scatter with atomics
+  int track_vol;           // some scene carries the eps_pl_vol / pradhana rows (G2P variant)
 };
 constexpr int CQ_REC = 3;              // float4 per closure-queue record: {slot, scene, eps, F0} {F1..F4} {F5..F8}
 // The queue is split into one segment per XCD walker: segment x starts at the

@@ -122,6 +122,68 @@ def paint_4blob(cells=(64, 8, 64), dims=(256, 64, 256), steps=500, dx=DX, seed=4
                 plates=[floor_plate()], knife=kn, steps=steps)
 
 
+def spatula_keyframes(p0, velocity, tilt_deg=30.0, yaw_rate=0.5, n=13, fps=60.0):
+    """Synthetic knife animation in the format of the reference's
+    spatula_motion.bin (object_spatula.hpp:235-250): n frames at fps of
+    {pos[3], quat_xyzw[4]}.  The centre translates at `velocity`; the blade,
+    tilted by tilt_deg about x, turns about y at yaw_rate rad/s."""
+    frames = np.zeros((n, 7), np.float32)
+    a = math.radians(tilt_deg) / 2
+    qx = np.array([math.sin(a), 0.0, 0.0, math.cos(a)])
+    for i in range(n):
+        t = i / fps
+        frames[i, :3] = np.asarray(p0) + t * np.asarray(velocity)
+        b = 0.5 * yaw_rate * t
+        qy = np.array([0.0, math.sin(b), 0.0, math.cos(b)])
+        # q = qy * qx (Hamilton product, xyzw)
+        x1, y1, z1, w1 = qy
+        x2, y2, z2, w2 = qx
+        frames[i, 3:] = [w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2, w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2,
+                         w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2, w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2]
+    return frames
+
+
+def paint_app_2blob(cells=(6, 4, 6), dims=(24, 16, 24), dx=DX, seed=42, stretch=60.0, shear=30.0):
+    """The paint app's own step (simulation_units.cpp:28-186): sim_config.json's
+    material (E 5e5, nu 0.2, rho 1000, friction angle 5 deg, cohesion 500 Pa,
+    perzyna_visc 0.4 s) with the Drucker-Prager/Perzyna map and Pradhana's
+    correction, MUSL, and the trapezoid spatula on keyframes (SlipFree).  Two
+    blobs start with a velocity field of uniform expansion `stretch` (1/s) plus
+    simple shear `shear` (1/s) about their centre, so that within 100 steps
+    the map sees both the tension tip and the Perzyna overstress."""
+    cfgdir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "config")
+    c = abi.load_config_json(os.path.join(cfgdir, "sim_config.json"), os.path.join(cfgdir, "pigment_config.json"))
+    c.plastic_model = abi.PLASTIC_DPVISC
+    c.use_pradhana = 1
+    c.use_musl = 1
+    c.gravity[:] = [0.0, -9.81, 0.0]
+    c.dx = dx
+    c.ppc = 8.0
+    c.grid_dims[:] = list(dims)
+    c.origin_mode = 0
+    c.dt = 1e-5
+    c.fps = 60.0
+    c.closure = abi.CLOSURE_EXACT
+    cx, cy, cz = cells
+    specs = [((0, 0, 0), cells, "cobalt_blue"), ((cx, 0, 0), cells, "hansa_yellow")]
+    x, pig = abi.scene_boxes(_boxes(specs, dx), dx, 2, seed)
+    ctr = np.array([cx * dx, 0.5 * cy * dx, 0.5 * cz * dx], np.float32)
+    r = x - ctr
+    v = (stretch * r).astype(np.float32)
+    v[:, 0] += shear * r[:, 1]
+    kn = abi.Knife()
+    kn.shape = abi.KNIFE_SPATULA
+    kn.half[:] = [0.006, 0.0015, 0.008]      # halfWidth, halfThickness, halfLength
+    kn.offset = 0.0005
+    kn.bc = abi.BC_SLIPFREE
+    kn.friction = 0.0                        # simulation_units.cpp:120
+    kn.motion = abi.MOTION_KEYFRAMES
+    p0 = (cx * dx - 0.006, 0.5 * cy * dx, 0.5 * cz * dx)
+    frames = spatula_keyframes(p0, (0.1, 0.0, 0.0))
+    return dict(name=f"paint_app_2blob_{dims[0]}x{dims[1]}x{dims[2]}", cfg=c, x=x, v=v, pig=pig,
+                plates=[floor_plate()], knife=kn, frames=frames, frame_fps=60.0, steps=100)
+
+
 def batch_scene(s, seed=42, np_range=(20_000, 400_000), dx=DX):
     """Scene s of the batched workload (its own seed 42 + s)."""
     rng = np.random.default_rng(seed + s)
@@ -170,7 +232,9 @@ def make_batch_handle(scene_list):
 
 def make_handle(scene):
     h = abi.Handle(scene["cfg"])
-    h.upload(scene["x"], pig=scene["pig"])
+    h.upload(scene["x"], v=scene.get("v"), pig=scene["pig"])
     h.set_objects(scene["plates"], scene["knife"])
+    if scene.get("frames") is not None:
+        h.set_keyframes(scene["frames"], scene.get("frame_fps", 60.0))
     h.prepare()
     return h

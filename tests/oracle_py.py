@@ -52,6 +52,10 @@ def _load(path):
         ("orc_return_map", ctypes.c_int, [ctypes.POINTER(abi.Config), ctypes.c_int64, F, F, F, F, F,
                                           ctypes.POINTER(ctypes.c_int32)]),
         ("orc_kirchhoff", None, [ctypes.POINTER(abi.Config), F, F]),
+        ("orc_return_map_state", ctypes.c_int, [ctypes.POINTER(abi.Config), ctypes.c_int64, F, F, F, F,
+                                                ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
+        ("orc_download_vol", ctypes.c_int, [H, F, F]),
+        ("orc_upload_vol", ctypes.c_int, [H, F, F]),
         ("orc_omp_max_threads", ctypes.c_int, []),
     ]
     for name, res, args in sig:
@@ -97,6 +101,10 @@ class Oracle:
         self.lib.orc_upload(self.h, self.n, fp(x), fp(v), fp(F_), fp(C), fp(pig), fp(diff), fp(eps),
                             None if act is None else act.ctypes.data_as(abi.c_u8_p))
 
+    def upload_vol(self, eps_vol=None, prad=None):
+        conv = lambda a: None if a is None else np.ascontiguousarray(a, np.float32)
+        self.lib.orc_upload_vol(self.h, fp(conv(eps_vol)), fp(conv(prad)))
+
     def set_objects(self, plates=(), knife=None):
         P = (abi.Plate * max(1, len(plates)))(*plates) if plates else None
         K = ctypes.pointer(knife) if knife is not None else None
@@ -121,6 +129,9 @@ class Oracle:
         self.lib.orc_download(self.h, fp(out["x"]), fp(out["v"]), fp(out["F"]), fp(out["C"]), fp(out["pig"]),
                               fp(out["diff"]), fp(out["eps"]), fp(out["dgamma"]),
                               out["active"].ctypes.data_as(abi.c_u8_p))
+        out["eps_vol"] = np.zeros(n, np.float32)
+        out["prad"] = np.zeros(n, np.float32)
+        self.lib.orc_download_vol(self.h, fp(out["eps_vol"]), fp(out["prad"]))
         return out
 
     def lattice(self):
@@ -154,8 +165,10 @@ def from_scene(scene, fast=False, n_threads=1):
     cfg = abi.Config.from_buffer_copy(scene["cfg"])
     cfg.n_threads = n_threads
     o = Oracle(cfg, fast=fast)
-    o.upload(scene["x"], pig=scene["pig"])
+    o.upload(scene["x"], v=scene.get("v"), pig=scene["pig"])
     o.set_objects(scene["plates"], scene["knife"])
+    if scene.get("frames") is not None:
+        o.set_keyframes(scene["frames"], scene.get("frame_fps", 60.0))
     o.prepare()
     return o
 
@@ -180,6 +193,24 @@ def return_map(cfg, Ft, eps=None):
     rc = lib().orc_return_map(ctypes.byref(cfg), n, fp(Ft), fp(e), fp(Fo), fp(eo), fp(dg),
                               pl.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
     return Fo, eo, dg, pl, rc
+
+
+def return_map_state(cfg, Ft, state3=None):
+    """The configured return map with the whole plastic state: state3 rows are
+    (eps_pl_dev, eps_pl_vol, eps_pl_vol_pradhana).  Returns F, out4 rows
+    (the three after the map, delta_gamma), changed (F replaced), counted
+    (plastic_count increment) and the status."""
+    Ft = np.ascontiguousarray(Ft, np.float32).reshape(-1, 3, 3)
+    n = Ft.shape[0]
+    st = np.zeros((n, 3), np.float32) if state3 is None else np.ascontiguousarray(state3, np.float32).reshape(n, 3)
+    Fo = np.zeros_like(Ft)
+    out4 = np.zeros((n, 4), np.float32)
+    ch = np.zeros(n, np.int32)
+    cnt = np.zeros(n, np.int32)
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    rc = lib().orc_return_map_state(ctypes.byref(cfg), n, fp(Ft), fp(st), fp(Fo), fp(out4),
+                                    ch.ctypes.data_as(i32p), cnt.ctypes.data_as(i32p))
+    return Fo, out4, ch, cnt, rc
 
 
 def kirchhoff(cfg, F_):

@@ -183,6 +183,115 @@ def test_parity_musl_paint_2blob_100_steps():
     check_particles(h.download(), o.download(), scenes.DX, "paint_2blob/musl/100")
 
 
+def test_parity_paint_app_dpvisc_musl_spatula_100_steps():
+    """The paint app's own step (simulation_units.cpp:28-186): MUSL, the
+    Drucker-Prager/Perzyna map with Pradhana's correction (plasticity.cpp:240-340)
+    and the trapezoid spatula on keyframes; the initial expansion + shear field
+    drives particles past the tension tip and through the Perzyna branch.
+    The tip is a threshold on the trial pressure, so this scene amplifies
+    rounding: the oracle's own response to a 1-ulp change of the initial
+    positions is 7e-4 relative in F and v after 100 steps.  Per-particle parity
+    to rounding is pinned from a common state by the restart test below; this
+    one bounds the accumulated drift: x <= 1e-3 dx, F <= 2e-3 relative,
+    v median <= 1e-3 and max <= 1e-2 relative (floor 0.1 m/s), latent <= 1e-5,
+    and the populations: plastic count within 1 %, particles that reached the
+    tip within 2 %, mean eps_pl_vol within 1 %."""
+    s = scenes.paint_app_2blob()
+    h, o = run_both(s, 100)
+    g, r = h.download(), o.download()
+    np.testing.assert_array_equal(g["active"], r["active"])
+    dx = scenes.DX
+    ex = np.linalg.norm(g["x"] - r["x"], axis=1)
+    ev = np.linalg.norm(g["v"] - r["v"], axis=1) / np.maximum(np.linalg.norm(r["v"], axis=1), 0.1)
+    eF = np.linalg.norm((g["F"] - r["F"]).reshape(-1, 9), axis=1) / np.linalg.norm(r["F"].reshape(-1, 9), axis=1)
+    ep = np.abs(g["pig"] - r["pig"]).max()
+    st, ost = h.stats(), o.stats()
+    tip_g, tip_r = (g["eps_vol"] != 0).sum(), (r["eps_vol"] != 0).sum()
+    log(f"[parity] paint_app/100: max|dx|/dx={ex.max() / dx:.2e} rel dv max {ev.max():.2e} median {np.median(ev):.2e} "
+        f"max rel dF={eF.max():.2e} max|dpig|={ep:.2e} plastic gpu={st.plastic_count} oracle={ost['plastic_count']} "
+        f"tip gpu={tip_g} oracle={tip_r} mean eps_vol gpu={g['eps_vol'].mean():.4e} oracle={r['eps_vol'].mean():.4e}")
+    assert ex.max() <= 1e-3 * dx
+    assert eF.max() <= 2e-3
+    assert np.median(ev) <= 1e-3 and ev.max() <= 1e-2
+    assert ep <= 1e-5
+    assert ost["plastic_count"] > 100 and tip_r > 100
+    assert abs(st.plastic_count - ost["plastic_count"]) <= max(5, 0.01 * ost["plastic_count"])
+    assert abs(int(tip_g) - int(tip_r)) <= 0.02 * tip_r
+    np.testing.assert_allclose(g["eps_vol"].mean(), r["eps_vol"].mean(), rtol=1e-2)
+
+
+def restart_pair(s, state, origin, knife=None):
+    """GPU handle and oracle both starting from `state` (a download) on the
+    lattice with the given origin (origin_mode 1), including the plastic state."""
+    cfg = abi.Config.from_buffer_copy(s["cfg"])
+    cfg.origin_mode = 1
+    cfg.grid_origin[:] = [float(v) for v in origin]
+    args = dict(v=state["v"], C=state["C"], pig=state["pig"], diff=state["diff"], eps=state["eps"],
+                active=state["active"])
+    h = abi.Handle(cfg)
+    h.upload(state["x"], F=state["F"], **args)
+    h.upload_plastic_state(state["eps_vol"], state["prad"])
+    h.set_objects(s["plates"], knife)
+    h.prepare()
+    o = orc.Oracle(cfg)
+    o.upload(state["x"], F_=state["F"], **args)
+    o.upload_vol(state["eps_vol"], state["prad"])
+    o.set_objects(s["plates"], knife)
+    o.prepare()
+    return h, o
+
+
+def test_parity_paint_app_dpvisc_restart_5_steps():
+    """Step-local parity of the DPVisc + MUSL step: the oracle runs the paint-app
+    scene 25 steps (1,599 particles at the tension tip, 1,563 with a Pradhana
+    accumulation), then the GPU and the oracle both restart from that state on
+    the same lattice and advance 5 steps.  From a common state the step agrees
+    to rounding: x <= 1e-5 dx, v <= 1e-4 relative (floor 0.1 m/s), F <= 2e-5
+    relative, latent <= 1e-6, wherever both sides took the same tip decision,
+    and at most 0.5 % of the particles (trial pressure within rounding of
+    p_tip) decide differently.  eps_pl_dev agrees to 1e-6 + 5e-3 of its change
+    over the 5 steps: just right of the tip the yield stress
+    q_y = M (p + p_shift) + c cancels to near zero, and the Perzyna denominator
+    f + q_y visc/dt turns a rounding of tr h (~1e-7) into ~1e-3 of dgam
+    (condition number K M visc / (dt f) ~ 9e3 for this material); the same
+    dgam moves F by up to ~1e-5.  eps_pl_vol and eps_pl_vol_pradhana are
+    volumetric strains of that F and agree to F's bar, 2e-5 absolute."""
+    s = scenes.paint_app_2blob()
+    o0 = orc.from_scene(s)
+    o0.step(25)
+    state = o0.download()
+    _, origin, *_ = o0.lattice()
+    log(f"[parity] restart state: tip {(state['eps_vol'] != 0).sum()} prad {(state['prad'] != 0).sum()}")
+    assert (state["prad"] != 0).sum() > 100
+    h, o = restart_pair(s, state, origin)
+    h.step(5)
+    o.step(5)
+    g, r = h.download(), o.download()
+    np.testing.assert_array_equal(g["active"], r["active"])
+    dx = scenes.DX
+    ex = np.linalg.norm(g["x"] - r["x"], axis=1)
+    ev = np.linalg.norm(g["v"] - r["v"], axis=1) / np.maximum(np.linalg.norm(r["v"], axis=1), 0.1)
+    eF = np.linalg.norm((g["F"] - r["F"]).reshape(-1, 9), axis=1) / np.linalg.norm(r["F"].reshape(-1, 9), axis=1)
+    same = (g["eps_vol"] != state["eps_vol"]) == (r["eps_vol"] != state["eps_vol"])
+    log(f"[parity] restart/5: max|dx|/dx={ex.max() / dx:.2e} max rel dv={ev.max():.2e} max rel dF={eF.max():.2e} "
+        f"tip decisions differ: {(~same).sum()}")
+    assert (~same).sum() <= 0.005 * len(same)
+    assert ex[same].max() <= 1e-5 * dx
+    assert ev[same].max() <= 1e-4
+    assert eF[same].max() <= 2e-5
+    dp = np.abs(g["pig"] - r["pig"]).max(axis=1)[same]
+    log(f"[parity] restart/5 pig: max |d| = {dp.max():.2e}")
+    assert dp.max() <= 1e-6
+    d = np.abs(g["eps"] - r["eps"])[same]
+    worst = np.max(d / (1e-6 + 5e-3 * np.abs(r["eps"] - state["eps"])[same]))
+    log(f"[parity] restart/5 eps: max |d| = {d.max():.2e}, max |d| / (1e-6 + 5e-3 |change|) = {worst:.2f}")
+    assert worst <= 1.0
+    for key in ("eps_vol", "prad"):
+        d = np.abs(g[key] - r[key])[same]
+        log(f"[parity] restart/5 {key}: max |d| = {d.max():.2e}")
+        assert d.max() <= 2e-5, key
+
+
 def test_grid_mass_independent_of_particle_and_atomic_order():
     """Same particles uploaded in two different orders: the first scatter runs
     with different lane assignments and atomic orders, yet node masses are
