@@ -290,6 +290,30 @@ int mpmx_download_particles(mpmx_handle* h, float* x, float* v, float* F, float*
  * them.  Either pointer may be NULL. */
 int mpmx_download_plastic_state(mpmx_handle* h, float* eps_pl_vol, float* eps_pl_vol_pradhana);
 
+/* Particle output as Simulation::saveParticleData (save_data.cpp:7-166):
+ * binary little-endian PLY, element `vertex` with one float property per
+ * field, in the caller's original particle order: x y z vx vy vz, pressure p
+ * = -tr(tau)/3, deviatoric stress q = q_prefac |dev tau| and Je = det F, then
+ * eps_pl_vol eps_pl_dev delta_gamma when a plastic model is set (tau is the
+ * Hencky Kirchhoff stress of F).  Single-scene handles. */
+int mpmx_write_particles_ply(mpmx_handle* h, const char* path);
+
+/* Grid output as Simulation::saveGridData (save_data.cpp:168-238): every node
+ * of the lattice in Simulation::ind order, x y z (node coordinates) vx vy vz
+ * (post-collision velocity of the last grid update) and mass. */
+int mpmx_write_grid_ply(mpmx_handle* h, const char* path);
+
+/* Resumable checkpoint of a single-scene handle: the whole particle state
+ * (including eps_pl_vol, the Pradhana accumulation and the Kirchhoff stress
+ * rows), the step count and clocks, the lattice (saved as an explicit origin
+ * and node counts), plates, knife and keyframes.  mpmx_checkpoint_load creates
+ * a prepared handle on the current device that continues from that step.
+ * The learned closure's network is not part of it (set it again after
+ * loading).  No counterpart in the reference, which restarts from PLY frames
+ * only. */
+int mpmx_checkpoint_save(mpmx_handle* h, const char* path);
+int mpmx_checkpoint_load(const char* path, mpmx_handle** out);
+
 /* Renderer format (MPMIntegrationSim::recountParticles,
  * MPMIntegrationSim.cpp:100-116): pos4 = (x,y,z,1), lat7, diff — host arrays in
  * original particle order. */

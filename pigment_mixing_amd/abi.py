@@ -150,6 +150,10 @@ def _sig(lib):
          [H, c_float_p, c_float_p, c_float_p, c_float_p, c_float_p, c_float_p, c_float_p, c_float_p, c_u8_p]),
         ("mpmx_download_plastic_state", ctypes.c_int, [H, c_float_p, c_float_p]),
         ("mpmx_upload_plastic_state", ctypes.c_int, [H, ctypes.c_int, c_float_p, c_float_p]),
+        ("mpmx_write_particles_ply", ctypes.c_int, [H, ctypes.c_char_p]),
+        ("mpmx_write_grid_ply", ctypes.c_int, [H, ctypes.c_char_p]),
+        ("mpmx_checkpoint_save", ctypes.c_int, [H, ctypes.c_char_p]),
+        ("mpmx_checkpoint_load", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(H)]),
         ("mpmx_pack_render", ctypes.c_int, [H, c_float_p, c_float_p, c_float_p]),
         ("mpmx_pack_render_device", ctypes.c_int, [H, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
         ("mpmx_get_grid", ctypes.c_int, [H, c_float_p, c_float_p, c_float_p, c_float_p, c_float_p]),
@@ -301,6 +305,31 @@ class Handle:
             self.n_scenes = 1
         self.cfg = cfg
         self.n = 0
+
+    @classmethod
+    def load_checkpoint(cls, path):
+        """A prepared handle continuing from a checkpoint file (mpmx_checkpoint_load)."""
+        self = cls.__new__(cls)
+        self.lib = load()
+        self.h = ctypes.c_void_p()
+        rc = self.lib.mpmx_checkpoint_load(os.fsencode(path), ctypes.byref(self.h))
+        if rc != OK:
+            raise MpmxError(rc, f"checkpoint_load({path})")
+        self.n_scenes = 1
+        self.cfg = None
+        st = Stats()
+        self._chk(self.lib.mpmx_stats_get(self.h, ctypes.byref(st)), "stats")
+        self.n = st.n_particles
+        return self
+
+    def save_checkpoint(self, path):
+        self._chk(self.lib.mpmx_checkpoint_save(self.h, os.fsencode(path)), "checkpoint_save")
+
+    def write_particles_ply(self, path):
+        self._chk(self.lib.mpmx_write_particles_ply(self.h, os.fsencode(path)), "write_particles_ply")
+
+    def write_grid_ply(self, path):
+        self._chk(self.lib.mpmx_write_grid_ply(self.h, os.fsencode(path)), "write_grid_ply")
 
     def _chk(self, rc, what):
         if rc != OK:

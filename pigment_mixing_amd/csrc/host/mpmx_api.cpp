@@ -59,6 +59,14 @@ struct mpmx_handle {
   double time = 0.0;
   float time_f = 0.0f;         // float accumulation as Simulation::time (simulation_units.cpp:228)
   int frame = 0;
+  // restart from a checkpoint: the step, clocks and stresses the next
+  // mpmx_prepare starts from (cleared by it and by every particle upload)
+  bool resume = false;
+  bool tau_given = false;      // staged Kirchhoff-stress rows are the saved ones: no recomputation
+  int64_t resume_step = 0;
+  double resume_time = 0.0;
+  float resume_time_f = 0.0f;
+  int resume_frame = 0;
   // device memory
   void* arena = nullptr;
   std::vector<void*> allocs;
@@ -607,6 +615,7 @@ int mpmx_upload_particles(mpmx_handle* h, int64_t n, const float* x, const float
   if (n <= 0 || !x) return fail(h, MPMX_ERR_ARG, "upload: need n > 0 and positions");
   if (n > (int64_t)0x7fffffff / 2) return fail(h, MPMX_ERR_CAPACITY, "upload: too many particles");
   h->np = n;
+  h->resume = h->tau_given = false;
   stage(h->h_comp, h->h_active, n, x, v, F, C, pig, diff, eps, active);
   h->h_pscene.assign(n, 0);
   h->scenes[0].p_begin = 0;
@@ -833,14 +842,15 @@ int mpmx_prepare(mpmx_handle* h) {
   HIPCHK(h, hipMemcpyAsync(psc, h->h_pscene.data(), sizeof(int) * n, hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(act, h->h_active.data(), n, hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(bsc, bscene.data(), sizeof(int) * h->nkeys, hipMemcpyHostToDevice, h->stream));
-  // ---- knife state for step 0, first scatter and grid update
-  h->step = 0;
-  h->time = 0.0;
-  h->time_f = 0.0f;
-  h->frame = 0;
+  // ---- knife state of the first step (0, or a checkpoint's), first scatter and grid update
+  h->step = h->resume ? h->resume_step : 0;
+  h->time = h->resume ? h->resume_time : 0.0;
+  h->time_f = h->resume ? h->resume_time_f : 0.0f;
+  h->frame = h->resume ? h->resume_frame : 0;
   h->parity_swapped = false;
-  if ((rc = upload_kstates(h, 0, 0, 0.0f)) != 0) return rc;
-  launch_init_tau(d, n, h->stream);
+  if ((rc = upload_kstates(h, h->step, 0, h->time_f)) != 0) return rc;
+  if (!h->tau_given) launch_init_tau(d, n, h->stream);
+  h->resume = h->tau_given = false;
   launch_keys(d, n, h->stream);
   launch_scan(d, h->nkeys, 0, h->stream);
   launch_cellscan(d, grid_grid(h), h->stream);
@@ -984,6 +994,214 @@ int mpmx_download_plastic_state(mpmx_handle* h, float* eps_pl_vol, float* eps_pl
   }
   (void)hipFree(tmp);
   return rc;
+}
+
+namespace {
+// Particle rows [first, first + ncomp) in upload order (interleaved per particle).
+int download_rows(mpmx_handle* h, int first, int ncomp, float* out) {
+  const int64_t n = h->np;
+  float* tmp = nullptr;
+  HIPCHK(h, hipMalloc(&tmp, sizeof(float) * ncomp * n));
+  launch_unsort(current(h), n, tmp, first, ncomp, h->stream);
+  int rc = MPMX_OK;
+  if (hipMemcpyAsync(out, tmp, sizeof(float) * ncomp * n, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+      hipStreamSynchronize(h->stream) != hipSuccess)
+    rc = fail(h, MPMX_ERR_HIP, "download copy failed");
+  (void)hipFree(tmp);
+  return rc;
+}
+
+// Checkpoint file: little-endian binary.
+//   char[8] "MPMXCKP1"; u32 sizeof(mpmx_config); mpmx_config (origin_mode 1, the
+//   lattice's origin and node counts); i64 n, step; f64 time; f32 time_f; i32 frame;
+//   i32 n_plates; mpmx_plate[n_plates]; i32 has_knife; mpmx_knife (if any);
+//   i32 n_frames; f32 fps; f32[7 n_frames];
+//   f32 x[3n] v[3n] F[9n] C[9n] pig[7n] diff[n] eps_pl_dev[n] delta_gamma[n]
+//   eps_pl_vol[n] eps_pl_vol_pradhana[n] tau[6n]; u8 active[n]   (upload order)
+constexpr char CKPT_MAGIC[8] = {'M', 'P', 'M', 'X', 'C', 'K', 'P', '1'};
+
+struct FileCloser {
+  FILE* f;
+  ~FileCloser() { if (f) std::fclose(f); }
+};
+}  // namespace
+
+int mpmx_checkpoint_save(mpmx_handle* h, const char* path) {
+  if (!h || !path) return MPMX_ERR_ARG;
+  if (h->batch) return fail(h, MPMX_ERR_STATE, "checkpoint: one scene per handle");
+  if (!h->prepared) return fail(h, MPMX_ERR_STATE, "checkpoint: prepare first");
+  const int64_t n = h->np;
+  std::vector<float> x(3 * n), v(3 * n), F(9 * n), C(9 * n), pig(7 * n), diff(n), eps(n), dg(n), epsv(n), prad(n),
+      tau(6 * n);
+  std::vector<uint8_t> act(n);
+  int rc = mpmx_download_particles(h, x.data(), v.data(), F.data(), C.data(), pig.data(), diff.data(), eps.data(),
+                                   dg.data(), act.data());
+  if (rc == MPMX_OK) rc = mpmx_download_plastic_state(h, epsv.data(), prad.data());
+  if (rc == MPMX_OK) rc = download_rows(h, P_TAU, 6, tau.data());
+  if (rc != MPMX_OK) return rc;
+  const Scene& s = h->scenes[0];
+  mpmx_config cfg = s.cfg;
+  cfg.origin_mode = 1;
+  for (int d = 0; d < 3; ++d) { cfg.grid_origin[d] = s.origin[d]; cfg.grid_dims[d] = s.dims[d]; }
+  FileCloser fc{std::fopen(path, "wb")};
+  if (!fc.f) return fail(h, MPMX_ERR_IO, std::string("checkpoint: cannot write ") + path);
+  bool ok = true;
+  auto put = [&](const void* p, size_t bytes) { ok = ok && (bytes == 0 || std::fwrite(p, 1, bytes, fc.f) == bytes); };
+  const uint32_t csize = sizeof(mpmx_config);
+  const int32_t frame = h->frame, np = (int32_t)s.plates.size(), hk = s.has_knife ? 1 : 0;
+  const int32_t nf = (int32_t)(s.frames.size() / 7);
+  put(CKPT_MAGIC, 8); put(&csize, 4); put(&cfg, sizeof cfg);
+  put(&n, 8); put(&h->step, 8); put(&h->time, 8); put(&h->time_f, 4); put(&frame, 4);
+  put(&np, 4); put(s.plates.data(), sizeof(mpmx_plate) * np);
+  put(&hk, 4); if (hk) put(&s.knife, sizeof(mpmx_knife));
+  put(&nf, 4); put(&s.frame_fps, 4); put(s.frames.data(), sizeof(float) * s.frames.size());
+  for (const std::vector<float>* a : {&x, &v, &F, &C, &pig, &diff, &eps, &dg, &epsv, &prad, &tau})
+    put(a->data(), sizeof(float) * a->size());
+  put(act.data(), act.size());
+  if (!ok) return fail(h, MPMX_ERR_IO, std::string("checkpoint: write failed: ") + path);
+  return MPMX_OK;
+}
+
+int mpmx_checkpoint_load(const char* path, mpmx_handle** out) {
+  if (!path || !out) return MPMX_ERR_ARG;
+  *out = nullptr;
+  FileCloser fc{std::fopen(path, "rb")};
+  if (!fc.f) return MPMX_ERR_IO;
+  bool ok = true;
+  auto get = [&](void* p, size_t bytes) { ok = ok && (bytes == 0 || std::fread(p, 1, bytes, fc.f) == bytes); };
+  char magic[8];
+  uint32_t csize = 0;
+  mpmx_config cfg;
+  get(magic, 8); get(&csize, 4);
+  if (!ok || std::memcmp(magic, CKPT_MAGIC, 8) != 0 || csize != sizeof(mpmx_config)) return MPMX_ERR_IO;
+  int64_t n = 0, step = 0;
+  double time = 0.0;
+  float time_f = 0.0f;
+  int32_t frame = 0, np = 0, hk = 0, nf = 0;
+  float fps = 60.0f;
+  get(&cfg, sizeof cfg); get(&n, 8); get(&step, 8); get(&time, 8); get(&time_f, 4); get(&frame, 4); get(&np, 4);
+  if (!ok || n <= 0 || n > (int64_t)0x7fffffff / 2 || np < 0 || np > MAX_PLATES || step < 0) return MPMX_ERR_IO;
+  std::vector<mpmx_plate> plates(np);
+  mpmx_knife knife;
+  get(plates.data(), sizeof(mpmx_plate) * np); get(&hk, 4);
+  if (hk) get(&knife, sizeof knife);
+  get(&nf, 4); get(&fps, 4);
+  if (!ok || nf < 0 || nf > (1 << 24)) return MPMX_ERR_IO;
+  std::vector<float> frames(7 * (size_t)nf);
+  get(frames.data(), sizeof(float) * frames.size());
+  std::vector<float> x(3 * n), v(3 * n), F(9 * n), C(9 * n), pig(7 * n), diff(n), eps(n), dg(n), epsv(n), prad(n),
+      tau(6 * n);
+  std::vector<uint8_t> act(n);
+  for (std::vector<float>* a : {&x, &v, &F, &C, &pig, &diff, &eps, &dg, &epsv, &prad, &tau})
+    get(a->data(), sizeof(float) * a->size());
+  get(act.data(), act.size());
+  if (!ok) return MPMX_ERR_IO;
+  mpmx_handle* h = nullptr;
+  int rc = mpmx_create(&cfg, &h);
+  if (rc != MPMX_OK) { mpmx_destroy(h); return rc; }
+  if ((rc = mpmx_upload_particles(h, n, x.data(), v.data(), F.data(), C.data(), pig.data(), diff.data(), eps.data(),
+                                  act.data())) != MPMX_OK ||
+      (rc = mpmx_upload_plastic_state(h, 0, epsv.data(), prad.data())) != MPMX_OK ||
+      (rc = mpmx_set_objects(h, plates.data(), np, hk ? &knife : nullptr, hk)) != MPMX_OK ||
+      (nf > 0 && (rc = mpmx_set_knife_keyframes(h, 0, frames.data(), nf, fps)) != MPMX_OK)) {
+    mpmx_destroy(h);
+    return rc;
+  }
+  for (int64_t p = 0; p < n; ++p) {
+    h->h_comp[(size_t)P_DG * n + p] = dg[p];
+    for (int c = 0; c < 6; ++c) h->h_comp[(size_t)(P_TAU + c) * n + p] = tau[6 * p + c];
+  }
+  h->tau_given = true;
+  h->resume = true;
+  h->resume_step = step;
+  h->resume_time = time;
+  h->resume_time_f = time_f;
+  h->resume_frame = frame;
+  if ((rc = mpmx_prepare(h)) != MPMX_OK) { mpmx_destroy(h); return rc; }
+  *out = h;
+  return MPMX_OK;
+}
+
+namespace {
+// Binary little-endian PLY with one `vertex` element of float properties,
+// values interleaved per vertex (the layout tinyply writes for the
+// reference's add_properties_to_element calls).
+int write_ply(mpmx_handle* h, const char* path, const std::vector<std::string>& names, int64_t n,
+              const std::vector<const float*>& cols, const std::vector<int>& strides) {
+  FileCloser fc{std::fopen(path, "wb")};
+  if (!fc.f) return fail(h, MPMX_ERR_IO, std::string("ply: cannot write ") + path);
+  std::string hdr = "ply\nformat binary_little_endian 1.0\nelement vertex " + std::to_string(n) + "\n";
+  for (const std::string& nm : names) hdr += "property float " + nm + "\n";
+  hdr += "end_header\n";
+  bool ok = std::fwrite(hdr.data(), 1, hdr.size(), fc.f) == hdr.size();
+  const size_t nc = names.size();
+  std::vector<float> row(nc * 4096);
+  for (int64_t p0 = 0; ok && p0 < n; p0 += 4096) {
+    const int64_t m = std::min<int64_t>(4096, n - p0);
+    for (int64_t p = 0; p < m; ++p)
+      for (size_t c = 0; c < nc; ++c) row[p * nc + c] = cols[c][(p0 + p) * strides[c]];
+    ok = std::fwrite(row.data(), sizeof(float) * nc, m, fc.f) == (size_t)m;
+  }
+  if (!ok) return fail(h, MPMX_ERR_IO, std::string("ply: write failed: ") + path);
+  return MPMX_OK;
+}
+}  // namespace
+
+int mpmx_write_particles_ply(mpmx_handle* h, const char* path) {
+  if (!h || !path) return MPMX_ERR_ARG;
+  if (h->batch) return fail(h, MPMX_ERR_STATE, "ply: one scene per handle");
+  if (!h->prepared) return fail(h, MPMX_ERR_STATE, "ply: prepare first");
+  const int64_t n = h->np;
+  std::vector<float> x(3 * n), v(3 * n), F(9 * n), eps(n), dg(n), epsv(n), tau(6 * n);
+  int rc = mpmx_download_particles(h, x.data(), v.data(), F.data(), nullptr, nullptr, nullptr, eps.data(), dg.data(),
+                                   nullptr);
+  if (rc == MPMX_OK) rc = mpmx_download_plastic_state(h, epsv.data(), nullptr);
+  if (rc == MPMX_OK) rc = download_rows(h, P_TAU, 6, tau.data());
+  if (rc != MPMX_OK) return rc;
+  // pressure, deviatoric stress and elastic volume ratio (save_data.cpp:13-31);
+  // tau is the Kirchhoff stress of the particle's current F
+  const float q_prefac = h->scenes[0].cfg.q_prefac;
+  std::vector<float> pr(n), q(n), je(n);
+  for (int64_t p = 0; p < n; ++p) {
+    const float* t = &tau[6 * p];   // xx yy zz xy xz yz
+    const float pressure = -((t[0] + t[1]) + t[2]) / 3;
+    const float a = t[0] + pressure, b = t[1] + pressure, c = t[2] + pressure;
+    const float dd = a * a + t[3] * t[3] + t[4] * t[4] + t[3] * t[3] + b * b + t[5] * t[5] + t[4] * t[4] +
+                     t[5] * t[5] + c * c;
+    pr[p] = pressure;
+    q[p] = q_prefac * std::sqrt(dd);
+    const float* f = &F[9 * p];
+    je[p] = f[0] * (f[4] * f[8] - f[5] * f[7]) - f[1] * (f[3] * f[8] - f[5] * f[6]) + f[2] * (f[3] * f[7] - f[4] * f[6]);
+  }
+  std::vector<std::string> names = {"x", "y", "z", "vx", "vy", "vz", "p", "q", "Je"};
+  std::vector<const float*> cols = {&x[0], &x[1], &x[2], &v[0], &v[1], &v[2], pr.data(), q.data(), je.data()};
+  std::vector<int> strides = {3, 3, 3, 3, 3, 3, 1, 1, 1};
+  if (h->scenes[0].cfg.plastic_model != MPMX_PLASTIC_NONE) {
+    for (const char* nm : {"eps_pl_vol", "eps_pl_dev", "delta_gamma"}) names.push_back(nm);
+    for (const float* c : {epsv.data(), eps.data(), dg.data()}) { cols.push_back(c); strides.push_back(1); }
+  }
+  return write_ply(h, path, names, n, cols, strides);
+}
+
+int mpmx_write_grid_ply(mpmx_handle* h, const char* path) {
+  if (!h || !path) return MPMX_ERR_ARG;
+  if (h->batch) return fail(h, MPMX_ERR_STATE, "ply: one scene per handle");
+  if (!h->prepared) return fail(h, MPMX_ERR_STATE, "ply: prepare first");
+  const Scene& s = h->scenes[0];
+  const int64_t nn = (int64_t)s.dims[0] * s.dims[1] * s.dims[2];
+  std::vector<float> mass(nn), v(3 * nn), X(3 * nn);
+  int rc = mpmx_get_grid(h, mass.data(), v.data(), nullptr, nullptr, nullptr);
+  if (rc != MPMX_OK) return rc;
+  int64_t c = 0;   // nodes in Simulation::ind order (i, j, k), k fastest (save_data.cpp:176-189)
+  for (int i = 0; i < s.dims[0]; ++i)
+    for (int j = 0; j < s.dims[1]; ++j)
+      for (int k = 0; k < s.dims[2]; ++k, ++c) {
+        X[3 * c] = s.xs[i];
+        X[3 * c + 1] = s.ys[j];
+        X[3 * c + 2] = s.zs[k];
+      }
+  return write_ply(h, path, {"x", "y", "z", "vx", "vy", "vz", "mass"}, nn,
+                   {&X[0], &X[1], &X[2], &v[0], &v[1], &v[2], mass.data()}, {3, 3, 3, 3, 3, 3, 1});
 }
 
 int mpmx_pack_render_device(mpmx_handle* h, void* d_pos4, void* d_lat7, void* d_diff) {

@@ -92,6 +92,7 @@ class Simulation {
   std::vector<ObjectPlate> plates;
   std::unique_ptr<ObjectKnife> knife;
   std::string sim_name;
+  std::string directory = "output/";   // simulation.hpp:205
 
   // ---- lifecycle (simulation_units.cpp:9-245)
   void initializeBasic(const std::string& name) {
@@ -177,6 +178,18 @@ class Simulation {
     return f != 0;
   }
   bool isFinished() const { return frame >= end_frame; }
+
+  // Simulation::saveParticleData / saveGridData (save_data.cpp:7-238): PLY
+  // files at the reference's paths, directory + sim_name + "/particles_f" +
+  // extra + frame + ".ply" (and "/grid_f...").
+  void saveParticleData(const std::string& extra = "") {
+    check(mpmx_write_particles_ply(h_, (directory + sim_name + "/particles_f" + extra + std::to_string(frame) + ".ply").c_str()),
+          "saveParticleData");
+  }
+  void saveGridData(const std::string& extra = "") {
+    check(mpmx_write_grid_ply(h_, (directory + sim_name + "/grid_f" + extra + std::to_string(frame) + ".ply").c_str()),
+          "saveGridData");
+  }
 
   // Refresh the host mirror of the particles (what the adapter reads).
   void syncParticles() {

@@ -4,6 +4,7 @@
 // finishes -> read particles.  Prints one JSON line for the test to check.
 #include <cstdio>
 #include <cmath>
+#include <sys/stat.h>
 #include <string>
 #include <vector>
 
@@ -49,11 +50,21 @@ int main(int argc, char** argv) {
     vmin = std::fmin(vmin, sim.particles.v[p][1]);
     act += sim.particles.active[p];
   }
+  // saveParticleData into <outdir>/api_demo/particles_f<frame>.ply
+  long ply_bytes = -1;
+  if (argc > 2) {
+    sim.directory = std::string(argv[2]) + "/";
+    mkdir((sim.directory + sim.sim_name).c_str(), 0755);
+    sim.saveParticleData();
+    struct stat st;
+    const std::string f = sim.directory + sim.sim_name + "/particles_f" + std::to_string(sim.frame) + ".ply";
+    if (stat(f.c_str(), &st) == 0) ply_bytes = (long)st.st_size;
+  }
   const auto gb = sim.getGridBoundaries();
   std::printf("{\"config_rc\": %d, \"Np\": %u, \"steps\": %d, \"frame\": %u, \"active\": %d, \"ymean\": %.9g, "
               "\"vymin\": %.9g, \"exit\": %d, \"pos_w\": %.3g, \"lo\": [%.6g, %.6g, %.6g], \"hi\": [%.6g, %.6g, %.6g], "
-              "\"time\": %.9g}\n",
+              "\"time\": %.9g, \"ply_bytes\": %ld}\n",
               rc, sim.Np, steps, sim.frame, act, ymean / sim.Np, vmin, sim.exit, pos4[0][3], gb.first[0], gb.first[1],
-              gb.first[2], gb.second[0], gb.second[1], gb.second[2], sim.time());
+              gb.first[2], gb.second[0], gb.second[1], gb.second[2], sim.time(), ply_bytes);
   return 0;
 }

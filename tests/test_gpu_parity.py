@@ -346,14 +346,15 @@ def test_mass_budget_paint_4blob():
     np.testing.assert_allclose(g["pig"][m, :4].sum(1), 1.0, atol=1e-5)
 
 
-def test_cpp_simulation_facade_lifecycle():
-    """The C++ Simulation facade (reference API) runs a frame on the device."""
+def test_cpp_simulation_facade_lifecycle(tmp_path):
+    """The C++ Simulation facade (reference API) runs a frame on the device and
+    writes it with saveParticleData."""
     import json
     import subprocess
     from pigment_mixing_amd import build as b
     repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-    r = subprocess.run([b.DEMO, os.path.join(repo, "pigment_mixing_amd", "config")], capture_output=True, text=True,
-                       timeout=120, cwd=repo)
+    r = subprocess.run([b.DEMO, os.path.join(repo, "pigment_mixing_amd", "config"), str(tmp_path)],
+                       capture_output=True, text=True, timeout=120, cwd=repo)
     assert r.returncode == 0, r.stderr
     out = json.loads(r.stdout.strip().splitlines()[-1])
     log(f"[facade] {out}")
@@ -362,3 +363,8 @@ def test_cpp_simulation_facade_lifecycle():
     assert out["steps"] == 100 and out["frame"] == 1      # fps 1000, dt 1e-5
     assert out["exit"] == 0 and out["pos_w"] == 1.0
     assert out["vymin"] < 0                              # falling under gravity
+    # x y z vx vy vz p q Je eps_pl_vol eps_pl_dev delta_gamma per particle after the header
+    with open(tmp_path / "api_demo" / f"particles_f{out['frame']}.ply", "rb") as f:
+        head = f.read(512)
+    hdr_len = head.index(b"end_header\n") + len(b"end_header\n")
+    assert out["ply_bytes"] == hdr_len + out["Np"] * 12 * 4

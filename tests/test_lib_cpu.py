@@ -145,3 +145,20 @@ def _lib_knife_state(k, dt, step):
     ks = abi.KnifeState()
     assert f(ctypes.byref(k), None, 0, 60.0, dt, step, ctypes.byref(ks)) == 0
     return dict(T=np.array(ks.T), Tinv=np.array(ks.Tinv), v=np.array(ks.v), w=np.array(ks.w))
+
+
+def test_checkpoint_load_rejects_missing_and_foreign_files(tmp_path):
+    """mpmx_checkpoint_load checks the file before it touches a device: a
+    missing file, a file of another format and a truncated checkpoint header
+    all return MPMX_ERR_IO and no handle."""
+    lib = abi.load()
+    h = ctypes.c_void_p()
+    assert lib.mpmx_checkpoint_load(str(tmp_path / "none.ckpt").encode(), ctypes.byref(h)) == abi.ERR_IO
+    assert not h
+    other = tmp_path / "other.ply"
+    other.write_bytes(b"ply\nformat binary_little_endian 1.0\nelement vertex 0\nend_header\n")
+    assert lib.mpmx_checkpoint_load(str(other).encode(), ctypes.byref(h)) == abi.ERR_IO
+    trunc = tmp_path / "trunc.ckpt"
+    trunc.write_bytes(b"MPMXCKP1" + (ctypes.sizeof(abi.Config)).to_bytes(4, "little") + bytes(16))
+    assert lib.mpmx_checkpoint_load(str(trunc).encode(), ctypes.byref(h)) == abi.ERR_IO
+    assert not h
