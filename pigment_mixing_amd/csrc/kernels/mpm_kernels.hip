@@ -500,6 +500,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     const int start = ds.y, cnt = ds.z;
     const int iters = (cnt + blockDim.x - 1) / blockDim.x;
     for (int it = 0; it < iters; ++it) {
+      // a wavefront whose lanes all lie past the block's count has nothing to
+      // do in this pass: it skips to the next block's barrier and leaves the
+      // SIMD to the other wavefronts
+      if (it * (int)blockDim.x + (int)(threadIdx.x & ~63u) >= cnt) continue;
       const int t = it * blockDim.x + threadIdx.x;
       const bool valid = t < cnt;
       // Every lane computes; a lane past the block's count recomputes the block's
@@ -1068,7 +1072,7 @@ __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[2
         const float wgt = (N[n / 9] * N[3 + (n / 3) % 3]) * N[6 + n % 3];
         unsigned long long m = ((unsigned long long)__float_as_uint(acc[n][1]) << 32) |
                                (unsigned long long)__float_as_uint(acc[n][0]);
-        m += wgt > 1e-25f ? mass_to_fx(wgt) : 0ull;
+        m += mass_to_fx(wgt > 1e-25f ? wgt : 0.0f);   // branch-free: mass_to_fx(0) = 0
         acc[n][0] = __uint_as_float((unsigned)m);
         acc[n][1] = __uint_as_float((unsigned)(m >> 32));
       }
@@ -1093,13 +1097,28 @@ __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[2
             az[o][r] = cz * pz;
           }
         }
+        // w (a + b + c) = (Nx Ny) (Nz (a + b) + Nz c): two FMAs per node and component
+        float nzaz[3][3];
 #pragma unroll
-        for (int n = 0; n < 27; ++n) {
-          const int oi = n / 9, oj = (n / 3) % 3, ok = n % 3;
-          const float wgt = N[oi] * NyNz[n % 9];
+        for (int ok = 0; ok < 3; ++ok)
 #pragma unroll
-          for (int r = 0; r < 3; ++r) acc[n][r] += ((ax[oi][r] + ay[oj][r]) + az[ok][r]) * wgt;
-        }
+          for (int r = 0; r < 3; ++r) nzaz[ok][r] = N[6 + ok] * az[ok][r];
+#pragma unroll
+        for (int oi = 0; oi < 3; ++oi)
+#pragma unroll
+          for (int oj = 0; oj < 3; ++oj) {
+            const float wxy = N[oi] * N[3 + oj];
+            float sxy[3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) sxy[r] = ax[oi][r] + ay[oj][r];
+#pragma unroll
+            for (int ok = 0; ok < 3; ++ok)
+#pragma unroll
+              for (int r = 0; r < 3; ++r) {
+                const int n = 9 * oi + 3 * oj + ok;
+                acc[n][r] = fmaf(wxy, fmaf(N[6 + ok], sxy[r], nzaz[ok][r]), acc[n][r]);
+              }
+          }
       } else if constexpr (G == 2) {
         // f_r = tau_r0 dNx Ny Nz + tau_r1 Nx dNy Nz + tau_r2 Nx Ny dNz (gradient / dx)
         float dN[9];
@@ -1129,7 +1148,7 @@ __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[2
           const int oi = n / 9, p = n % 9;
           const float a = NyNz[p], b = N[oi] * dyz[p], c = N[oi] * ydz[p];
 #pragma unroll
-          for (int r = 0; r < 3; ++r) acc[n][r] += tx[oi][r] * a + T[r][1] * b + T[r][2] * c;
+          for (int r = 0; r < 3; ++r) acc[n][r] = fmaf(T[r][2], c, fmaf(T[r][1], b, fmaf(tx[oi][r], a, acc[n][r])));
         }
       } else {
         // latent / shear: three (group 5: two) weighted scalars, q * Nx folded per offset
