@@ -158,7 +158,7 @@ __global__ void k_keys(DevPtrs d, int64_t n) {
 // non-empty live block list (ascending key order) and totals.  Two phases over
 // chunks of SCAN_CHUNK keys; the scanned 64-bit value packs (count<<32 | nonempty).
 constexpr int SCAN_THREADS = 256;
-constexpr int SCAN_PER_THREAD = 16;
+constexpr int SCAN_PER_THREAD = 2;
 constexpr int SCAN_CHUNK = SCAN_THREADS * SCAN_PER_THREAD;
 
 __device__ __forceinline__ unsigned long long scan_item(const DevPtrs& d, int k, int nkeys) {
@@ -1362,6 +1362,7 @@ __global__ void __launch_bounds__(256) k_grid(DevPtrs d) {
   const int parity = d.counters[CNT_GSTEP] & 1;
   if (blockIdx.x == 0 && threadIdx.x == 0) d.counters[CNT_NBLOCKS0 + (parity ^ 1)] = 0;
   const int nblk = d.counters[CNT_NBLOCKS0 + parity];
+  const bool scattered = d.counters[CNT_PBLOCKS] > d.tile_cap;   // P2G used its atomic path
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int nwaves = (gridDim.x * blockDim.x) >> 6;
   const int lane = threadIdx.x & 63;
@@ -1377,11 +1378,13 @@ __global__ void __launch_bounds__(256) k_grid(DevPtrs d) {
     float* acc = d.acc + (size_t)nb * ACC_CH * BLK3;
     // this node's sums: atomically scattered part (blocks past tile_cap) plus,
     // in a fixed order, the merged tiles of the up to 8 particle blocks whose
-    // stencils reach it (the block itself and its lower neighbours)
-    unsigned long long mfx = d.massfx[(size_t)nb * BLK3 + lane];
+    // stencils reach it (the block itself and its lower neighbours).  When
+    // every listed block had a tile, nothing was scattered: the accumulators
+    // are still zero and are neither read nor cleared.
+    unsigned long long mfx = scattered ? d.massfx[(size_t)nb * BLK3 + lane] : 0ull;
     float a[ACC_CH];
 #pragma unroll
-    for (int c = 0; c < ACC_CH; ++c) a[c] = acc[c * BLK3 + lane];
+    for (int c = 0; c < ACC_CH; ++c) a[c] = scattered ? acc[c * BLK3 + lane] : 0.0f;
     for (int q = 0; q < 8; ++q) {
       const int ox = q >> 2, oy = (q >> 1) & 1, oz = q & 1;
       const int pi = bi - ox, pj = bj - oy, pk = bk - oz;
@@ -1437,9 +1440,11 @@ __global__ void __launch_bounds__(256) k_grid(DevPtrs d) {
     g[1] = make_float4(out[4], out[5], out[6], out[7]);
     g[2] = make_float4(out[8], out[9], out[10], out[11]);
     g[3] = make_float4(out[12], out[13], out[14], out[15]);
+    if (scattered) {
 #pragma unroll
-    for (int c = 0; c < ACC_CH; ++c) acc[c * BLK3 + lane] = 0.0f;
-    d.massfx[(size_t)nb * BLK3 + lane] = 0ull;
+      for (int c = 0; c < ACC_CH; ++c) acc[c * BLK3 + lane] = 0.0f;
+      d.massfx[(size_t)nb * BLK3 + lane] = 0ull;
+    }
     if (lane == 0) d.nb_flag[nb] = 0;
   }
   __syncthreads();
@@ -1454,6 +1459,7 @@ __global__ void __launch_bounds__(256) k_grid(DevPtrs d) {
 __global__ void __launch_bounds__(256) k_grid_musl(DevPtrs d) {
   const int parity = d.counters[CNT_GSTEP] & 1;
   const int nblk = d.counters[CNT_NBLOCKS0 + parity];
+  const bool scattered = d.counters[CNT_PBLOCKS] > d.tile_cap;   // the second scatter used atomics
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int nwaves = (gridDim.x * blockDim.x) >> 6;
   const int lane = threadIdx.x & 63;
@@ -1466,7 +1472,7 @@ __global__ void __launch_bounds__(256) k_grid_musl(DevPtrs d) {
     float* acc = d.acc + (size_t)nb * ACC_CH * BLK3;
     float mom[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) mom[c] = acc[(A_MOM + c) * BLK3 + lane];
+    for (int c = 0; c < 3; ++c) mom[c] = scattered ? acc[(A_MOM + c) * BLK3 + lane] : 0.0f;
     for (int q = 0; q < 8; ++q) {
       const int ox = q >> 2, oy = (q >> 1) & 1, oz = q & 1;
       const int pi = bi - ox, pj = bj - oy, pk = bk - oz;
@@ -1491,8 +1497,10 @@ __global__ void __launch_bounds__(256) k_grid_musl(DevPtrs d) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) g[G_V + c] = 0.0f;
     }
+    if (scattered) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) acc[(A_MOM + c) * BLK3 + lane] = 0.0f;
+      for (int c = 0; c < 3; ++c) acc[(A_MOM + c) * BLK3 + lane] = 0.0f;
+    }
   }
 }
 

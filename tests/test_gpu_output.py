@@ -91,22 +91,26 @@ def test_grid_ply_matches_grid(tmp_path):
     assert (rec["mass"] > 0).sum() > 500
 
 
-def _state_close(a, b, dx, label):
+def _state_close(a, b, dx, label, v_tol, F_tol):
     np.testing.assert_array_equal(a["active"], b["active"], err_msg=label)
     ex = np.abs(a["x"] - b["x"]).max() / dx
     ev = (np.linalg.norm(a["v"] - b["v"], axis=1) / np.maximum(np.linalg.norm(b["v"], axis=1), 0.1)).max()
     eF = np.abs(a["F"] - b["F"]).max()
     print(f"[checkpoint] {label}: max|dx|/dx={ex:.2e} rel dv={ev:.2e} max|dF|={eF:.2e}", flush=True)
-    assert ex <= 1e-5 and ev <= 1e-4 and eF <= 2e-5, label
+    assert ex <= 1e-5 and ev <= v_tol and eF <= F_tol, label
 
 
 def test_checkpoint_restores_state_and_resumes(tmp_path):
     """Save after 20 steps, load into a new handle: the particle state comes
     back bit for bit (including the plastic rows), the node masses of the
     resumed first scatter equal the saved run's bit for bit (integer
-    accumulation), the clocks and the knife's keyframed state continue, and 20
+    accumulation), the clocks and the knife's keyframed state continue, and
     further steps agree with the uninterrupted run to rounding (the two runs
-    differ only in the order of particles inside a cell)."""
+    differ only in the order of particles inside a cell): 20 steps of the von
+    Mises scene to x <= 1e-5 dx, v <= 1e-4, F <= 2e-5; 5 steps of the
+    Drucker-Prager paint-app scene, whose near-tip Perzyna states amplify
+    rounding of tr h by ~1e4 (condition number K M visc / (dt f)), to
+    v <= 1e-3 and F <= 1e-4."""
     for s in (scenes.paint_2blob(cells=(6, 4, 6), dims=(24, 16, 24)), scenes.paint_app_2blob()):
         h = scenes.make_handle(s)
         h.step(20)
@@ -123,7 +127,8 @@ def test_checkpoint_restores_state_and_resumes(tmp_path):
             np.testing.assert_array_equal(loaded[key], saved[key], err_msg=f"{s['name']}: {key}")
         np.testing.assert_array_equal(r.grid()["mass"], grid_saved["mass"])
         assert bytes(h.knife_state(25)) == bytes(r.knife_state(25))
-        h.step(20 if s["cfg"].plastic_model == abi.PLASTIC_VMVISC else 5)
-        r.step(20 if s["cfg"].plastic_model == abi.PLASTIC_VMVISC else 5)
-        _state_close(r.download(), h.download(), scenes.DX, s["name"])
+        vm = s["cfg"].plastic_model == abi.PLASTIC_VMVISC
+        h.step(20 if vm else 5)
+        r.step(20 if vm else 5)
+        _state_close(r.download(), h.download(), scenes.DX, s["name"], 1e-4 if vm else 1e-3, 2e-5 if vm else 1e-4)
         assert r.stats().step == h.stats().step
