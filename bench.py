@@ -84,21 +84,24 @@ def cpu_baseline(scene, steps=2, threads=None):
 
 def timed_run(h, steps, warmup, tune=True):
     """Autotune the launch shapes (its steps are part of the untimed warm-up),
-    warm up, then time `steps` graph-launched steps."""
+    warm up, then time `steps` graph-launched steps.  Returns the wall time,
+    the stats and the chosen workgroups per CU."""
+    shape = None
     if tune:
-        h.autotune(2)
+        r = h.autotune(4)
+        shape = {"p2g": r.wg_p2g, "g2p": r.wg_g2p, "grid": r.wg_grid}
     h.step(warmup)
     h.sync()
     t0 = time.perf_counter()
     h.step(steps)
     h.sync()
     wall = time.perf_counter() - t0
-    return wall, h.stats()
+    return wall, h.stats(), shape
 
 
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                             "hbm_traffic_paint_4blob_256x64x256.json")
-TRAFFIC_NAMES = {"k_g2p": ("k_g2p<false, 0>", "k_g2p<true, 0>"), "k_p2g": ("k_p2g<false>",) * 2,
+TRAFFIC_NAMES = {"k_g2p": ("k_g2p<false, 0, false>", "k_g2p<true, 0, false>"), "k_p2g": ("k_p2g<false>",) * 2,
                  "k_grid": ("k_grid",) * 2}
 
 
@@ -186,7 +189,7 @@ def main():
 
     # ---- plain (exact return map)
     h = scenes.make_handle(scene)
-    wall_p, st_p = timed_run(h, args.steps, args.warmup)
+    wall_p, st_p, shape_p = timed_run(h, args.steps, args.warmup)
     value_p = st_p.n_active * args.steps / wall_p
     kt_p = h.profile(args.steps)
     rf_p = kernel_report(kt_p, learned=False, traffic=True)
@@ -223,7 +226,7 @@ def main():
         err_max = err_med = None
 
     # ---- learned closure run (same scene, same steps)
-    wall, st = timed_run(hl, args.steps, args.warmup)
+    wall, st, shape_l = timed_run(hl, args.steps, args.warmup)
     value = st.n_active * args.steps / wall
     print(f"[bench] learned: {args.steps} steps in {wall:.3f}s -> {value:.4g} particle-steps/s; last step plastic "
           f"{st.plastic_count} accepted {st.learned_count} fallback {st.fallback_count}; held-out stress err max "
@@ -243,7 +246,7 @@ def main():
         nb = sum(sc["x"].shape[0] for sc in sl)
         hb = scenes.make_batch_handle(sl)
         tune = hb.autotune(4)
-        wall_b, st_b = timed_run(hb, args.batch_steps, 5, tune=False)
+        wall_b, st_b, _ = timed_run(hb, args.batch_steps, 5, tune=False)
         kt_b = hb.profile(10)
         hb.close()
         batch = {"workload": "batch_64_scenes", "scenes": 64, "particles": nb,
@@ -285,9 +288,9 @@ def main():
             "knife": "slab 60x2x30 mm tilted 30 deg, circular sweep r=40 mm at 0.15 m/s",
             "parallelism": "single GPU",
         },
-        "plain": {"value": value_p, "ms_per_step": wall_p / args.steps * 1e3, "roofline": rf_p},
+        "plain": {"value": value_p, "ms_per_step": wall_p / args.steps * 1e3, "wg_per_cu": shape_p, "roofline": rf_p},
         "learned": {
-            "mode": "refined", "closure_tol": 1e-4,
+            "mode": "refined", "closure_tol": 1e-4, "wg_per_cu": shape_l,
             "speedup_vs_plain": value / value_p,
             "last_step": {"plastic": st_last.plastic_count, "accepted": st_last.learned_count,
                           "fallback": st_last.fallback_count},
