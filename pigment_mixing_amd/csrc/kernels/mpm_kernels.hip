@@ -35,21 +35,29 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 }
 
 // Rank of this lane's key among equal keys, one atomic per distinct key in the
-// wavefront.  Only lanes with `valid` participate; all lanes must call it.
+// wavefront (all issued by one instruction).  Only lanes with `valid`
+// participate; all lanes must call it.
 __device__ __forceinline__ int wave_rank(bool valid, int key, int* counts) {
+  // group the lanes by key (ballots only), then one vector atomic in which
+  // every group's first lane adds the group size, so the atomics' round
+  // trips overlap instead of following one another
   uint64_t todo = __ballot(valid);
-  int rank = 0;
+  int leader_of = 0, below = 0, size = 0;
   while (todo) {
     const int leader = __ffsll((long long)todo) - 1;
-    const int k0 = __shfl(key, leader);
+    const int k0 = __builtin_amdgcn_readlane(key, leader);
     const uint64_t same = __ballot(valid && key == k0);
-    int base = 0;
-    if (__lane_id() == leader) base = atomicAdd(&counts[k0], (int)__popcll(same));
-    base = __shfl(base, leader);
-    if (valid && key == k0) rank = base + (int)__popcll(same & lanemask_lt());
+    if (valid && key == k0) {
+      leader_of = leader;
+      below = (int)__popcll(same & lanemask_lt());
+    }
+    if ((int)__lane_id() == leader) size = (int)__popcll(same);
     todo &= ~same;
   }
-  return rank;
+  int base = 0;
+  if (size > 0) base = atomicAdd(&counts[key], size);
+  base = __shfl(base, leader_of);
+  return valid ? base + below : 0;
 }
 
 __device__ __forceinline__ void activate_node_block(const DevPtrs& d, int nb, int parity) {
