@@ -224,8 +224,10 @@ def batch_scenes(n=64, seed=42):
 def make_batch_handle(scene_list):
     h = abi.Handle([sc["cfg"] for sc in scene_list])
     for i, sc in enumerate(scene_list):
-        h.upload_scene(i, sc["x"], pig=sc["pig"])
+        h.upload_scene(i, sc["x"], v=sc.get("v"), pig=sc["pig"])
         h.set_objects_scene(i, sc["plates"], sc["knife"])
+        if sc.get("frames") is not None:
+            h.set_keyframes(sc["frames"], sc.get("frame_fps", 60.0), k=i)
     h.prepare()
     return h
 

@@ -60,3 +60,61 @@ def test_autotune_keeps_results():
     hb.step(30 + r.steps)
     assert ha.stats().step == hb.stats().step
     check_particles(ha.download(), hb.download(), scenes.DX, "autotuned vs default", v_tol=1e-3)
+
+
+def test_batch_mixing_drucker_prager_and_von_mises_scenes():
+    """A batch holding the paint app's Drucker-Prager scene (volume rows,
+    Pradhana accumulation) next to a von Mises scene (both MUSL, as a batch
+    shares the MUSL flag): each equals its own run.  The Drucker-Prager scene
+    starts from the oracle's step-25 state (1,599 particles at the tension tip,
+    restarted through mpmx_upload_plastic_state on an explicit lattice origin),
+    so that the comparison runs where the map is active without the
+    ill-conditioned transient before it; the von Mises scene runs the
+    volume-tracking G2P variant in the batch and the plain one alone, and
+    leaves the volume rows at zero.  Bars of the restart parity test: x <= 1e-5
+    dx, v <= 1e-4 relative (floor 0.1 m/s), F <= 2e-5 relative, eps_pl_vol and
+    the accumulator <= 2e-5, at most 0.5 % differing tip decisions (excluded)."""
+    app = scenes.paint_app_2blob()
+    o0 = orc.from_scene(app)
+    o0.step(25)
+    state = o0.download()
+    _, origin, *_ = o0.lattice()
+    cfg = abi.Config.from_buffer_copy(app["cfg"])
+    cfg.origin_mode = 1
+    cfg.grid_origin[:] = [float(v) for v in origin]
+    vm = scenes.batch_scene(1, seed=100, np_range=(3000, 9000))
+    vm["cfg"].use_musl = 1
+    args = dict(v=state["v"], F=state["F"], C=state["C"], pig=state["pig"], diff=state["diff"], eps=state["eps"],
+                active=state["active"])
+    hb = abi.Handle([cfg, vm["cfg"]])
+    hb.upload_scene(0, state["x"], **args)
+    hb.upload_plastic_state(state["eps_vol"], state["prad"], scene=0)
+    hb.set_objects_scene(0, app["plates"], None)
+    hb.upload_scene(1, vm["x"], pig=vm["pig"])
+    hb.set_objects_scene(1, vm["plates"], vm["knife"])
+    hb.prepare()
+    hb.step(5)
+    ob = hb.download()
+    h0 = abi.Handle(cfg)
+    h0.upload(state["x"], **args)
+    h0.upload_plastic_state(state["eps_vol"], state["prad"])
+    h0.set_objects(app["plates"], None)
+    h0.prepare()
+    h1 = scenes.make_handle(vm)
+    for i, h in enumerate((h0, h1)):
+        h.step(5)
+        a, b = split(ob, hb.scene_range(i)), h.download()
+        np.testing.assert_array_equal(a["active"], b["active"])
+        if i == 0:
+            same = (a["eps_vol"] != state["eps_vol"]) == (b["eps_vol"] != state["eps_vol"])
+            assert (~same).sum() <= 0.005 * len(same)
+        else:
+            same = np.ones(len(a["x"]), bool)
+            assert not a["eps_vol"].any() and not a["prad"].any()
+        ex = np.abs(a["x"] - b["x"])[same].max() / scenes.DX
+        ev = (np.linalg.norm(a["v"] - b["v"], axis=1) / np.maximum(np.linalg.norm(b["v"], axis=1), 0.1))[same].max()
+        eF = (np.linalg.norm((a["F"] - b["F"]).reshape(-1, 9), axis=1) /
+              np.linalg.norm(b["F"].reshape(-1, 9), axis=1))[same].max()
+        ee = max(np.abs(a["eps_vol"] - b["eps_vol"])[same].max(), np.abs(a["prad"] - b["prad"])[same].max())
+        log(f"[batch] mixed/{i}: max|dx|/dx={ex:.2e} rel dv={ev:.2e} rel dF={eF:.2e} max|d eps_vol, prad|={ee:.2e}")
+        assert ex <= 1e-5 and ev <= 1e-4 and eF <= 2e-5 and ee <= 2e-5
