@@ -10,6 +10,9 @@ reference's own Mixbox).
                          translating at 0.1 m/s through the shared face, 2000 steps
   paint_4blob_256x64x256 four 64x8x64-cell blobs, 1,048,576 particles, slab knife
                          on a 40 mm circle at 0.15 m/s, 500 steps
+  paint_6blob_512x128x512 six 102x16x102-cell blobs, 7,990,272 particles, yield
+                         100 Pa, viscosity 20 Pa s, knife sweeping linearly then on a
+                         40 mm circle, 300 steps
   batch_scenes           64 independent scenes (seeds 42 + s): particle counts
                          log-uniform in 20k-400k, 2-4 box blobs of random extents
                          and palette colours, cubic lattices of 64^3-160^3 nodes,
@@ -182,6 +185,23 @@ def paint_app_2blob(cells=(6, 4, 6), dims=(24, 16, 24), dx=DX, seed=42, stretch=
     frames = spatula_keyframes(p0, (0.1, 0.0, 0.0))
     return dict(name=f"paint_app_2blob_{dims[0]}x{dims[1]}x{dims[2]}", cfg=c, x=x, v=v, pig=pig,
                 plates=[floor_plate()], knife=kn, frames=frames, frame_fps=60.0, steps=100)
+
+
+def paint_6blob(cells=(102, 16, 102), dims=(512, 128, 512), steps=300, dx=DX, seed=42):
+    """The large scene: six 102x16x102-cell blobs (8 particles per cell with
+    jitter, 7,990,272 particles) of six palette latents in a 3 x 2 layout on
+    a 512x128x512 lattice; yield 100 Pa, viscosity 20 Pa s; the slab knife
+    sweeps linearly at 0.15 m/s across the blob faces for 1.5 ms, then turns
+    onto a 40 mm circle at the same speed."""
+    cx, cy, cz = cells
+    cols = ["cobalt_blue", "hansa_yellow", "cadmium_red", "phthalo_green", "ultramarine_blue", "cadmium_orange"]
+    specs = [((ix * cx, 0, iz * cz), cells, cols[3 * iz + ix]) for iz in range(2) for ix in range(3)]
+    x, pig = abi.scene_boxes(_boxes(specs, dx), dx, 2, seed)
+    cfg = material_config(dims, yield_pa=100.0, viscosity=20.0, dx=dx)
+    kn = slab_knife(abi.MOTION_LINEAR_THEN_CIRCULAR, (cx * dx - 0.02, 0.5 * cy * dx, cz * dx), velocity=(0.15, 0.0, 0.0),
+                    radius=0.040, speed=0.15, phase0=0.0, t_switch=1.5e-3)
+    return dict(name=f"paint_6blob_{dims[0]}x{dims[1]}x{dims[2]}", cfg=cfg, x=x, pig=pig,
+                plates=[floor_plate()], knife=kn, steps=steps)
 
 
 def batch_scene(s, seed=42, np_range=(20_000, 400_000), dx=DX):

@@ -369,6 +369,45 @@ def test_mass_budget_paint_4blob():
     np.testing.assert_allclose(g["pig"][m, :4].sum(1), 1.0, atol=1e-5)
 
 
+def test_large_scene_mass_budget():
+    """The 7,990,272-particle six-blob scene on its 512x128x512 lattice.  The
+    total node mass equals, to 1e-6, the particle mass times the sum over
+    particles of their stencil weights evaluated in numpy from the lattice's
+    own float node coordinates (the reference's arange: the partition of unity
+    is inexact far from the origin, so N m_p alone is off by ~1e-5), and it
+    stays put over 5 steps (1e-6); every particle stays active."""
+    s = scenes.paint_6blob()
+    n = s["x"].shape[0]
+    log(f"[parity] 6blob n={n}")
+    assert n == 6 * 102 * 16 * 102 * 8
+    h = scenes.make_handle(s)
+    dims, origin, xs, ys, zs = h.lattice()
+    assert list(dims) == [512, 128, 512]
+    mp = s["cfg"].rho * s["cfg"].dx ** 3 / s["cfg"].ppc
+    inv = np.float32(1.0) / np.float32(s["cfg"].dx)
+    total_w = np.zeros(n)
+    sums = []
+    for a, coords in enumerate((xs, ys, zs)):
+        xa = s["x"][:, a]
+        f = (xa - origin[a]) * inv
+        b = np.floor(f - np.float32(0.5)).astype(np.int64)
+        u = [(xa - coords[b + o]) * inv for o in range(3)]
+        t0 = np.float32(1.5) - u[0]
+        t2 = np.float32(1.5) + u[2]
+        N0 = np.where(t0 > 0, np.float32(0.5) * (t0 * t0), np.float32(0))
+        N1 = np.float32(0.75) - u[1] * u[1]
+        N2 = np.where(t2 > 0, np.float32(0.5) * (t2 * t2), np.float32(0))
+        sums.append(N0.astype(np.float64) + N1 + N2)
+    total_w = sums[0] * sums[1] * sums[2]
+    m0 = h.grid()["mass"].sum(dtype=np.float64)
+    log(f"[parity] 6blob mass: grid {m0:.9e} kg, weight sum x m_p {total_w.sum() * mp:.9e}, N m_p {n * mp:.9e}")
+    np.testing.assert_allclose(m0, total_w.sum() * mp, rtol=1e-6)
+    h.step(5)
+    st = h.stats()
+    assert st.n_active == n and st.device_error == 0
+    np.testing.assert_allclose(h.grid()["mass"].sum(dtype=np.float64), m0, rtol=1e-6)
+
+
 def test_cpp_simulation_facade_lifecycle(tmp_path):
     """The C++ Simulation facade (reference API) runs a frame on the device and
     writes it with saveParticleData."""
