@@ -55,6 +55,13 @@ MPMX_HD unsigned long long mass_to_fx(float w) {
 #endif
 }
 
+// mass_to_fx of a weight already scaled by 2^20 (t = w 2^20 in [0, 2^20]):
+// truncation is the floor for t >= 0, and fract(t) = t - floor(t) is exact.
+__device__ __forceinline__ void mass_fx_scaled(float t, unsigned& hi, unsigned& lo) {
+  hi = (unsigned)t;
+  lo = (unsigned)rintf(__builtin_amdgcn_fractf(t) * 4294967296.0f);
+}
+
 // channel indices
 enum { A_MOM = 0, A_FRC = 3, A_PIG = 6, A_SHR = 13 };
 enum { G_V = 0, G_FLIP = 3, G_PIG = 6, G_SHR = 13, G_MASS = 14 };

@@ -787,6 +787,7 @@ int mpmx_prepare(mpmx_handle* h) {
   ALLOC(d.rank_out, int, n);
   ALLOC(d.cell_out, int, n);
   ALLOC(d.perm, int, n);
+  ALLOC(d.kslot, int, n);
   ALLOC(d.cell_cnt, int, (size_t)h->nkeys * BLK3);
   ALLOC(d.cell_off, int, (size_t)h->nkeys * BLK3);
   ALLOC(d.cell_num, int, (size_t)h->nkeys * BLK3);

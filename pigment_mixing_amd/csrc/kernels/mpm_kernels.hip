@@ -111,11 +111,14 @@ __device__ __forceinline__ XcdRange xcd_range(int nblk) {
 }
 
 // Count a particle for the next sort: block count (+ rank for dead particles)
-// and per-cell rank for live ones.  All lanes call.
+// and per-cell rank for live ones.  All lanes call.  Block keys repeat across
+// a wavefront (one atomic per distinct key); cell keys mostly do not (the
+// lanes walk a rank-major block layout, one particle per cell and rank), so
+// each live lane takes its cell rank with its own atomic.
 __device__ __forceinline__ void count_for_sort(const DevPtrs& d, bool valid, int key, int cell, int64_t j) {
   const bool live = valid && cell >= 0;
   const int rb = wave_rank(valid, key, d.count_next);
-  const int rc = wave_rank(live, key * BLK3 + (cell < 0 ? 0 : cell), d.cell_cnt);
+  const int rc = live ? atomicAdd(&d.cell_cnt[key * BLK3 + cell], 1) : 0;
   if (valid) {
     d.key_out[j] = key;
     d.cell_out[j] = live ? cell : -1;
@@ -263,12 +266,24 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_down(DevPtrs d, int nkeys
   }
 }
 
-// Per block: exclusive scan of the 64 cell counts (one wavefront per block).
+// Per block (one wavefront, lane = cell): exclusive scan of the 64 cell
+// counts, and the rank-major slot of every (cell, rank) of the block.
+//
+// Inside a block the particles are laid out rank-major: first rank 0 of every
+// non-empty cell (cells in lane order), then rank 1 of every cell holding two
+// or more, and so on.  The scatter walks a block with lane = cell and step k =
+// rank, so at every step its active lanes read consecutive slots (coalesced),
+// where a cell-major layout would spread one load over ~8 particles' stride.
+// Slot of (cell c, rank r) = block start + sum_{r' < r} #{cells with count > r'}
+// + #{cells below c with count > r}.  kslot maps the cell-major position
+// (block start + cell offset + rank, what the ranks of the next sort give)
+// to it.
 __global__ void __launch_bounds__(256) k_cellscan(DevPtrs d) {
   const int nblk = d.counters[CNT_PBLOCKS];
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int nwaves = (gridDim.x * blockDim.x) >> 6;
   const int lane = threadIdx.x & 63;
+  const uint64_t below = lanemask_lt();
   for (int e = wave; e < nblk; e += nwaves) {
     const int key = d.pb_list[e];
     const int idx = key * BLK3 + lane;
@@ -279,21 +294,31 @@ __global__ void __launch_bounds__(256) k_cellscan(DevPtrs d) {
       const int t = __shfl_up(incl, off);
       if (lane >= off) incl += t;
     }
-    d.cell_off[e * BLK3 + lane] = incl - c;
+    const int coff = incl - c;
+    d.cell_off[e * BLK3 + lane] = coff;
     d.cell_num[e * BLK3 + lane] = c;
     d.cell_cnt[idx] = 0;
+    const int start = d.offsets[key];
+    int koff = 0;                          // slots of the ranks below r (wave-uniform)
+    for (int r = 0;; ++r) {
+      const uint64_t m = __ballot(c > r);
+      if (m == 0ull) break;
+      if (c > r) d.kslot[start + coff + r] = start + koff + (int)__popcll(m & below);
+      koff += (int)__popcll(m);
+    }
   }
 }
 
 // ------------------------------------------------------------------ k_perm
-// New slot of every particle: block offset + cell offset + rank in cell
-// (dead particles: their block is the DEAD key).
+// New slot of every particle: its block's rank-major slot for (cell, rank in
+// cell) (dead particles: block offset of the DEAD key + rank).
 __global__ void k_perm(DevPtrs d, int64_t n) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const int key = d.key_out[j];
   const int cell = d.cell_out[j];
-  const int dest = d.offsets[key] + (cell >= 0 ? d.cell_off[d.list_pos[key] * BLK3 + cell] : 0) + d.rank_out[j];
+  const int dest = cell >= 0 ? d.kslot[d.offsets[key] + d.cell_off[d.list_pos[key] * BLK3 + cell] + d.rank_out[j]]
+                             : d.offsets[key] + d.rank_out[j];
   d.perm[dest] = (int)j;
 }
 
@@ -983,9 +1008,9 @@ constexpr int P2G_GROUPS = 6;
 struct P2GItem {
   const float* P;      // particle side holding the sorted step's state
   const float* PB;     // MUSL: the first half's output side (indexed by sort slot)
-  int jb;              // MUSL: sort slot of this cell's first particle
   int64_t cap;
-  const int* pp;       // permutation entries of this lane's cell (in bounds)
+  const int* perm;     // permutation
+  int start;           // the block's first slot
   int cnt, kmax;       // particles of this lane's cell, deepest cell of the wave
   float X[3][3];       // node coordinates of the lane's stencil per axis
   float inv_dx, apicDinv;
@@ -1002,40 +1027,53 @@ struct P2GLoad {
   float q[NQ > 0 ? NQ : 1];
 };
 
+// Row `row` of slot i: a wave-uniform row base plus a 32-bit byte offset per
+// lane, which the compiler issues as a scalar-base global load with a 32-bit
+// vector offset (no 64-bit per-lane address arithmetic).  Slots stay below
+// 2^30, which the allocation checks.
+__device__ __forceinline__ float ld_slot(const float* __restrict__ P, int64_t cap, int row, int i) {
+  const char* base = (const char*)(P + row * cap);
+  return *(const float*)(base + (uint64_t)((uint32_t)i * 4u));
+}
+__device__ __forceinline__ int ld_slot(const int* __restrict__ A, int i) {
+  return *(const int*)((const char*)A + (uint64_t)((uint32_t)i * 4u));
+}
+
 template <int G>
 __device__ __forceinline__ void p2g_load(const float* __restrict__ P, const float* __restrict__ PB, int64_t cap,
                                          int src, int j, P2GLoad<G>& o) {
 #pragma unroll
-  for (int c = 0; c < 3; ++c) o.x[c] = P[(P_X + c) * cap + src];
+  for (int c = 0; c < 3; ++c) o.x[c] = ld_slot(P, cap, P_X + c, src);
   if constexpr (G == 6) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) o.q[c] = PB[(P_V + c) * cap + j];
+    for (int c = 0; c < 3; ++c) o.q[c] = ld_slot(PB, cap, P_V + c, j);
 #pragma unroll
-    for (int c = 0; c < 9; ++c) o.q[3 + c] = PB[(P_C + c) * cap + j];
+    for (int c = 0; c < 9; ++c) o.q[3 + c] = ld_slot(PB, cap, P_C + c, j);
   } else if constexpr (G == 1) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) o.q[c] = P[(P_V + c) * cap + src];
+    for (int c = 0; c < 3; ++c) o.q[c] = ld_slot(P, cap, P_V + c, src);
 #pragma unroll
-    for (int c = 0; c < 9; ++c) o.q[3 + c] = P[(P_C + c) * cap + src];
+    for (int c = 0; c < 9; ++c) o.q[3 + c] = ld_slot(P, cap, P_C + c, src);
   } else if constexpr (G == 2) {
 #pragma unroll
-    for (int c = 0; c < 6; ++c) o.q[c] = P[(P_TAU + c) * cap + src];
+    for (int c = 0; c < 6; ++c) o.q[c] = ld_slot(P, cap, P_TAU + c, src);
   } else if constexpr (G >= 3) {
     constexpr int q0 = G == 3 ? P_PIG : (G == 4 ? P_PIG + 3 : P_PIG + 6);
-    o.q[0] = P[q0 * cap + src];
-    o.q[1] = P[(G == 5 ? P_DIFF : q0 + 1) * cap + src];
-    o.q[2] = G == 5 ? 0.0f : P[(q0 + 2) * cap + src];
+    o.q[0] = ld_slot(P, cap, q0, src);
+    o.q[1] = ld_slot(P, cap, G == 5 ? P_DIFF : q0 + 1, src);
+    o.q[2] = G == 5 ? 0.0f : ld_slot(P, cap, q0 + 2, src);
   }
 }
 
 // Accumulate this lane's cell into acc[27][3] for channel group G.  The loop
-// runs kmax (wave-uniform) times; a lane past its own count contributes exact
-// zeros through selects, so the accumulators never sit behind a branch.  Every
-// load is in bounds for every lane (an empty cell reads its block's first slot,
-// a finished lane re-reads its last particle).  The next particle's inputs are
-// loaded while the current one is accumulated.
+// runs kmax (wave-uniform) times, step k = rank in cell; a lane past its own
+// count contributes exact zeros through selects, so the accumulators never sit
+// behind a branch.  With the block's rank-major layout the active lanes of a
+// step read consecutive slots; every load is in bounds for every lane (a lane
+// without rank k reads the block's first slot).  The next particle's inputs
+// are loaded while the current one is accumulated.
 //
-// The path is VALU-bound, so the per-node arithmetic uses the separable
+// The per-node arithmetic uses the separable
 // structure of the quadratic B-spline stencil: w = Nx Ny Nz with the 9 products
 // Ny Nz precomputed, the APIC term v + C (X_node - x_p) split into per-axis
 // parts (one add per axis per node), and the force gradients as products of
@@ -1047,15 +1085,28 @@ __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[2
   const float* __restrict__ P = it.P;
   const int64_t cap = it.cap;
   const float inv_dx = it.inv_dx;
-  const int klast = max(it.cnt - 1, 0);
+  // Slot of this lane's rank k in the block's rank-major layout: the block
+  // start + the slots of the ranks below k (kb, wave-uniform) + the cells
+  // below this one that hold a rank k.  A lane without rank k reads the
+  // block's first slot instead (in bounds; its contribution is zeroed).
+  const uint64_t below = lanemask_lt();
+  uint64_t m = __ballot(it.cnt > 0);
+  const int j0 = it.cnt > 0 ? it.start + (int)__popcll(m & below) : it.start;
+  int kb = (int)__popcll(m);
+  m = __ballot(it.cnt > 1);
+  int j_next = it.cnt > 1 ? it.start + kb + (int)__popcll(m & below) : it.start;
+  kb += (int)__popcll(m);
   P2GLoad<G> cur;
-  p2g_load<G>(P, it.PB, cap, it.pp[0], it.jb, cur);
-  int src_next = it.pp[min(1, klast)];
+  p2g_load<G>(P, it.PB, cap, ld_slot(it.perm, j0), j0, cur);
+  int src_next = ld_slot(it.perm, j_next);
   for (int k = 0; k < it.kmax; ++k) {
     const bool act = k < it.cnt;
     P2GLoad<G> nxt;
-    p2g_load<G>(P, it.PB, cap, src_next, it.jb + min(k + 1, klast), nxt);   // next particle (clamped)
-    src_next = it.pp[min(k + 2, klast)];
+    p2g_load<G>(P, it.PB, cap, src_next, j_next, nxt);   // rank k + 1
+    m = __ballot(it.cnt > k + 2);
+    j_next = it.cnt > k + 2 ? it.start + kb + (int)__popcll(m & below) : it.start;
+    kb += (int)__popcll(m);
+    src_next = ld_slot(it.perm, j_next);                 // rank k + 2
     const float x0 = act ? cur.x[0] : it.X[0][1];
     const float x1 = act ? cur.x[1] : it.X[1][1];
     const float x2 = act ? cur.x[2] : it.X[2][1];
@@ -1075,14 +1126,24 @@ __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[2
 #pragma unroll
     for (int o = 0; o < 3; ++o) N[o] = act ? N[o] : 0.0f;
     if constexpr (G == 0) {
+      // t = w 2^20 with the power of two folded into the x weights: exact for
+      // every weight whose fixed-point value is nonzero (an intermediate
+      // product below the normal range means w < 2^-126, which rounds to 0
+      // either way).  The reference's cut w > 1e-25 needs no test here: such
+      // weights give hi = 0 and lo = rn(< 2^-40) = 0.
+      float Nx20[3];
+#pragma unroll
+      for (int o = 0; o < 3; ++o) Nx20[o] = N[o] * 1048576.0f;
 #pragma unroll
       for (int n = 0; n < 27; ++n) {
-        const float wgt = (N[n / 9] * N[3 + (n / 3) % 3]) * N[6 + n % 3];
-        unsigned long long m = ((unsigned long long)__float_as_uint(acc[n][1]) << 32) |
-                               (unsigned long long)__float_as_uint(acc[n][0]);
-        m += mass_to_fx(wgt > 1e-25f ? wgt : 0.0f);   // branch-free: mass_to_fx(0) = 0
-        acc[n][0] = __uint_as_float((unsigned)m);
-        acc[n][1] = __uint_as_float((unsigned)(m >> 32));
+        const float t = (Nx20[n / 9] * N[3 + (n / 3) % 3]) * N[6 + n % 3];
+        // 64-bit add as add with carry: low word, then high word + carry
+        unsigned hi, lo;
+        mass_fx_scaled(t, hi, lo);
+        const unsigned lo_acc = __float_as_uint(acc[n][0]) + lo;
+        const unsigned hi_acc = __float_as_uint(acc[n][1]) + hi + (lo_acc < lo ? 1u : 0u);
+        acc[n][0] = __uint_as_float(lo_acc);
+        acc[n][1] = __uint_as_float(hi_acc);
       }
     } else {
       float NyNz[9];
@@ -1140,23 +1201,21 @@ __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[2
 #pragma unroll
         for (int c = 0; c < 6; ++c) tau[c] = act ? cur.q[c] : 0.0f;
         const float T[3][3] = {{tau[0], tau[3], tau[4]}, {tau[3], tau[1], tau[5]}, {tau[4], tau[5], tau[2]}};
-        float dyz[9], ydz[9];                   // dNy Nz, Ny dNz per (oj, ok)
+        // per (oj, ok): f_r = dNx (tau_r0 Ny Nz) + Nx (tau_r1 dNy Nz + tau_r2 Ny dNz),
+        // two FMAs per node and component
 #pragma unroll
         for (int p = 0; p < 9; ++p) {
-          dyz[p] = dN[3 + p / 3] * N[6 + p % 3];
-          ydz[p] = N[3 + p / 3] * dN[6 + p % 3];
-        }
-        float tx[3][3];                         // [oi][r]: tau_r0 dNx
+          const float b = dN[3 + p / 3] * N[6 + p % 3], c = N[3 + p / 3] * dN[6 + p % 3];
+          float Ar[3], Br[3];
 #pragma unroll
-        for (int o = 0; o < 3; ++o)
+          for (int r = 0; r < 3; ++r) {
+            Ar[r] = T[r][0] * NyNz[p];
+            Br[r] = fmaf(T[r][2], c, T[r][1] * b);
+          }
 #pragma unroll
-          for (int r = 0; r < 3; ++r) tx[o][r] = T[r][0] * dN[o];
+          for (int oi = 0; oi < 3; ++oi)
 #pragma unroll
-        for (int n = 0; n < 27; ++n) {
-          const int oi = n / 9, p = n % 9;
-          const float a = NyNz[p], b = N[oi] * dyz[p], c = N[oi] * ydz[p];
-#pragma unroll
-          for (int r = 0; r < 3; ++r) acc[n][r] = fmaf(T[r][2], c, fmaf(T[r][1], b, fmaf(tx[oi][r], a, acc[n][r])));
+            for (int r = 0; r < 3; ++r) acc[9 * oi + p][r] = fmaf(N[oi], Br[r], fmaf(dN[oi], Ar[r], acc[9 * oi + p][r]));
         }
       } else {
         // latent / shear: three (group 5: two) weighted scalars, q * Nx folded per offset
@@ -1231,13 +1290,19 @@ __device__ __forceinline__ bool slot_node(int xs, int ys, int zs, int ci, int cj
   return (xs == 0 || ci == 3) && (ys == 0 || cj == 3) && (zs == 0 || ck == 3);
 }
 
-// Merged tile of block e: listed blocks below tile_cap store it (every owned
-// slot, zeros included) for the grid kernel to gather; later blocks add it to
-// the grid accumulators with atomics.
+// Merged tile of block e: listed blocks below tile_cap store it (all 216
+// nodes, zeros included) for the grid kernel to gather; later blocks add it to
+// the grid accumulators with atomics.  A tile leaves through the wavefront's
+// LDS staging area: every lane writes its 27 merged slots to stage[n*64 + lane]
+// (no lane masks; a slot the lane does not own is never read), then each lane
+// reads tile nodes lane, lane + 64, ... from their owners' words (rd[], fixed
+// per lane) and the 216 nodes go out with four full-wavefront stores.
 template <int G>
 __device__ __forceinline__ void p2g_group(const P2GItem& it, const DevPtrs& d, const SceneDev& sc, int e, int B0i,
-                                          int B0j, int B0k, int ci, int cj, int ck, unsigned& touched) {
+                                          int B0j, int B0k, int ci, int cj, int ck, unsigned& touched,
+                                          unsigned long long* stage, const int (&rd)[4]) {
   const bool tiled = e < d.tile_cap;
+  const int lane = threadIdx.x;
   float acc[27][3];
 #pragma unroll
   for (int n = 0; n < 27; ++n) { acc[n][0] = acc[n][1] = acc[n][2] = 0.0f; }
@@ -1248,11 +1313,14 @@ __device__ __forceinline__ void p2g_group(const P2GItem& it, const DevPtrs& d, c
     for (int n = 0; n < 27; ++n)
       m[n] = ((unsigned long long)__float_as_uint(acc[n][1]) << 32) | (unsigned long long)__float_as_uint(acc[n][0]);
     p2g_merge(m, ci, cj, ck);
+    if (tiled) {
+#pragma unroll
+      for (int n = 0; n < 27; ++n) stage[n * 64 + lane] = m[n];
+    }
 #pragma unroll
     for (int n = 0; n < 27; ++n) {
       int tx, ty, tz;
       const bool own = slot_node(n / 9, (n / 3) % 3, n % 3, ci, cj, ck, tx, ty, tz);
-      if (own && tiled) d.tile_mass[(size_t)e * PT3 + (tx * PT + ty) * PT + tz] = m[n];
       if (own && m[n] != 0ull) {
         if (!tiled) {
           const int gi = B0i + tx, gj = B0j + ty, gk = B0k + tz;
@@ -1261,25 +1329,42 @@ __device__ __forceinline__ void p2g_group(const P2GItem& it, const DevPtrs& d, c
         touched |= 1u << ((((tx >> 2) * 2) + (ty >> 2)) * 2 + (tz >> 2));
       }
     }
+    if (tiled) {
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (lane + 64 * r < PT3) d.tile_mass[(size_t)e * PT3 + lane + 64 * r] = stage[rd[r]];
+      __syncthreads();
+    }
   } else {
     // group 5's channels are latent 6 and shear (A_PIG + 6, A_SHR = A_PIG + 7)
     constexpr int ch0 = (G == 1 || G == 6) ? A_MOM : (G == 2 ? A_FRC : (G == 3 ? A_PIG : (G == 4 ? A_PIG + 3 : A_PIG + 6)));
     constexpr int nch = G == 5 ? 2 : 3;
+    float* stage_f = reinterpret_cast<float*>(stage);
 #pragma unroll
     for (int c = 0; c < nch; ++c) {
       float v[27];
 #pragma unroll
       for (int n = 0; n < 27; ++n) v[n] = acc[n][c];
       p2g_merge(v, ci, cj, ck);
+      if (tiled) {
 #pragma unroll
-      for (int n = 0; n < 27; ++n) {
-        int tx, ty, tz;
-        const bool own = slot_node(n / 9, (n / 3) % 3, n % 3, ci, cj, ck, tx, ty, tz);
-        if (own && tiled) d.tile_acc[((size_t)e * ACC_CH + ch0 + c) * PT3 + (tx * PT + ty) * PT + tz] = v[n];
-        if (own && !tiled && v[n] != 0.0f) {
-          const int gi = B0i + tx, gj = B0j + ty, gk = B0k + tz;
-          atomicAdd(d.acc + ((size_t)node_block_of(sc, gi, gj, gk) * ACC_CH + ch0 + c) * BLK3 + node_local(gi, gj, gk),
-                    v[n]);
+        for (int n = 0; n < 27; ++n) stage_f[n * 64 + lane] = v[n];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (lane + 64 * r < PT3) d.tile_acc[((size_t)e * ACC_CH + ch0 + c) * PT3 + lane + 64 * r] = stage_f[rd[r]];
+        __syncthreads();
+      } else {
+#pragma unroll
+        for (int n = 0; n < 27; ++n) {
+          int tx, ty, tz;
+          const bool own = slot_node(n / 9, (n / 3) % 3, n % 3, ci, cj, ck, tx, ty, tz);
+          if (own && v[n] != 0.0f) {
+            const int gi = B0i + tx, gj = B0j + ty, gk = B0k + tz;
+            atomicAdd(d.acc + ((size_t)node_block_of(sc, gi, gj, gk) * ACC_CH + ch0 + c) * BLK3 + node_local(gi, gj, gk),
+                      v[n]);
+          }
         }
       }
     }
@@ -1288,16 +1373,28 @@ __device__ __forceinline__ void p2g_group(const P2GItem& it, const DevPtrs& d, c
 
 // MUSL: only the momentum of group 6 (MUSL's second scatter), no activation.
 template <bool MUSL>
-__global__ void __launch_bounds__(64) k_p2g(DevPtrs d, const SceneDev* __restrict__ scenes,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_p2g(DevPtrs d, const SceneDev* __restrict__ scenes,
                                             const float* __restrict__ axis, const float* __restrict__ P,
                                             const int* __restrict__ perm, const int4* __restrict__ pdesc,
-                                            const int* __restrict__ cell_off, const int* __restrict__ cell_num,
-                                            const float* __restrict__ PB) {
+                                            const int* __restrict__ cell_num, const float* __restrict__ PB) {
   constexpr int NG = MUSL ? 1 : P2G_GROUPS;
+  // tile staging area of this workgroup's one wavefront: 27 slots x 64 lanes
+  // (8-byte words for the mass group, 4-byte for the others)
+  __shared__ unsigned long long stage[MUSL ? 27 * 32 : 27 * 64];
   const int nblk = d.counters[CNT_PBLOCKS];
   const int parity = d.counters[CNT_GSTEP] & 1;
   const int lane = threadIdx.x;
   const int ci = lane >> 4, cj = (lane >> 2) & 3, ck = lane & 3;
+  // staging word of the owner of tile node q = lane + 64 r: the cell
+  // min(t, 3) per axis holds node t in merged slot t - min(t, 3)
+  int rd[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int q = min(lane + 64 * r, PT3 - 1);
+    const int tx = q / (PT * PT), ty = (q / PT) % PT, tz = q % PT;
+    const int oi = min(tx, 3), oj = min(ty, 3), ok = min(tz, 3);
+    rd[r] = ((tx - oi) * 9 + (ty - oj) * 3 + (tz - ok)) * 64 + oi * 16 + oj * 4 + ok;
+  }
   // items of this XCD: its contiguous segment of the block list x 6 groups
   const int nx = 8;
   const int xcd = blockIdx.x % nx;
@@ -1314,7 +1411,6 @@ __global__ void __launch_bounds__(64) k_p2g(DevPtrs d, const SceneDev* __restric
     const SceneDev& sc = scenes[ds.w];
     P2GItem it;
     it.cnt = cell_num[e * BLK3 + lane];
-    const int coff = cell_off[e * BLK3 + lane];
     int bi, bj, bk;
     block_coords(sc, key, bi, bj, bk);
     const int B0i = bi * 4, B0j = bj * 4, B0k = bk * 4;
@@ -1322,9 +1418,9 @@ __global__ void __launch_bounds__(64) k_p2g(DevPtrs d, const SceneDev* __restric
     it.cap = d.cap;
     it.inv_dx = sc.one_over_dx;
     it.apicDinv = sc.apicDinv;
-    it.pp = perm + ds.y + (it.cnt > 0 ? coff : 0);
+    it.perm = perm;
+    it.start = ds.y;
     it.PB = PB;
-    it.jb = ds.y + (it.cnt > 0 ? coff : 0);
     int kmax = it.cnt;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) kmax = max(kmax, __shfl_xor(kmax, off));
@@ -1338,14 +1434,17 @@ __global__ void __launch_bounds__(64) k_p2g(DevPtrs d, const SceneDev* __restric
       it.X[2][o] = xs[sc.dims[0] + sc.dims[1] + min(B0k + ck + o, sc.dims[2] - 1)];
     }
     unsigned touched = 0u;
-    switch (group) {
-      case 0: p2g_group<0>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched); break;
-      case 1: p2g_group<1>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched); break;
-      case 2: p2g_group<2>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched); break;
-      case 3: p2g_group<3>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched); break;
-      case 4: p2g_group<4>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched); break;
-      case 5: p2g_group<5>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched); break;
-      default: p2g_group<6>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched); break;
+    if constexpr (MUSL) {
+      p2g_group<6>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched, stage, rd);
+    } else {
+      switch (group) {
+        case 0: p2g_group<0>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched, stage, rd); break;
+        case 1: p2g_group<1>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched, stage, rd); break;
+        case 2: p2g_group<2>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched, stage, rd); break;
+        case 3: p2g_group<3>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched, stage, rd); break;
+        case 4: p2g_group<4>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched, stage, rd); break;
+        default: p2g_group<5>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched, stage, rd); break;
+      }
     }
     if (group == 0) {
       // node blocks receiving mass from this block: OR of the lanes' bits
@@ -1626,12 +1725,12 @@ void launch_closure_eval(const SceneDev* scenes, const void* mlp_blob, const flo
 void launch_p2g(const DevPtrs& d, int grid, int src_is_b, hipStream_t st) {
   hipLaunchKernelGGL(k_p2g<false>, dim3(grid), dim3(64), 0, st, d, (const SceneDev*)d.scenes,
                      (const float*)d.axis_coords, (const float*)(src_is_b ? d.pb : d.pa), (const int*)d.perm,
-                     (const int4*)d.pdesc, (const int*)d.cell_off, (const int*)d.cell_num, (const float*)nullptr);
+                     (const int4*)d.pdesc, (const int*)d.cell_num, (const float*)nullptr);
 }
 void launch_musl(const DevPtrs& d, int grid_p2g, int grid_grid, hipStream_t st) {
   hipLaunchKernelGGL(k_p2g<true>, dim3(grid_p2g), dim3(64), 0, st, d, (const SceneDev*)d.scenes,
                      (const float*)d.axis_coords, (const float*)d.pa, (const int*)d.perm, (const int4*)d.pdesc,
-                     (const int*)d.cell_off, (const int*)d.cell_num, (const float*)d.pb);
+                     (const int*)d.cell_num, (const float*)d.pb);
   hipLaunchKernelGGL(k_grid_musl, dim3(grid_grid), dim3(256), 0, st, d);
 }
 void launch_grid(const DevPtrs& d, int grid, hipStream_t st) {

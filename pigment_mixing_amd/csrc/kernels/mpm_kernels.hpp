@@ -51,6 +51,7 @@ struct DevPtrs {
   int* rank_out;
   int* cell_out;           // cell (0..63) of the particle in its block, -1 if dead
   int* perm;
+  int* kslot;              // cell-major position (block start + cell offset + rank) -> rank-major slot
   int* cell_cnt;           // per (block, cell) particle counts of the next sort
   int* cell_off;           // per (listed block, cell) offsets inside the block, list order
   int* cell_num;           // per (listed block, cell) counts of the current sort, list order
