@@ -1385,13 +1385,14 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
   const int parity = d.counters[CNT_GSTEP] & 1;
   const int lane = threadIdx.x;
   const int ci = lane >> 4, cj = (lane >> 2) & 3, ck = lane & 3;
-  // staging word of the owner of tile node q = lane + 64 r: the cell
-  // min(t, 3) per axis holds node t in merged slot t - min(t, 3)
+  // staging word of the owner of the tile node stored at position lane + 64 r
+  // (tile_pos order): the cell min(t, 3) per axis holds node t in merged slot
+  // t - min(t, 3)
   int rd[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int q = min(lane + 64 * r, PT3 - 1);
-    const int tx = q / (PT * PT), ty = (q / PT) % PT, tz = q % PT;
+    int tx, ty, tz;
+    tile_node_at(min(lane + 64 * r, PT3 - 1), tx, ty, tz);
     const int oi = min(tx, 3), oj = min(ty, 3), ok = min(tz, 3);
     rd[r] = ((tx - oi) * 9 + (ty - oj) * 3 + (tz - ok)) * 64 + oi * 16 + oj * 4 + ok;
   }
@@ -1502,7 +1503,7 @@ __global__ void __launch_bounds__(256) k_grid(DevPtrs d) {
       if (e >= d.tile_cap) continue;
       const int tx = li + 4 * ox, ty = lj + 4 * oy, tz = lk + 4 * oz;
       if (tx >= PT || ty >= PT || tz >= PT) continue;
-      const int t = (tx * PT + ty) * PT + tz;
+      const int t = tile_pos(tx, ty, tz);   // this block's group: one contiguous run
       mfx += d.tile_mass[(size_t)e * PT3 + t];
 #pragma unroll
       for (int c = 0; c < ACC_CH; ++c) a[c] += d.tile_acc[((size_t)e * ACC_CH + c) * PT3 + t];
@@ -1590,7 +1591,7 @@ __global__ void __launch_bounds__(256) k_grid_musl(DevPtrs d) {
       if (te >= d.tile_cap) continue;
       const int tx = li + 4 * ox, ty = lj + 4 * oy, tz = lk + 4 * oz;
       if (tx >= PT || ty >= PT || tz >= PT) continue;
-      const int t = (tx * PT + ty) * PT + tz;
+      const int t = tile_pos(tx, ty, tz);
 #pragma unroll
       for (int c = 0; c < 3; ++c) mom[c] += d.tile_acc[((size_t)te * ACC_CH + A_MOM + c) * PT3 + t];
     }

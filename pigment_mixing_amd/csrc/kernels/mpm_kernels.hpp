@@ -95,6 +95,36 @@ constexpr int CQ_CNT_STRIDE = 32;      // ints: one 128-byte line per segment co
 constexpr int PT = 6;                  // scatter tile edge (nodes) = 4 bins + 2
 constexpr int PT3 = PT * PT * PT;      // 216
 
+// Stored order of a scatter tile: node (tx, ty, tz) of the 6x6x6 tile belongs
+// to the node block at offset (ox, oy, oz) = (tx, ty, tz) >> 2 from the tile's
+// particle block.  The 8 groups are stored one after another (offsets 000,
+// 001, ..., 111), each its 4- or 2-node-per-axis sub-box in row-major order, so
+// the grid kernel reads the group of every node block as one contiguous run.
+MPMX_HD int tile_group_base(int ox, int oy, int oz) {
+  const int ex = 4 - 2 * ox, ey = 4 - 2 * oy;
+  return 144 * ox + 24 * ex * oy + 4 * ex * ey * oz;
+}
+MPMX_HD int tile_pos(int tx, int ty, int tz) {
+  const int ox = tx >> 2, oy = ty >> 2, oz = tz >> 2;
+  const int ey = 4 - 2 * oy, ez = 4 - 2 * oz;
+  return tile_group_base(ox, oy, oz) + ((tx - 4 * ox) * ey + (ty - 4 * oy)) * ez + (tz - 4 * oz);
+}
+// Inverse of tile_pos for a stored position p in [0, 216).
+MPMX_HD void tile_node_at(int p, int& tx, int& ty, int& tz) {
+  tx = ty = tz = 0;
+  for (int q = 0; q < 8; ++q) {
+    const int ox = q >> 2, oy = (q >> 1) & 1, oz = q & 1;
+    const int ex = 4 - 2 * ox, ey = 4 - 2 * oy, ez = 4 - 2 * oz;
+    const int base = tile_group_base(ox, oy, oz);
+    if (p >= base && p < base + ex * ey * ez) {
+      const int l = p - base;
+      tx = 4 * ox + l / (ey * ez);
+      ty = 4 * oy + (l / ez) % ey;
+      tz = 4 * oz + l % ez;
+    }
+  }
+}
+
 void launch_init_tau(const DevPtrs& d, int64_t n, hipStream_t st);
 void launch_keys(const DevPtrs& d, int64_t n, hipStream_t st);
 void launch_scan(const DevPtrs& d, int nkeys, int advance, hipStream_t st);
