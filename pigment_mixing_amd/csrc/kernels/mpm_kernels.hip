@@ -9,15 +9,16 @@
 //             stress of the new F for the next scatter) in sorted order, and counts
 //             the particles' new blocks and cells for the next sort.
 //   k_scan_*  prefix sums over block counts -> block offsets + block list.
-//   k_cellscan per block: offsets of the 64 cells inside the block.
-//   k_perm    slot <- particle permutation: block-major, cell-major inside a block.
+//   k_cellscan per block: offsets of the 64 cells inside the block, and the
+//             rank-major slot of every (cell, rank).
+//   k_perm    slot <- particle permutation: block-major, rank-major inside a block.
 //   k_p2g     lane = cell, wave = (block, channel group): each lane walks its
 //             cell's particles and accumulates their 27-node contributions (mass,
 //             APIC momentum, Kirchhoff force, pigment latent, shear intensity;
 //             p2g.cpp:6-126, explicit_euler_update.cpp:12-55) in registers; the
 //             lanes' stencils merge across lanes (DPP / shuffles, one axis at a
-//             time) into the block's 6x6x6 node tile with one owner per node, which
-//             adds it to the grid accumulators with a global atomic.
+//             time) into the block's 6x6x6 node tile with one owner per node,
+//             stored for the grid kernel (grouped by destination node block).
 //   k_grid    one wavefront per active 4x4x4 node block: normalise, gravity + force,
 //             boundary collision with plates and knife (explicit_euler_update.cpp:64-110,
 //             boundary_collision.cpp:5-316), write the gather grid, clear accumulators.
@@ -992,17 +993,22 @@ __global__ void k_copy_dead(DevPtrs d) {
 // ------------------------------------------------------------------ k_p2g
 // Work item = (particle block of the new sort, channel group); one wavefront
 // per item, lane = cell of the block (4x4x4 bins):
-//   0: mass (u64 fixed point)   1: APIC momentum     2: Kirchhoff force
-//   3: latent 0..2              4: latent 3..5       5: latent 6 + shear intensity
-// Each lane walks its cell's particles (cell-sorted, so the stencil base is the
-// cell for all of them) and accumulates the 27 stencil nodes x 3 channels in
-// registers.  The 64 lanes' stencils then merge into the block's 6x6x6 node tile
-// entirely in registers, one axis at a time (see p2g_merge), so that every tile
-// node ends with exactly one owning lane, which adds it to the grid
-// accumulators with one global atomic per channel.  No LDS is used.  Items are
-// independent, so there is no workgroup-wide phase: many small waves per CU
-// overlap their particle loads with other waves' arithmetic.
-constexpr int P2G_GROUPS = 6;
+//   0: mass (u64 fixed point) + latent 6 + shear intensity
+//   1: APIC momentum    2: Kirchhoff force    3: latent 0..5
+// Each lane walks its cell's particles (block-sorted, so the stencil base is
+// the cell for all of them) and accumulates the 27 stencil nodes x the group's
+// channels in registers.  The 64 lanes' stencils then merge into the block's
+// 6x6x6 node tile in registers, one axis at a time (see p2g_merge), so that
+// every tile node ends with exactly one owning lane; the tile leaves through a
+// small LDS staging area (see p2g_group).  Items are independent, so there is
+// no workgroup-wide phase: many small waves per CU overlap their particle
+// loads with other waves' arithmetic.  Fewer, wider groups evaluate the spline
+// weights and walk the particles fewer times (six 3-channel groups: 146 us,
+// five: 137 us, four: 133 us on the 1M-particle scene).
+constexpr int P2G_GROUPS = 4;
+// Accumulator channels of group G: mass (u64 as two words) + latent 6 + shear |
+// momentum 3 | force 3 | latent 0..5 | (MUSL) momentum 3.
+__host__ __device__ constexpr int p2g_nch(int G) { return G == 0 ? 4 : (G == 3 ? 6 : 3); }
 
 // Per-item inputs shared by the channel-group bodies.
 struct P2GItem {
@@ -1022,7 +1028,7 @@ struct P2GItem {
 // half's output (slot j of the sort).
 template <int G>
 struct P2GLoad {
-  static constexpr int NQ = G == 0 ? 0 : (G == 1 || G == 6 ? 12 : (G == 2 ? 6 : 3));
+  static constexpr int NQ = G == 0 ? 2 : (G == 1 || G == 6 ? 12 : 6);
   float x[3];
   float q[NQ > 0 ? NQ : 1];
 };
@@ -1057,11 +1063,12 @@ __device__ __forceinline__ void p2g_load(const float* __restrict__ P, const floa
   } else if constexpr (G == 2) {
 #pragma unroll
     for (int c = 0; c < 6; ++c) o.q[c] = ld_slot(P, cap, P_TAU + c, src);
-  } else if constexpr (G >= 3) {
-    constexpr int q0 = G == 3 ? P_PIG : (G == 4 ? P_PIG + 3 : P_PIG + 6);
-    o.q[0] = ld_slot(P, cap, q0, src);
-    o.q[1] = ld_slot(P, cap, G == 5 ? P_DIFF : q0 + 1, src);
-    o.q[2] = G == 5 ? 0.0f : ld_slot(P, cap, q0 + 2, src);
+  } else if constexpr (G == 3) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) o.q[c] = ld_slot(P, cap, P_PIG + c, src);
+  } else if constexpr (G == 0) {
+    o.q[0] = ld_slot(P, cap, P_PIG + 6, src);
+    o.q[1] = ld_slot(P, cap, P_DIFF, src);
   }
 }
 
@@ -1081,7 +1088,7 @@ __device__ __forceinline__ void p2g_load(const float* __restrict__ P, const floa
 // by O(ulp); the mass channel, which decides node activation, keeps the
 // reference's product order and its tiny-weight cut.)
 template <int G>
-__device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[27][3]) {
+__device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[27][p2g_nch(G)]) {
   const float* __restrict__ P = it.P;
   const int64_t cap = it.cap;
   const float inv_dx = it.inv_dx;
@@ -1144,6 +1151,20 @@ __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[2
         const unsigned hi_acc = __float_as_uint(acc[n][1]) + hi + (lo_acc < lo ? 1u : 0u);
         acc[n][0] = __uint_as_float(lo_acc);
         acc[n][1] = __uint_as_float(hi_acc);
+      }
+      // latent 6 and shear intensity: weighted scalars, q * Nx folded per offset
+      float fq[2][3];
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const float q = act ? cur.q[c] : 0.0f;
+#pragma unroll
+        for (int o = 0; o < 3; ++o) fq[c][o] = q * N[o];
+      }
+#pragma unroll
+      for (int n = 0; n < 27; ++n) {
+        const float nyz = N[3 + (n / 3) % 3] * N[6 + n % 3];
+        acc[n][2] += fq[0][n / 9] * nyz;
+        acc[n][3] += fq[1][n / 9] * nyz;
       }
     } else {
       float NyNz[9];
@@ -1218,17 +1239,20 @@ __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[2
             for (int r = 0; r < 3; ++r) acc[9 * oi + p][r] = fmaf(N[oi], Br[r], fmaf(dN[oi], Ar[r], acc[9 * oi + p][r]));
         }
       } else {
-        // latent / shear: three (group 5: two) weighted scalars, q * Nx folded per offset
-        float qa = act ? cur.q[0] : 0.0f, qb = act ? cur.q[1] : 0.0f, qc = (act && G != 5) ? cur.q[2] : 0.0f;
-        float fa[3], fb[3], fc[3];
+        // latent / shear: weighted scalars, q * Nx folded per offset
+        constexpr int NC = p2g_nch(G);
+        float fq[NC][3];
 #pragma unroll
-        for (int o = 0; o < 3; ++o) { fa[o] = qa * N[o]; fb[o] = qb * N[o]; fc[o] = qc * N[o]; }
+        for (int c = 0; c < NC; ++c) {
+          const float q = act ? cur.q[c] : 0.0f;
+#pragma unroll
+          for (int o = 0; o < 3; ++o) fq[c][o] = q * N[o];
+        }
 #pragma unroll
         for (int n = 0; n < 27; ++n) {
           const int oi = n / 9, p = n % 9;
-          acc[n][0] += fa[oi] * NyNz[p];
-          acc[n][1] += fb[oi] * NyNz[p];
-          if constexpr (G != 5) acc[n][2] += fc[oi] * NyNz[p];
+#pragma unroll
+          for (int c = 0; c < NC; ++c) acc[n][c] += fq[c][oi] * NyNz[p];
         }
       }
     }
@@ -1303,9 +1327,11 @@ __device__ __forceinline__ void p2g_group(const P2GItem& it, const DevPtrs& d, c
                                           unsigned long long* stage, const int (&rd)[4]) {
   const bool tiled = e < d.tile_cap;
   const int lane = threadIdx.x;
-  float acc[27][3];
+  float acc[27][p2g_nch(G)];
 #pragma unroll
-  for (int n = 0; n < 27; ++n) { acc[n][0] = acc[n][1] = acc[n][2] = 0.0f; }
+  for (int n = 0; n < 27; ++n)
+#pragma unroll
+    for (int c = 0; c < p2g_nch(G); ++c) acc[n][c] = 0.0f;
   p2g_accumulate<G>(it, acc);
   if constexpr (G == 0) {
     unsigned long long m[27];
@@ -1336,13 +1362,16 @@ __device__ __forceinline__ void p2g_group(const P2GItem& it, const DevPtrs& d, c
         if (lane + 64 * r < PT3) d.tile_mass[(size_t)e * PT3 + lane + 64 * r] = stage[rd[r]];
       __syncthreads();
     }
-  } else {
-    // group 5's channels are latent 6 and shear (A_PIG + 6, A_SHR = A_PIG + 7)
-    constexpr int ch0 = (G == 1 || G == 6) ? A_MOM : (G == 2 ? A_FRC : (G == 3 ? A_PIG : (G == 4 ? A_PIG + 3 : A_PIG + 6)));
-    constexpr int nch = G == 5 ? 2 : 3;
+  }
+  {
+    // float channels: acc channel c goes to grid channel ch0 + c (group 0's
+    // channels 2, 3 are latent 6 and shear: A_PIG + 6, A_SHR = A_PIG + 7)
+    constexpr int CF = G == 0 ? 2 : 0;
+    constexpr int ch0 = (G == 1 || G == 6) ? A_MOM : (G == 2 ? A_FRC : (G == 3 ? A_PIG : A_PIG + 6 - CF));
+    constexpr int nch = p2g_nch(G);
     float* stage_f = reinterpret_cast<float*>(stage);
 #pragma unroll
-    for (int c = 0; c < nch; ++c) {
+    for (int c = CF; c < nch; ++c) {
       float v[27];
 #pragma unroll
       for (int n = 0; n < 27; ++n) v[n] = acc[n][c];
@@ -1442,9 +1471,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
         case 0: p2g_group<0>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched, stage, rd); break;
         case 1: p2g_group<1>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched, stage, rd); break;
         case 2: p2g_group<2>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched, stage, rd); break;
-        case 3: p2g_group<3>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched, stage, rd); break;
-        case 4: p2g_group<4>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched, stage, rd); break;
-        default: p2g_group<5>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched, stage, rd); break;
+        default: p2g_group<3>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched, stage, rd); break;
       }
     }
     if (group == 0) {
