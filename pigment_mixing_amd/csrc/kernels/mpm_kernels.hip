@@ -323,6 +323,25 @@ __global__ void k_perm(DevPtrs d, int64_t n) {
   d.perm[dest] = (int)j;
 }
 
+// Dead particles keep their state: copy them to the output side and rank
+// them.  Run by G2P's workgroups after their blocks (the dead slots are
+// disjoint from the live ones), a share per workgroup: no launch of its own.
+__device__ __forceinline__ void copy_dead_share(const DevPtrs& d) {
+  const int dead_start = d.offsets[d.dead_key];
+  const int ndead = d.count_cur[d.dead_key];
+  for (int t0 = blockIdx.x * blockDim.x; t0 < ndead; t0 += gridDim.x * blockDim.x) {
+    const int t = t0 + threadIdx.x;
+    const bool valid = t < ndead;
+    const int64_t j = dead_start + (valid ? t : 0);
+    if (valid) {
+      const int src = d.perm[j];
+      for (int c = 0; c < P_NCOMP; ++c) d.pb[c * d.cap + j] = d.pa[c * d.cap + src];
+      d.pb_id[j] = d.pa_id[src];
+    }
+    count_for_sort(d, valid, d.dead_key, -1, j);
+  }
+}
+
 // ------------------------------------------------------------------ k_g2p
 constexpr int GT = 6;              // gather tile edge: stencils of bins [B0, B0+4) span nodes [B0, B0+6)
 constexpr int GT3 = GT * GT * GT;  // 216 nodes
@@ -831,6 +850,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
       if (newton) atomicAdd(&newton_wg, (unsigned)newton);
     }
   }
+  if constexpr (PHASE != 1) copy_dead_share(d);
   __syncthreads();
   if (threadIdx.x == 0) {
     if (plastic_wg) atomicAdd(&d.stats[ST_PLASTIC_CUR], (unsigned long long)plastic_wg);
@@ -970,23 +990,6 @@ __global__ void __launch_bounds__(256) k_closure_eval(const SceneDev* __restrict
       exact4[4 * i + 3] = pl ? dge * sc.dt : 0.0f;
       accepted[i] = ok ? 1 : 0;
     }
-  }
-}
-
-// Dead particles keep their state: copy them to the output side and rank them.
-__global__ void k_copy_dead(DevPtrs d) {
-  const int dead_start = d.offsets[d.dead_key];
-  const int ndead = d.count_cur[d.dead_key];
-  for (int t0 = blockIdx.x * blockDim.x; t0 < ndead; t0 += gridDim.x * blockDim.x) {
-    const int t = t0 + threadIdx.x;
-    const bool valid = t < ndead;
-    const int64_t j = dead_start + (valid ? t : 0);
-    if (valid) {
-      const int src = d.perm[j];
-      for (int c = 0; c < P_NCOMP; ++c) d.pb[c * d.cap + j] = d.pa[c * d.cap + src];
-      d.pb_id[j] = d.pa_id[src];
-    }
-    count_for_sort(d, valid, d.dead_key, -1, j);
   }
 }
 
@@ -1738,7 +1741,6 @@ void launch_g2p(const DevPtrs& d, int grid, int learned, int phase, hipStream_t 
     hipLaunchKernelGGL((k_g2p<false, 0, true>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, pd);
   else
     hipLaunchKernelGGL((k_g2p<false, 0, false>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, pd);
-  if (phase != 1) hipLaunchKernelGGL(k_copy_dead, dim3(64), dim3(256), 0, st, d);
 }
 void launch_closure(const DevPtrs& d, const void* mlp_blob, int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_closure, dim3(grid), dim3(256), 0, st, d, d.scenes, (const uint8_t*)mlp_blob);
