@@ -1523,13 +1523,21 @@ __global__ void __launch_bounds__(256) k_grid(DevPtrs d) {
     float a[ACC_CH];
 #pragma unroll
     for (int c = 0; c < ACC_CH; ++c) a[c] = scattered ? acc[c * BLK3 + lane] : 0.0f;
+    // the 8 source tiles' list positions, resolved up front (independent
+    // loads) so that the tile reads below do not wait on each other's lookups
+    int te[8];
+#pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int ox = q >> 2, oy = (q >> 1) & 1, oz = q & 1;
       const int pi = bi - ox, pj = bj - oy, pk = bk - oz;
-      if (pi < 0 || pj < 0 || pk < 0) continue;
       const int key = sc.node_block_base + (pi * sc.bdims[1] + pj) * sc.bdims[2] + pk;
-      if (d.count_cur[key] <= 0) continue;
-      const int e = d.list_pos[key];
+      const bool in = pi >= 0 && pj >= 0 && pk >= 0;
+      te[q] = in && d.count_cur[in ? key : 0] > 0 ? d.list_pos[key] : d.tile_cap;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int ox = q >> 2, oy = (q >> 1) & 1, oz = q & 1;
+      const int e = te[q];
       if (e >= d.tile_cap) continue;
       const int tx = li + 4 * ox, ty = lj + 4 * oy, tz = lk + 4 * oz;
       if (tx >= PT || ty >= PT || tz >= PT) continue;
