@@ -1658,10 +1658,14 @@ __global__ void k_unsort(DevPtrs d, int64_t n, float* out, int ncomp_first, int 
   for (int c = 0; c < ncomp; ++c) out[id * ncomp + c] = d.pa[(ncomp_first + c) * d.cap + j];
 }
 
+// Active flag of the particle in row slot j: the sort's record of its new
+// cell (-1 = dead), written for that same slot by the key pass that follows
+// every state update (k_keys at prepare, G2P each step).  The rows are in the
+// order of the sort before it, so the new sort's dead range does not apply.
 __global__ void k_active_out(DevPtrs d, int64_t n, uint8_t* out) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  out[d.pa_id[j]] = (j < d.offsets[d.dead_key]) ? 1 : 0;
+  out[d.pa_id[j]] = d.cell_out[j] >= 0 ? 1 : 0;
 }
 
 __global__ void k_pack_render(DevPtrs d, int64_t n, float* pos4, float* lat7, float* diff) {

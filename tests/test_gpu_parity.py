@@ -332,6 +332,36 @@ def test_grid_mass_independent_of_particle_and_atomic_order():
     np.testing.assert_array_equal(masses[0], masses[1])
 
 
+def test_dead_particles_keep_state_and_stay_out_of_the_grid():
+    """Particles uploaded inactive (every 7th) keep their whole state bit for
+    bit over 50 steps and contribute nothing: the live ones follow the oracle
+    run with the same flags (the reference skips inactive particles in every
+    loop, e.g. p2g.cpp:6-126 and g2p.cpp:7-135), and the active count stays
+    the live count.  Inactive particles sit in the sort's dead range, which
+    the G2P kernel copies forward each step."""
+    s = small()
+    n = s["x"].shape[0]
+    active = np.ones(n, np.uint8)
+    active[::7] = 0
+    dead = active == 0
+    h = abi.Handle(s["cfg"])
+    h.upload(s["x"], pig=s["pig"], active=active)
+    h.set_objects(s["plates"], s["knife"])
+    h.prepare()
+    o = orc.Oracle(abi.Config.from_buffer_copy(s["cfg"]))
+    o.upload(s["x"], pig=s["pig"], active=active)
+    o.set_objects(s["plates"], s["knife"])
+    o.prepare()
+    g0 = h.download()
+    h.step(50)
+    o.step(50)
+    g, r = h.download(), o.download()
+    for key in ("x", "v", "C", "F", "pig", "diff", "eps", "active"):
+        np.testing.assert_array_equal(g[key][dead], g0[key][dead], err_msg=f"dead particles: {key}")
+    assert h.stats().n_active == int(active.sum())
+    check_particles(g, r, scenes.DX, f"{int(dead.sum())} of {n} inactive")
+
+
 def test_render_pack_matches_download():
     s = small()
     h = scenes.make_handle(s)
