@@ -39,24 +39,12 @@ constexpr int MAX_PLATES = 4;
 // reproducible whatever the atomic order; a node holds up to 4096 units.
 constexpr double MASS_FX_SCALE = 4503599627370496.0;        // 2^52
 constexpr double MASS_FX_INV = 1.0 / 4503599627370496.0;
-MPMX_HD unsigned long long mass_to_fx(float w) {
-#ifdef __HIP_DEVICE_COMPILE__
-  // round(w 2^52) for a weight w in [0, 1], in FP32: w 2^52 = hi 2^32 + lo with
-  // hi = floor(w 2^20) and lo = rn(frac(w 2^20) 2^32).  The scalings by powers
-  // of two and the subtraction of the floor are exact, so the only rounding is
-  // the round-to-nearest-even of lo, which is the rounding of w 2^52 itself
-  // (hi carries no fraction): bit-identical to rn((double)w * 2^52).
-  const float t = w * 1048576.0f;
-  const float hf = floorf(t);
-  const float lf = rintf((t - hf) * 4294967296.0f);
-  return ((unsigned long long)(unsigned)hf << 32) | (unsigned long long)(unsigned)lf;
-#else
-  return (unsigned long long)((double)w * MASS_FX_SCALE + 0.5);
-#endif
-}
-
-// mass_to_fx of a weight already scaled by 2^20 (t = w 2^20 in [0, 2^20]):
-// truncation is the floor for t >= 0, and fract(t) = t - floor(t) is exact.
+// Fixed-point mass of a weight w in [0, 1] already scaled by 2^20
+// (t = w 2^20): rn(w 2^52) = hi 2^32 + lo with hi = floor(t) and
+// lo = rn(frac(t) 2^32), in FP32.  The scalings by powers of two and the
+// fraction are exact (truncation is the floor for t >= 0), so the only
+// rounding is the round-to-nearest-even of lo, which is the rounding of
+// w 2^52 itself (hi carries no fraction): bit-identical to rn((double)w 2^52).
 __device__ __forceinline__ void mass_fx_scaled(float t, unsigned& hi, unsigned& lo) {
   hi = (unsigned)t;
   lo = (unsigned)rintf(__builtin_amdgcn_fractf(t) * 4294967296.0f);
