@@ -170,6 +170,48 @@ def kernel_report(kt, learned=False, traffic=False):
     return rep
 
 
+def run_large(args, cap):
+    """The large scene (7,990,272 particles, 512x128x512): plain, then the
+    learned closure trained on pairs harvested from its own plain run."""
+    from pigment_mixing_amd import scenes
+    s6 = scenes.paint_6blob()
+    n6 = s6["x"].shape[0]
+    c6 = s6["cfg"]
+    h6 = scenes.make_handle(s6)
+    wall6, st6, shape6 = timed_run(h6, args.large_steps, args.warmup)
+    kt6 = h6.profile(20)
+    h6.harvest_begin(cap)
+    h6.step(args.harvest_steps)
+    rec6 = h6.harvest_end(cap)
+    h6.close()
+    c6.closure = 2
+    c6.closure_tol = 1e-4
+    hl6 = scenes.make_handle(s6)
+    c6.closure = 0
+    rep6, _ = hl6.closure_train(rec6, steps=args.train_steps, batch=8192, lr=2e-3, lr_final=5e-5, seed=7,
+                                report_every=500)
+    wall6l, st6l, shape6l = timed_run(hl6, args.large_steps, args.warmup)
+    kt6l = hl6.profile(20)
+    st6x = hl6.stats()
+    hl6.close()
+    large = {"workload": s6["name"], "particles": n6, "steps": args.large_steps,
+             "plain": {"value": st6.n_active * args.large_steps / wall6, "unit": "particle-steps/s",
+                       "ms_per_step": wall6 / args.large_steps * 1e3, "wg_per_cu": shape6,
+                       "roofline": kernel_report(kt6)},
+             "learned": {"value": st6l.n_active * args.large_steps / wall6l, "unit": "particle-steps/s",
+                         "ms_per_step": wall6l / args.large_steps * 1e3, "wg_per_cu": shape6l,
+                         "last_step": {"plastic": st6x.plastic_count, "accepted": st6x.learned_count,
+                                       "fallback": st6x.fallback_count},
+                         "training": {"pairs": int(rec6.shape[0]), "loss_full": rep6.loss_full},
+                         "roofline": kernel_report(kt6l, learned=True)}}
+    print(f"[bench] large: {n6} particles, plain {large['plain']['value']:.4g} particle-steps/s "
+          f"({large['plain']['ms_per_step']:.3f} ms/step), learned {large['learned']['value']:.4g} "
+          f"({large['learned']['ms_per_step']:.3f} ms/step); kernels ms g2p={kt6.ms_g2p:.4f} "
+          f"p2g={kt6.ms_p2g:.4f} grid={kt6.ms_grid:.4f}, closure={kt6l.ms_closure:.4f} "
+          f"({kt6l.queued_per_step:.0f} queued)", file=sys.stderr, flush=True)
+    return large
+
+
 def principal_tau(hp, E, nu):
     mu = E / (2 * (1 + nu))
     lam = nu * E / ((1 + nu) * (1 - 2 * nu))
@@ -189,6 +231,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--no-large", action="store_true", help="skip the 8M-particle six-blob scene")
     ap.add_argument("--large-steps", type=int, default=300)
+    ap.add_argument("--only-large", action="store_true",
+                    help="run only the 8M-particle scene and print its JSON object (for kernel traces)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (multi-GPU sharding is design-only); use --gpus 1",
@@ -196,6 +240,10 @@ def main():
         sys.exit(2)
 
     from pigment_mixing_amd import abi, scenes
+
+    if args.only_large:
+        print(json.dumps({"large": run_large(args, 6_000_000)}), flush=True)
+        return
 
     scene = scenes.paint_4blob()
     n = scene["x"].shape[0]
@@ -274,45 +322,9 @@ def main():
               f"({batch['ms_per_step']:.3f} ms/step); tuned wg/CU p2g={tune.wg_p2g} g2p={tune.wg_g2p} "
               f"grid={tune.wg_grid}", file=sys.stderr, flush=True)
 
-    # ---- the large scene (7,990,272 particles, 512x128x512): plain, then the
-    # learned closure trained on pairs harvested from its own plain run
     large = None
     if not args.no_large:
-        s6 = scenes.paint_6blob()
-        n6 = s6["x"].shape[0]
-        c6 = s6["cfg"]
-        h6 = scenes.make_handle(s6)
-        wall6, st6, shape6 = timed_run(h6, args.large_steps, args.warmup)
-        kt6 = h6.profile(20)
-        h6.harvest_begin(cap)
-        h6.step(args.harvest_steps)
-        rec6 = h6.harvest_end(cap)
-        h6.close()
-        c6.closure = 2
-        c6.closure_tol = 1e-4
-        hl6 = scenes.make_handle(s6)
-        c6.closure = 0
-        rep6, _ = hl6.closure_train(rec6, steps=args.train_steps, batch=8192, lr=2e-3, lr_final=5e-5, seed=7,
-                                    report_every=500)
-        wall6l, st6l, shape6l = timed_run(hl6, args.large_steps, args.warmup)
-        kt6l = hl6.profile(20)
-        st6x = hl6.stats()
-        hl6.close()
-        large = {"workload": s6["name"], "particles": n6, "steps": args.large_steps,
-                 "plain": {"value": st6.n_active * args.large_steps / wall6, "unit": "particle-steps/s",
-                           "ms_per_step": wall6 / args.large_steps * 1e3, "wg_per_cu": shape6,
-                           "roofline": kernel_report(kt6)},
-                 "learned": {"value": st6l.n_active * args.large_steps / wall6l, "unit": "particle-steps/s",
-                             "ms_per_step": wall6l / args.large_steps * 1e3, "wg_per_cu": shape6l,
-                             "last_step": {"plastic": st6x.plastic_count, "accepted": st6x.learned_count,
-                                           "fallback": st6x.fallback_count},
-                             "training": {"pairs": int(rec6.shape[0]), "loss_full": rep6.loss_full},
-                             "roofline": kernel_report(kt6l, learned=True)}}
-        print(f"[bench] large: {n6} particles, plain {large['plain']['value']:.4g} particle-steps/s "
-              f"({large['plain']['ms_per_step']:.3f} ms/step), learned {large['learned']['value']:.4g} "
-              f"({large['learned']['ms_per_step']:.3f} ms/step); kernels ms g2p={kt6.ms_g2p:.4f} "
-              f"p2g={kt6.ms_p2g:.4f} grid={kt6.ms_grid:.4f}, closure={kt6l.ms_closure:.4f} "
-              f"({kt6l.queued_per_step:.0f} queued)", file=sys.stderr, flush=True)
+        large = run_large(args, cap)
 
     cpu = None
     if not args.no_cpu_baseline:
