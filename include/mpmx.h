@@ -469,6 +469,41 @@ int mpmx_mlp_forward(const float* params, const float* norm24, const float* x8, 
 int mpmx_mlp_gradient(const float* params, const float* norm24, const float* x8, const float* t4, int64_t n,
                       float* grad, float* loss);
 
+/* ---- scene initialisation: particle sampling of blob objects (host code).
+ *
+ * A blob object's level set as a dense signed-distance grid: phi[(i*dims[1] +
+ * j)*dims[2] + k] at world point origin + (i, j, k) * voxel, negative inside;
+ * the object is placed as X = p * scale + offset.  This is what the
+ * reference's ObjectVdb (object_vdb.hpp:30-118) holds over its active voxel
+ * bounding box; reading .vdb files (OpenVDB) is not part of this library. */
+typedef struct mpmx_level_set {
+  const float* phi;
+  int32_t dims[3];
+  float voxel;
+  float origin[3];
+  float offset[3];
+  float scale;
+} mpmx_level_set;
+
+/* Poisson-disk points with minimum distance `radius` in the box [lo, hi]
+ * (Bridson's algorithm, `attempts` candidates per active sample, seeded),
+ * replacing thinks::PoissonDiskSampling (tph_poisson 0.3) as
+ * sampling_particles_vdb.hpp:33 and :113-117 call it.  *n_out = number of
+ * points; MPMX_ERR_CAPACITY when it exceeds cap (nothing written). */
+int mpmx_sample_poisson_box(const float lo[3], const float hi[3], float radius, int32_t attempts, uint32_t seed,
+                            float* xyz, int64_t cap, int64_t* n_out);
+
+/* The multi-object sampler of sampling_particles_vdb.hpp:86-160: per object
+ * i, box samples over its bounds (object_vdb.hpp:107-118) with seed + i, kept
+ * where the trilinear level set is <= 0 (object_vdb.hpp:73-85) and tagged
+ * object[p] = i (the reference's pigment index); *dx_out = cbrt(ppc / (all box
+ * samples) * (sum of box volumes)), the reference's dx rule.  The reference
+ * calls it with radius 0.01, 30 attempts, seed 42, ppc 8
+ * (simulation_blobs.cpp:149). */
+int mpmx_sample_level_sets(const mpmx_level_set* objects, int32_t n_objects, float radius, int32_t attempts,
+                           uint32_t seed, float ppc, float* xyz, int32_t* object, int64_t cap, int64_t* n_out,
+                           float* dx_out);
+
 #ifdef __cplusplus
 }
 #endif

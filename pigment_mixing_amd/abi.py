@@ -20,9 +20,16 @@ OK, ERR_ARG, ERR_HIP, ERR_STATE, ERR_DEVICE, ERR_CAPACITY, ERR_IO = range(7)
 BC_NOSLIP, BC_SLIPSTICK, BC_SLIPFREE = 0, 1, 2
 PLATE_TOP, PLATE_BOTTOM, PLATE_LEFT, PLATE_RIGHT, PLATE_FRONT, PLATE_BACK = range(6)
 PLASTIC_NONE, PLASTIC_VMVISC, PLASTIC_DPVISC = 0, 5, 6
+ERR_CAPACITY = 5   # MPMX_ERR_CAPACITY
 CLOSURE_EXACT, CLOSURE_LEARNED = 0, 1
 KNIFE_SLAB, KNIFE_SPATULA = 0, 1
 MOTION_STATIC, MOTION_LINEAR, MOTION_CIRCULAR, MOTION_LINEAR_THEN_CIRCULAR, MOTION_KEYFRAMES = range(5)
+
+
+class LevelSet(ctypes.Structure):
+    """mpmx_level_set: a dense signed-distance grid placed as p * scale + offset."""
+    _fields_ = [("phi", ctypes.POINTER(ctypes.c_float)), ("dims", ctypes.c_int32 * 3), ("voxel", ctypes.c_float),
+                ("origin", ctypes.c_float * 3), ("offset", ctypes.c_float * 3), ("scale", ctypes.c_float)]
 
 
 class Config(ctypes.Structure):
@@ -185,6 +192,11 @@ def _sig(lib):
         ("mpmx_mlp_forward", ctypes.c_int, [c_float_p, c_float_p, c_float_p, ctypes.c_int64, c_float_p]),
         ("mpmx_mlp_gradient", ctypes.c_int,
          [c_float_p, c_float_p, c_float_p, c_float_p, ctypes.c_int64, c_float_p, c_float_p]),
+        ("mpmx_sample_poisson_box", ctypes.c_int, [c_float_p, c_float_p, ctypes.c_float, ctypes.c_int32,
+                                                   ctypes.c_uint32, c_float_p, ctypes.c_int64, c_i64_p]),
+        ("mpmx_sample_level_sets", ctypes.c_int,
+         [ctypes.POINTER(LevelSet), ctypes.c_int32, ctypes.c_float, ctypes.c_int32, ctypes.c_uint32, ctypes.c_float,
+          c_float_p, ctypes.POINTER(ctypes.c_int32), ctypes.c_int64, c_i64_p, c_float_p]),
     ]
     for name, res, args in S:
         f = getattr(lib, name)
@@ -280,6 +292,49 @@ def mlp_gradient(params, norm24, x8, t4):
     if rc:
         raise MpmxError(rc, "mpmx_mlp_gradient")
     return float(loss[0]), g
+
+
+def sample_poisson_box(lo, hi, radius, attempts=30, seed=42):
+    """Poisson-disk points (N, 3) with minimum distance `radius` in [lo, hi]."""
+    lib = load()
+    lo3, hi3 = _f32(lo), _f32(hi)
+    n = ctypes.c_int64(0)
+    rc = lib.mpmx_sample_poisson_box(fptr(lo3), fptr(hi3), radius, attempts, seed, None, 0, ctypes.byref(n))
+    if rc not in (0, ERR_CAPACITY):
+        raise MpmxError(rc, "mpmx_sample_poisson_box")
+    out = np.zeros((n.value, 3), np.float32)
+    rc = lib.mpmx_sample_poisson_box(fptr(lo3), fptr(hi3), radius, attempts, seed, fptr(out), n.value, ctypes.byref(n))
+    if rc:
+        raise MpmxError(rc, "mpmx_sample_poisson_box")
+    return out
+
+
+def sample_level_sets(objects, radius=0.01, attempts=30, seed=42, ppc=8.0):
+    """Particles of blob objects given as (phi[nx, ny, nz], voxel, origin3, offset3, scale)
+    tuples: returns (x (N, 3), object index (N,), dx)."""
+    lib = load()
+    keep, sets = [], (LevelSet * len(objects))()
+    for i, (phi, voxel, origin, offset, scale) in enumerate(objects):
+        a = np.ascontiguousarray(phi, np.float32)
+        keep.append(a)
+        sets[i].phi = fptr(a)
+        sets[i].dims[:] = list(a.shape)
+        sets[i].voxel = voxel
+        sets[i].origin[:] = [float(v) for v in origin]
+        sets[i].offset[:] = [float(v) for v in offset]
+        sets[i].scale = scale
+    n, dx = ctypes.c_int64(0), ctypes.c_float(0.0)
+    args = (sets, len(objects), radius, attempts, seed, ppc)
+    rc = lib.mpmx_sample_level_sets(*args, None, None, 0, ctypes.byref(n), ctypes.byref(dx))
+    if rc not in (0, ERR_CAPACITY):
+        raise MpmxError(rc, "mpmx_sample_level_sets")
+    x = np.zeros((n.value, 3), np.float32)
+    obj = np.zeros(n.value, np.int32)
+    rc = lib.mpmx_sample_level_sets(*args, fptr(x), obj.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n.value,
+                                    ctypes.byref(n), ctypes.byref(dx))
+    if rc:
+        raise MpmxError(rc, "mpmx_sample_level_sets")
+    return x, obj, float(dx.value)
 
 
 class MpmxError(RuntimeError):

@@ -29,6 +29,7 @@ SOURCES = [
     "host/kinematics.cpp",
     "host/scene.cpp",
     "host/batch.cpp",
+    "host/sampling.cpp",
 ]
 
 ARCH = os.environ.get("MPMX_ARCH", "gfx950")
