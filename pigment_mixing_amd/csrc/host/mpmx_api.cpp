@@ -350,7 +350,7 @@ void enqueue_step(mpmx_handle* h, const DevPtrs& d) {
   enqueue_g2p(h, d);
   launch_scan(d, h->nkeys, 1, h->stream);
   launch_cellscan(d, grid_grid(h), h->stream);
-  launch_perm(d, h->np, h->stream);
+  launch_perm(d, h->np, grid_grid(h), h->stream);
   launch_p2g(d, grid_p2g(h), 1, h->stream);
   launch_grid(d, grid_grid(h), h->stream);
 }
@@ -788,6 +788,7 @@ int mpmx_prepare(mpmx_handle* h) {
   ALLOC(d.cell_out, int, n);
   ALLOC(d.perm, int, n);
   ALLOC(d.kslot, int, n);
+  ALLOC(d.cmaj, int, n);
   ALLOC(d.cell_cnt, int, (size_t)h->nkeys * BLK3);
   ALLOC(d.cell_off, int, (size_t)h->nkeys * BLK3);
   ALLOC(d.cell_num, int, (size_t)h->nkeys * BLK3);
@@ -855,7 +856,7 @@ int mpmx_prepare(mpmx_handle* h) {
   launch_keys(d, n, h->stream);
   launch_scan(d, h->nkeys, 0, h->stream);
   launch_cellscan(d, grid_grid(h), h->stream);
-  launch_perm(d, n, h->stream);
+  launch_perm(d, n, grid_grid(h), h->stream);
   launch_p2g(d, grid_p2g(h), 0, h->stream);
   launch_grid(d, grid_grid(h), h->stream);
   HIPCHK(h, hipGetLastError());
@@ -1421,7 +1422,7 @@ int mpmx_profile_steps(mpmx_handle* h, int nsteps, mpmx_kernel_times* out) {
       HIPCHK(h, hipEventRecord(ev[1], h->stream));
       launch_scan(d, h->nkeys, 1, h->stream);
       launch_cellscan(d, grid_grid(h), h->stream);
-      launch_perm(d, h->np, h->stream);
+      launch_perm(d, h->np, grid_grid(h), h->stream);
       HIPCHK(h, hipEventRecord(ev[2], h->stream));
       launch_p2g(d, grid_p2g(h), 1, h->stream);
       HIPCHK(h, hipEventRecord(ev[3], h->stream));

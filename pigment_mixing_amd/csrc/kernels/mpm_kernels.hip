@@ -11,7 +11,9 @@
 //   k_scan_*  prefix sums over block counts -> block offsets + block list.
 //   k_cellscan per block: offsets of the 64 cells inside the block, and the
 //             rank-major slot of every (cell, rank).
-//   k_perm    slot <- particle permutation: block-major, rank-major inside a block.
+//   k_perm    particles to cell-major positions by their counting ranks;
+//   k_cellsort orders each cell by previous slot (deterministic) and writes the
+//             permutation slot <- particle: block-major, rank-major in a block.
 //   k_p2g     lane = cell, wave = (block, channel group): each lane walks its
 //             cell's particles and accumulates their 27-node contributions (mass,
 //             APIC momentum, Kirchhoff force, pigment latent, shear intensity;
@@ -311,16 +313,63 @@ __global__ void __launch_bounds__(256) k_cellscan(DevPtrs d) {
 }
 
 // ------------------------------------------------------------------ k_perm
-// New slot of every particle: its block's rank-major slot for (cell, rank in
-// cell) (dead particles: block offset of the DEAD key + rank).
+// Live particles go to the cell-major position (block start + cell offset +
+// counting rank) of `cmaj`, which k_cellsort orders; dead particles go straight
+// to their slot (block offset of the DEAD key + rank).
 __global__ void k_perm(DevPtrs d, int64_t n) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const int key = d.key_out[j];
   const int cell = d.cell_out[j];
-  const int dest = cell >= 0 ? d.kslot[d.offsets[key] + d.cell_off[d.list_pos[key] * BLK3 + cell] + d.rank_out[j]]
-                             : d.offsets[key] + d.rank_out[j];
-  d.perm[dest] = (int)j;
+  if (cell >= 0)
+    d.cmaj[d.offsets[key] + d.cell_off[d.list_pos[key] * BLK3 + cell] + d.rank_out[j]] = (int)j;
+  else
+    d.perm[d.offsets[key] + d.rank_out[j]] = (int)j;
+}
+
+// ------------------------------------------------------------------ k_cellsort
+// Per listed block (one wavefront, lane = cell): the particles of a cell in
+// ascending order of their previous slot.  The counting ranks depend on the
+// timing of the atomics; ordering by previous slot makes the order inside a
+// cell, and with it every FP32 sum of the scatter, a function of the previous
+// order alone.  By induction from the upload order, two runs of a scene are
+// bit-identical, and a scene inside a batch (whose particles keep their
+// relative order) sorts exactly as the same scene alone.  Previous slots are
+// distinct, so the ranks are a permutation of 0..c-1; rank r of the cell goes
+// to its rank-major slot kslot[cell start + r].  Cells of up to CS_LDS
+// particles are ranked from an LDS column per lane, larger ones from global
+// memory (same arithmetic).
+constexpr int CS_LDS = 32;
+__global__ void __launch_bounds__(256) k_cellsort(DevPtrs d) {
+  __shared__ int col[4][CS_LDS][64];
+  const int nblk = d.counters[CNT_PBLOCKS];
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int nwaves = (gridDim.x * blockDim.x) >> 6;
+  const int w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  for (int e = wave; e < nblk; e += nwaves) {
+    const int c = d.cell_num[e * BLK3 + lane];
+    const int base = d.pdesc[e].y + d.cell_off[e * BLK3 + lane];
+    int cmax = c;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) cmax = max(cmax, __shfl_xor(cmax, off));
+    if (cmax <= CS_LDS) {
+      for (int r = 0; r < c; ++r) col[w][r][lane] = d.cmaj[base + r];
+      for (int r = 0; r < c; ++r) {
+        const int v = col[w][r][lane];
+        int rk = 0;
+        for (int s = 0; s < c; ++s) rk += col[w][s][lane] < v ? 1 : 0;
+        d.perm[d.kslot[base + rk]] = v;
+      }
+    } else {
+      for (int r = 0; r < c; ++r) {
+        const int v = d.cmaj[base + r];
+        int rk = 0;
+        for (int s = 0; s < c; ++s) rk += d.cmaj[base + s] < v ? 1 : 0;
+        d.perm[d.kslot[base + rk]] = v;
+      }
+    }
+  }
 }
 
 // Dead particles keep their state: copy them to the output side and rank
@@ -1728,8 +1777,10 @@ int scan_chunks(int nkeys) { return (nkeys + SCAN_CHUNK - 1) / SCAN_CHUNK; }
 void launch_cellscan(const DevPtrs& d, int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_cellscan, dim3(grid), dim3(256), 0, st, d);
 }
-void launch_perm(const DevPtrs& d, int64_t n, hipStream_t st) {
-  if (n > 0) hipLaunchKernelGGL(k_perm, dim3(blocks_for(n, 256)), dim3(256), 0, st, d, n);
+void launch_perm(const DevPtrs& d, int64_t n, int grid, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_perm, dim3(blocks_for(n, 256)), dim3(256), 0, st, d, n);
+  hipLaunchKernelGGL(k_cellsort, dim3(grid), dim3(256), 0, st, d);
 }
 void launch_g2p(const DevPtrs& d, int grid, int learned, int phase, hipStream_t st) {
   const SceneDev* sc = d.scenes;

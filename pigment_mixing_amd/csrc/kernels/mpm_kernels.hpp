@@ -52,6 +52,7 @@ struct DevPtrs {
   int* cell_out;           // cell (0..63) of the particle in its block, -1 if dead
   int* perm;
   int* kslot;              // cell-major position (block start + cell offset + rank) -> rank-major slot
+  int* cmaj;               // live particles at cell-major positions by counting rank (k_perm -> k_cellsort)
   int* cell_cnt;           // per (block, cell) particle counts of the next sort
   int* cell_off;           // per (listed block, cell) offsets inside the block, list order
   int* cell_num;           // per (listed block, cell) counts of the current sort, list order
@@ -130,7 +131,8 @@ void launch_keys(const DevPtrs& d, int64_t n, hipStream_t st);
 void launch_scan(const DevPtrs& d, int nkeys, int advance, hipStream_t st);
 int scan_chunks(int nkeys);
 void launch_cellscan(const DevPtrs& d, int grid, hipStream_t st);
-void launch_perm(const DevPtrs& d, int64_t n, hipStream_t st);
+// k_perm + k_cellsort (grid: workgroups of the per-block pass)
+void launch_perm(const DevPtrs& d, int64_t n, int grid, hipStream_t st);
 // phase 0: fused USL G2P (+ dead-particle copy); MUSL: phase 1 (first half),
 // launch_musl (second scatter + grid velocity), phase 2 (second half + copy).
 // learned (phases 0, 2): queue the closure's candidates for launch_closure.

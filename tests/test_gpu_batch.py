@@ -1,11 +1,12 @@
 """GPU checks of batched scenes (one handle, several independent scenes in the
 same kernels and graph) and of the launch-shape autotuner.
 
-A scene inside a batch must behave as the same scene alone: the only
-difference is FP32 atomic order in the scatter, so the small-scene parity bars
-apply (x <= 1e-4 dx, v <= 1e-3 relative with a 0.1 m/s floor, F <= 1e-4
-relative, latents <= 1e-5).  One batched scene is also checked against the
-CPU oracle."""
+A scene inside a batch must behave bit for bit as the same scene alone: the
+sort orders the particles of a cell by their previous slot (k_cellsort), so the
+scatter's FP32 sums run in the same order whatever the other scenes of the
+batch, the atomic timing or the launch shapes are.  One batched scene is also
+checked against the CPU oracle (x <= 1e-4 dx, v <= 1e-3 relative with a
+0.1 m/s floor, F <= 1e-4 relative, latents <= 1e-5)."""
 import numpy as np
 import pytest
 
@@ -25,6 +26,14 @@ def split(out, rng):
     return {k: v[a:a + n] for k, v in out.items()}
 
 
+def assert_identical(a, b, label):
+    """Every downloaded field equal bit for bit (NaN-free state)."""
+    for k in ("x", "v", "F", "C", "pig", "diff", "eps", "dgamma", "active", "eps_vol", "prad"):
+        diff = np.count_nonzero(a[k] != b[k])
+        assert diff == 0, f"{label}: {diff} values of {k} differ"
+    log(f"[batch] {label}: bit-identical ({len(a['x'])} particles)")
+
+
 def test_batched_scenes_match_individual_runs():
     sl = small_batch(3)
     log("[batch] scenes " + ", ".join(f"{s['name']} n={s['x'].shape[0]} yield={s['yield_pa']:.1f}" for s in sl))
@@ -34,7 +43,24 @@ def test_batched_scenes_match_individual_runs():
     for i, sc in enumerate(sl):
         h1 = scenes.make_handle(sc)
         h1.step(60)
-        check_particles(split(ob, hb.scene_range(i)), h1.download(), scenes.DX, f"batch/{i}", v_tol=1e-3)
+        assert_identical(split(ob, hb.scene_range(i)), h1.download(), f"batch/{i} vs alone")
+
+
+def test_two_runs_are_bit_identical():
+    """Reproducibility: the same batch stepped twice in two handles (USL and
+    MUSL) downloads bit-identical state, and so does the grid."""
+    for musl in (0, 1):
+        sl = small_batch(3)
+        for sc in sl:
+            sc["cfg"].use_musl = musl
+        ha = scenes.make_batch_handle(sl)
+        hb = scenes.make_batch_handle(sl)
+        ha.step(50)
+        hb.step(50)
+        assert_identical(ha.download(), hb.download(), f"two runs (musl={musl})")
+        ga, gb = ha.grid(), hb.grid()
+        for k in ga:
+            assert np.array_equal(ga[k], gb[k]), f"grid {k} differs between runs"
 
 
 def test_batched_scene_matches_oracle():
@@ -59,7 +85,7 @@ def test_autotune_keeps_results():
     ha.step(30)
     hb.step(30 + r.steps)
     assert ha.stats().step == hb.stats().step
-    check_particles(ha.download(), hb.download(), scenes.DX, "autotuned vs default", v_tol=1e-3)
+    assert_identical(ha.download(), hb.download(), "autotuned vs default shapes")
 
 
 def test_batch_mixing_drucker_prager_and_von_mises_scenes():
@@ -68,12 +94,13 @@ def test_batch_mixing_drucker_prager_and_von_mises_scenes():
     shares the MUSL flag): each equals its own run.  The Drucker-Prager scene
     starts from the oracle's step-25 state (1,599 particles at the tension tip,
     restarted through mpmx_upload_plastic_state on an explicit lattice origin),
-    so that the comparison runs where the map is active without the
-    ill-conditioned transient before it; the von Mises scene runs the
-    volume-tracking G2P variant in the batch and the plain one alone, and
-    leaves the volume rows at zero.  Bars of the restart parity test: x <= 1e-5
-    dx, v <= 1e-4 relative (floor 0.1 m/s), F <= 2e-5 relative, eps_pl_vol and
-    the accumulator <= 2e-5, at most 0.5 % differing tip decisions (excluded)."""
+    so that the comparison runs where the map is active.  The Drucker-Prager
+    scene runs the same (volume-tracking) G2P variant in both handles and must
+    match bit for bit.  The von Mises scene runs the volume-tracking variant in
+    the batch and the plain one alone (separately compiled code of the same
+    arithmetic) and must leave the volume rows at zero; its bars are those of
+    the restart parity test: x <= 1e-5 dx, v <= 1e-4 relative (floor 0.1 m/s),
+    F <= 2e-5 relative."""
     app = scenes.paint_app_2blob()
     o0 = orc.from_scene(app)
     o0.step(25)
@@ -106,15 +133,12 @@ def test_batch_mixing_drucker_prager_and_von_mises_scenes():
         a, b = split(ob, hb.scene_range(i)), h.download()
         np.testing.assert_array_equal(a["active"], b["active"])
         if i == 0:
-            same = (a["eps_vol"] != state["eps_vol"]) == (b["eps_vol"] != state["eps_vol"])
-            assert (~same).sum() <= 0.005 * len(same)
-        else:
-            same = np.ones(len(a["x"]), bool)
-            assert not a["eps_vol"].any() and not a["prad"].any()
-        ex = np.abs(a["x"] - b["x"])[same].max() / scenes.DX
-        ev = (np.linalg.norm(a["v"] - b["v"], axis=1) / np.maximum(np.linalg.norm(b["v"], axis=1), 0.1))[same].max()
+            assert_identical(a, b, "mixed/0 (Drucker-Prager) vs alone")
+            continue
+        assert not a["eps_vol"].any() and not a["prad"].any()
+        ex = np.abs(a["x"] - b["x"]).max() / scenes.DX
+        ev = (np.linalg.norm(a["v"] - b["v"], axis=1) / np.maximum(np.linalg.norm(b["v"], axis=1), 0.1)).max()
         eF = (np.linalg.norm((a["F"] - b["F"]).reshape(-1, 9), axis=1) /
-              np.linalg.norm(b["F"].reshape(-1, 9), axis=1))[same].max()
-        ee = max(np.abs(a["eps_vol"] - b["eps_vol"])[same].max(), np.abs(a["prad"] - b["prad"])[same].max())
-        log(f"[batch] mixed/{i}: max|dx|/dx={ex:.2e} rel dv={ev:.2e} rel dF={eF:.2e} max|d eps_vol, prad|={ee:.2e}")
-        assert ex <= 1e-5 and ev <= 1e-4 and eF <= 2e-5 and ee <= 2e-5
+              np.linalg.norm(b["F"].reshape(-1, 9), axis=1)).max()
+        log(f"[batch] mixed/{i}: max|dx|/dx={ex:.2e} rel dv={ev:.2e} rel dF={eF:.2e}")
+        assert ex <= 1e-5 and ev <= 1e-4 and eF <= 2e-5
