@@ -469,6 +469,18 @@ int mpmx_mlp_forward(const float* params, const float* norm24, const float* x8, 
 int mpmx_mlp_gradient(const float* params, const float* norm24, const float* x8, const float* t4, int64_t n,
                       float* grad, float* loss);
 
+/* mpmx_mlp_forward without allocation: the same result as mpmx_mlp_forward
+ * (same kernels) for n <= MPMX_MLP_NOALLOC_MAX samples, computed in a fixed
+ * device array of the library (about 3.2 MB, held by every device the library
+ * runs on until the process ends) on the device's null stream; it calls no
+ * hipMalloc / hipFree / stream creation.  n > MPMX_MLP_NOALLOC_MAX returns
+ * MPMX_ERR_CAPACITY before any device work.  Calls on one device share the
+ * array and the stream: calls that overlap in time on one device race.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (serving loops, harnesses that forbid hipMalloc). */
+#define MPMX_MLP_NOALLOC_MAX 65536
+int mpmx_mlp_forward_noalloc(const float* params, const float* norm24, const float* x8, int64_t n, float* y4);
+
 /* ---- scene initialisation: particle sampling of blob objects (host code).
  *
  * A blob object's level set as a dense signed-distance grid: phi[(i*dims[1] +

@@ -190,6 +190,7 @@ def _sig(lib):
                                               ctypes.POINTER(TrainReport), c_float_p, ctypes.c_int64]),
         ("mpmx_closure_eval", ctypes.c_int, [H, c_float_p, ctypes.c_int64, c_float_p, c_float_p, c_u8_p]),
         ("mpmx_mlp_forward", ctypes.c_int, [c_float_p, c_float_p, c_float_p, ctypes.c_int64, c_float_p]),
+        ("mpmx_mlp_forward_noalloc", ctypes.c_int, [c_float_p, c_float_p, c_float_p, ctypes.c_int64, c_float_p]),
         ("mpmx_mlp_gradient", ctypes.c_int,
          [c_float_p, c_float_p, c_float_p, c_float_p, ctypes.c_int64, c_float_p, c_float_p]),
         ("mpmx_sample_poisson_box", ctypes.c_int, [c_float_p, c_float_p, ctypes.c_float, ctypes.c_int32,
@@ -271,14 +272,20 @@ def train_config(**kw):
     return tc
 
 
-def mlp_forward(params, norm24, x8):
-    """Device forward pass of the closure network: y4 = net(x8) in output units."""
+MLP_NOALLOC_MAX = 65536   # MPMX_MLP_NOALLOC_MAX
+
+
+def mlp_forward(params, norm24, x8, noalloc=False):
+    """Device forward pass of the closure network: y4 = net(x8) in output units.
+    noalloc: through mpmx_mlp_forward_noalloc (fixed device array, at most
+    MLP_NOALLOC_MAX samples)."""
     lib = load()
     p, n24, x = _f32(params), _f32(norm24), _f32(x8, (-1, 8))
     y = np.zeros((x.shape[0], 4), np.float32)
-    rc = lib.mpmx_mlp_forward(fptr(p), fptr(n24), fptr(x), x.shape[0], fptr(y))
+    f = lib.mpmx_mlp_forward_noalloc if noalloc else lib.mpmx_mlp_forward
+    rc = f(fptr(p), fptr(n24), fptr(x), x.shape[0], fptr(y))
     if rc:
-        raise MpmxError(rc, "mpmx_mlp_forward")
+        raise MpmxError(rc, "mpmx_mlp_forward_noalloc" if noalloc else "mpmx_mlp_forward")
     return y
 
 

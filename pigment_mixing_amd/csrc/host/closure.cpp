@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <vector>
 
@@ -251,6 +252,50 @@ int closure_train(std::vector<float>& params, float* n24_io, const float* record
 
 }  // namespace mpmx
 
+// ---- forward pass that allocates nothing: device memory from a fixed array.
+//
+// `scratch` holds, 256-byte aligned: the packed network blob, the FP32
+// parameter and normalisation staging read by the pack kernel, and the input
+// and output rows of up to MPMX_MLP_NOALLOC_MAX samples.  Each device holds its
+// own copy; its address comes from hipGetSymbolAddress, looked up once per
+// device with that device current, and only that address is handed to copies
+// and kernels.  Calls on one device share the array: overlapping calls race.
+namespace mpmx {
+namespace {
+constexpr size_t NA_ALIGN = 256;
+constexpr size_t na_round(size_t b) { return (b + NA_ALIGN - 1) / NA_ALIGN * NA_ALIGN; }
+constexpr size_t NA_BLOB = 0;
+constexpr size_t NA_PAR = NA_BLOB + na_round(MLP_BLOB_BYTES);
+constexpr size_t NA_NORM = NA_PAR + na_round(sizeof(float) * MLP_N_PARAMS);
+constexpr size_t NA_X = NA_NORM + na_round(sizeof(float) * 24);
+constexpr size_t NA_Y = NA_X + na_round(sizeof(float) * 8 * (size_t)MPMX_MLP_NOALLOC_MAX);
+constexpr size_t NA_BYTES = NA_Y + na_round(sizeof(float) * 4 * (size_t)MPMX_MLP_NOALLOC_MAX);
+constexpr int NA_MAX_DEVICES = 64;
+
+__device__ __align__(256) uint8_t scratch[NA_BYTES];
+
+std::mutex na_mutex;
+uint8_t* na_ptr[NA_MAX_DEVICES] = {};
+
+// Device address of this device's `scratch` (looked up on the first call on
+// the device, with it current).
+int na_scratch(uint8_t** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return MPMX_ERR_HIP;
+  if (dev < 0 || dev >= NA_MAX_DEVICES) return MPMX_ERR_CAPACITY;
+  std::lock_guard<std::mutex> lock(na_mutex);
+  if (!na_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return MPMX_ERR_HIP;
+    na_ptr[dev] = static_cast<uint8_t*>(p);
+  }
+  *out = na_ptr[dev];
+  return MPMX_OK;
+}
+}  // namespace
+}  // namespace mpmx
+
+
 using namespace mpmx;
 
 extern "C" {
@@ -282,6 +327,34 @@ int mpmx_mlp_forward(const float* params, const float* norm24, const float* x8, 
   }
   (void)hipStreamDestroy(s);
   return rc;
+}
+
+int mpmx_mlp_forward_noalloc(const float* params, const float* norm24, const float* x8, int64_t n, float* y4) {
+  if (!params || !norm24 || n < 0 || (n > 0 && (!x8 || !y4)) || !norm_ok(norm24)) return MPMX_ERR_ARG;
+  if (n > MPMX_MLP_NOALLOC_MAX) return MPMX_ERR_CAPACITY;
+  if (n == 0) return MPMX_OK;
+  uint8_t* base = nullptr;
+  int rc = na_scratch(&base);
+  if (rc) return rc;
+  uint8_t* blob = base + NA_BLOB;
+  float* dpar = reinterpret_cast<float*>(base + NA_PAR);
+  float* dnorm = reinterpret_cast<float*>(base + NA_NORM);
+  float* dx = reinterpret_cast<float*>(base + NA_X);
+  float* dy = reinterpret_cast<float*>(base + NA_Y);
+  const hipStream_t s = nullptr;
+  // the blob's padding must read zero: cleared by a kernel on every call
+  launch_zero_words(reinterpret_cast<uint32_t*>(blob), MLP_BLOB_BYTES / 4, s);
+  if (hipMemcpyAsync(dpar, params, sizeof(float) * MLP_N_PARAMS, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(dnorm, norm24, sizeof(float) * 24, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(dx, x8, sizeof(float) * 8 * n, hipMemcpyHostToDevice, s) != hipSuccess)
+    return MPMX_ERR_HIP;
+  launch_mlp_pack(dpar, MLP_N_PARAMS, blob, nullptr, dnorm, s);
+  launch_mlp_forward(blob, dx, dy, n, s);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(y4, dy, sizeof(float) * 4 * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return MPMX_ERR_HIP;
+  return MPMX_OK;
 }
 
 int mpmx_mlp_gradient(const float* params, const float* norm24, const float* x8, const float* t4, int64_t n,

@@ -12,6 +12,8 @@
 // W[out][in] then b[out], layers 8->64, 64->64 (x3), 64->4.
 #include "mlp_kernels.hpp"
 
+#include <algorithm>
+
 namespace mpmx {
 
 namespace {
@@ -323,6 +325,14 @@ __global__ void k_mlp_sample(int* __restrict__ idx, int batch, int64_t n, uint64
 }
 
 // ---------------------------------------------------------------- launchers
+__global__ void k_zero_words(uint32_t* __restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+void launch_zero_words(uint32_t* p, int64_t n, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 1024)), dim3(256), 0,
+                                st, p, n);
+}
 void launch_mlp_pack(const float* params, int n_params, uint8_t* blob, uint16_t* wt, const float* norm24,
                      hipStream_t st) {
   const int n = n_params > 24 ? n_params : 24;

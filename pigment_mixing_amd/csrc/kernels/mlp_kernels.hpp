@@ -15,6 +15,8 @@ constexpr int MLP_WT_ELEMS = 3 * 64 * MLP_S + 64 * MLP_S0;
 
 constexpr int MLP_N_PARAMS = (64 * 8 + 64) + 3 * (64 * 64 + 64) + (4 * 64 + 4);
 
+// p[0..n) = 0 (a fill kernel: also valid where a caller captures the stream)
+void launch_zero_words(uint32_t* p, int64_t n, hipStream_t st);
 void launch_mlp_pack(const float* params, int n_params, uint8_t* blob, uint16_t* wt, const float* norm24,
                      hipStream_t st);
 void launch_mlp_forward(const uint8_t* blob, const float* x8, float* y4, int64_t n, hipStream_t st);
