@@ -509,9 +509,11 @@ int mpmx_sample_poisson_box(const float lo[3], const float hi[3], float radius, 
  * i, box samples over its bounds (object_vdb.hpp:107-118) with seed + i, kept
  * where the trilinear level set is <= 0 (object_vdb.hpp:73-85) and tagged
  * object[p] = i (the reference's pigment index); *dx_out = cbrt(ppc / (all box
- * samples) * (sum of box volumes)), the reference's dx rule.  The reference
- * calls it with radius 0.01, 30 attempts, seed 42, ppc 8
- * (simulation_blobs.cpp:149). */
+ * samples) * (sum of box volumes)), the reference's dx rule (volumes summed
+ * and the cube root taken in float, as there).  The reference calls it with
+ * radius 0.01, 30 attempts, seed 42, ppc 27 (simulation_blobs.cpp:149); the
+ * scene's particle volume is then dx^3 / ppc (sampling_particles_vdb.hpp:148),
+ * so mpmx_config.ppc must be the ppc passed here. */
 int mpmx_sample_level_sets(const mpmx_level_set* objects, int32_t n_objects, float radius, int32_t attempts,
                            uint32_t seed, float ppc, float* xyz, int32_t* object, int64_t cap, int64_t* n_out,
                            float* dx_out);

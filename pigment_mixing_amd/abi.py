@@ -316,9 +316,11 @@ def sample_poisson_box(lo, hi, radius, attempts=30, seed=42):
     return out
 
 
-def sample_level_sets(objects, radius=0.01, attempts=30, seed=42, ppc=8.0):
+def sample_level_sets(objects, radius=0.01, attempts=30, seed=42, ppc=27.0):
     """Particles of blob objects given as (phi[nx, ny, nz], voxel, origin3, offset3, scale)
-    tuples: returns (x (N, 3), object index (N,), dx)."""
+    tuples: returns (x (N, 3), object index (N,), dx).  The defaults are the
+    reference's call (simulation_blobs.cpp:149: radius 0.01, ppc 27); the
+    scene's config must use the same ppc (particle volume dx^3 / ppc)."""
     lib = load()
     keep, sets = [], (LevelSet * len(objects))()
     for i, (phi, voxel, origin, offset, scale) in enumerate(objects):

@@ -184,7 +184,9 @@ extern "C" int mpmx_sample_level_sets(const mpmx_level_set* objects, int32_t n_o
   }
   std::vector<P3> kept;
   std::vector<int32_t> tag;
-  double total_volume = 0.0;
+  // the reference accumulates the box volumes and takes the cube root in
+  // float (T = float, sampling_particles_vdb.hpp:100-141)
+  float total_volume = 0.0f;
   int64_t total_samples = 0;
   std::vector<P3> pts;
   for (int32_t i = 0; i < n_objects; ++i) {
@@ -202,7 +204,11 @@ extern "C" int mpmx_sample_level_sets(const mpmx_level_set* objects, int32_t n_o
     }
     const int rc = poisson_box(lo, hi, radius, attempts, seed + (uint32_t)i, pts);
     if (rc != MPMX_OK) return rc;
-    total_volume += ((double)hi[0] - lo[0]) * ((double)hi[1] - lo[1]) * ((double)hi[2] - lo[2]);
+    {
+#pragma clang fp contract(off)
+      const float l0 = hi[0] - lo[0], l1 = hi[1] - lo[1], l2 = hi[2] - lo[2];
+      total_volume += (l0 * l1) * l2;
+    }
     total_samples += (int64_t)pts.size();
     for (const P3& p : pts) {
       // the reference tests the float point it stores
@@ -215,7 +221,10 @@ extern "C" int mpmx_sample_level_sets(const mpmx_level_set* objects, int32_t n_o
   }
   if (total_samples == 0) return MPMX_ERR_ARG;
   *n_out = (int64_t)kept.size();
-  *dx_out = (float)std::cbrt((double)ppc / (double)total_samples * total_volume);
+  {
+#pragma clang fp contract(off)
+    *dx_out = std::cbrt(ppc / (float)total_samples * total_volume);
+  }
   if ((int64_t)kept.size() > cap) return MPMX_ERR_CAPACITY;
   for (size_t p = 0; p < kept.size(); ++p) {
     for (int a = 0; a < 3; ++a) xyz[3 * p + a] = (float)kept[p][a];

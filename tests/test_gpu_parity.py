@@ -401,8 +401,9 @@ def test_particles_leaving_the_lattice_are_deactivated_like_the_oracle():
 def test_scene_sampled_from_level_sets_steps_like_the_oracle():
     """A scene initialised like the reference's blob path (simulation_blobs.cpp:149,
     sampling_particles_vdb.hpp:86-160): two spherical blobs given as signed-distance
-    grids, Poisson-disk sampled (radius 2 mm, 30 attempts, seed 42, ppc 8), dx
-    from the sampler, pigments by object; they fall onto the floor under gravity.
+    grids, Poisson-disk sampled (radius 2 mm, 30 attempts, seed 42, ppc 27 as
+    the reference calls it), dx from the sampler, particle volume dx^3 / 27,
+    pigments by object; they fall onto the floor under gravity.
     The Poisson-disk positions sit off the lattice's sub-cell pattern, unlike the
     box scenes.  50 steps on the GPU match the oracle."""
     from tests.test_sampling_cpu import sphere_grid
@@ -410,11 +411,12 @@ def test_scene_sampled_from_level_sets_steps_like_the_oracle():
     phi, org = sphere_grid((0.0, 0.0, 0.0), R, voxel, 19)
     objects = [(phi, voxel, org, (0.030, R + 0.004, 0.030), 1.0),
                (phi, voxel, org, (0.030 + 2 * R + 0.002, R + 0.004, 0.030), 1.0)]
-    x, obj, dx = abi.sample_level_sets(objects, radius=0.002, attempts=30, seed=42, ppc=8.0)
-    assert set(np.unique(obj)) == {0, 1} and 2.5e-3 < dx < 5e-3
+    x, obj, dx = abi.sample_level_sets(objects, radius=0.002, attempts=30, seed=42, ppc=27.0)
+    assert set(np.unique(obj)) == {0, 1} and 4e-3 < dx < 9e-3
     lat = [scenes.LATENTS["cobalt_blue"], scenes.LATENTS["hansa_yellow"]]
     pig = np.array([lat[i] for i in obj], np.float32)
     cfg = scenes.material_config((40, 24, 24), dx=dx)
+    cfg.ppc = 27.0
     s = dict(name="level_set_2sphere", cfg=cfg, x=x, pig=pig, plates=[scenes.floor_plate()], knife=None)
     h, o = run_both(s, 50)
     g, r = h.download(), o.download()

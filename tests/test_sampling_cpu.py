@@ -72,7 +72,7 @@ def test_level_sets_follow_the_reference_rules():
     r, seed, ppc = 0.004, 42, 8.0
     x, obj, dx = abi.sample_level_sets(objects, radius=r, attempts=30, seed=seed, ppc=ppc)
     assert set(np.unique(obj)) == {0, 1}
-    total_n, total_v = 0, 0.0
+    total_n, total_v = 0, np.float32(0.0)
     f32 = np.float32
     for i, (phi, vox, org, off, sc) in enumerate(objects):
         # the library's FP32 bounds, one rounding per operation
@@ -81,7 +81,8 @@ def test_level_sets_follow_the_reference_rules():
         hi = ((o + (np.array(phi.shape) - 1).astype(f32) * v_) * s_ + of).astype(f32)
         box = abi.sample_poisson_box(lo, hi, r, 30, seed + i)        # seed + i per object
         total_n += box.shape[0]
-        total_v += float(np.prod(hi.astype(np.float64) - lo.astype(np.float64)))
+        ln = (hi - lo).astype(f32)
+        total_v = f32(total_v + f32(f32(ln[0] * ln[1]) * ln[2]))     # float, as the reference
         v = trilinear(phi, vox, org, off, sc, box)
         sure_in, sure_out = v <= -1e-6, v > 1e-6
         mine = x[obj == i]
@@ -90,8 +91,11 @@ def test_level_sets_follow_the_reference_rules():
         kept = {tuple(p) for p in mine}
         assert all(tuple(p) in kept for p in box[sure_in])
         assert not any(tuple(p) in kept for p in box[sure_out])
+        pos = {tuple(p): k for k, p in enumerate(box)}
+        idx = np.array([pos[tuple(p)] for p in mine])
+        assert np.all(np.diff(idx) > 0)
         assert np.all(trilinear(phi, vox, org, off, sc, mine) <= 1e-6)
-    np.testing.assert_allclose(dx, np.cbrt(ppc / total_n * total_v), rtol=1e-6)
+    np.testing.assert_allclose(dx, np.cbrt(f32(f32(ppc) / f32(total_n)) * total_v), rtol=3e-7)
     # the sphere's particles lie in the sphere up to the interpolation error
     c = np.array([0.05, 0.03, 0.04])
     assert np.linalg.norm(x[obj == 0] - c, axis=1).max() <= 0.02 + voxel
