@@ -31,7 +31,13 @@
 //   * Knife: slab (exact box SDF) or the reference trapezoid, with analytic or
 //     keyframed kinematics; the step-0 velocity is defined by the same backward
 //     difference as later steps (the reference leaves it uninitialised).
-//   * SVD: own two-sided Jacobi (oracle_math.hpp), not Eigen.
+//   * Principal frame of F: the reference's JacobiSVD of F followed by log sigma
+//     is computed as the symmetric eigenproblem of F^T F - I (formed from F - I)
+//     with log sigma = log1p(lam) / 2 (oracle_math.hpp principal_of): the same
+//     function of F, without the float rounding of sigma ~ 1 that costs the small
+//     Hencky strains 1e-5..1e-3 of their relative precision.  The plastic F is
+//     rebuilt as F_trial + F_trial V diag(expm1(h_new - h_old)) V^T, equal to
+//     U diag(e^h_new) V^T.
 //
 // n_threads == 1 gives the deterministic sequential order used for parity
 // (the reference itself is only reproducible single-threaded, survey §4).
@@ -463,17 +469,21 @@ struct Oracle {
     for (size_t l = 0; l < n; ++l) gmass[l] *= particle_mass;
   }
 
-  // simulation.hpp:287-294 HenckyPiola, then tau = P F^T (explicit_euler_update.cpp:20)
+  // simulation.hpp:287-294 HenckyPiola, then tau = P F^T (explicit_euler_update.cpp:20):
+  // P = U diag((2 mu ln s_i + lambda sum ln s) / s_i) V^T, so tau = P F^T =
+  // U diag(2 mu ln s_i + lambda sum ln s) U^T, with ln s_i from the principal
+  // frame of F (principal_of: log1p of the eigenvalues of F^T F - I).
   M3 kirchhoff(const M3& Fe) const {
-    M3 U, V;
-    V3 s;
-    svd3_jacobi(Fe, U, s, V);
-    M3 D = zero33();
-    float ls[3] = {std::log(std::fabs(s[0])), std::log(std::fabs(s[1])), std::log(std::fabs(s[2]))};
-    float tr = ls[0] + ls[1] + ls[2];
-    for (int i = 0; i < 3; ++i) D(i, i) = 2 * mu * (1.0f / s[i]) * ls[i] + lambda * tr * (1.0f / s[i]);
-    M3 P = matmul(matmul(U, D), transpose(V));
-    return matmul(P, transpose(Fe));
+    const Principal P = principal_of(Fe);
+    float ls[3] = {log_sigma(P.lam[0]), log_sigma(P.lam[1]), log_sigma(P.lam[2])};
+    float tr = (ls[0] + ls[1]) + ls[2];
+    float d[3];
+    for (int i = 0; i < 3; ++i) d[i] = 2 * mu * ls[i] + lambda * tr;
+    M3 t;
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c)
+        t(r, c) = (P.U(r, 0) * d[0] * P.U(c, 0) + P.U(r, 1) * d[1] * P.U(c, 1)) + P.U(r, 2) * d[2] * P.U(c, 2);
+    return t;
   }
 
   // ObjectSpatula / slab knife pieces (object_spatula.hpp:40-162)
@@ -818,29 +828,49 @@ struct Oracle {
     return dgam;
   }
 
-  static M3 f_from_hencky(const M3& U, const V3& h, const M3& V) {
-    M3 D = zero33();
-    for (int d = 0; d < 3; ++d) D(d, d) = std::exp(h[d]);
-    return matmul(matmul(U, D), transpose(V));
+  // F = U diag(e^h_new) V^T (plasticity.cpp:233).  While no singular value was
+  // clamped (e^h_old = sigma), this equals F_trial V diag(e^(h_new - h_old)) V^T,
+  // evaluated as F_trial + F_trial V diag(expm1(h_new - h_old)) V^T so that the
+  // small correction keeps its relative precision.
+  static M3 f_from_hencky(const M3& Ft, const Principal& P, const V3& h_old, const V3& h_new) {
+    bool clamped = false;
+    for (int d = 0; d < 3; ++d) clamped |= !(1.0f + P.lam[d] >= 1e-8f);
+    M3 out;
+    if (clamped) {
+      M3 D = zero33();
+      for (int d = 0; d < 3; ++d) D(d, d) = std::exp(h_new[d]);
+      return matmul(matmul(P.U, D), transpose(P.V));
+    }
+    float e[3];
+    for (int d = 0; d < 3; ++d) e[d] = std::expm1(h_new[d] - h_old[d]);
+    M3 W;  // V diag(e) V^T
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c)
+        W(r, c) = (P.V(r, 0) * e[0] * P.V(c, 0) + P.V(r, 1) * e[1] * P.V(c, 1)) + P.V(r, 2) * e[2] * P.V(c, 2);
+    M3 FW = matmul(Ft, W);
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) out(r, c) = Ft(r, c) + FW(r, c);
+    return out;
   }
 
-  // plasticity.cpp:17-27: principal Hencky strain, its trace and unit deviator.
-  static void hencky_split(const V3& sig, V3& h, float& htr, V3& dev, float& dn) {
-    for (int d = 0; d < 3; ++d) h[d] = std::log(std::max(std::fabs(sig[d]), 1e-4f));
+  // plasticity.cpp:17-27: principal Hencky strain log(max(sigma, 1e-4)) (sigma
+  // < 1e-4 is 1 + lam < 1e-8), its trace and unit deviator.
+  static void hencky_split(const Principal& P, V3& h, float& htr, V3& dev, float& dn) {
+    for (int d = 0; d < 3; ++d) h[d] = (1.0f + P.lam[d] >= 1e-8f) ? log_sigma(P.lam[d]) : std::log(1e-4f);
     htr = (h[0] + h[1]) + h[2];
     dev = v3(h[0] - htr / 3, h[1] - htr / 3, h[2] - htr / 3);
     dn = norm(dev);
     if (dn > 0) dev = scale(dev, 1.0f / dn);
   }
 
-  // plasticity.cpp:163-238 (VMVisc) on an already computed SVD.  Sets `changed`
+  // plasticity.cpp:163-238 (VMVisc) on the trial state's principal frame.  Sets `changed`
   // when F is replaced; returns 1 when the reference counts a projection
   // (plastic_count++: not for the capped-pressure branch, :172-180).
-  int vmvisc(const M3& U, const V3& sig, const M3& V, const PState& in, M3& Fout, PState& out, bool& changed,
+  int vmvisc(const M3& Ft, const Principal& P, const PState& in, M3& Fout, PState& out, bool& changed,
              int& err) const {
     V3 h, dev;
     float htr, dn;
-    hencky_split(sig, h, htr, dev, dn);
+    hencky_split(P, h, htr, dev, dn);
     float q_trial = e_mu_prefac * dn;
     float q_yield = cfg.q_min + (cfg.q_max - cfg.q_min) * std::exp(-cfg.xi * in.eps_dev);
     float p_trial = -K * htr;
@@ -851,7 +881,7 @@ struct Oracle {
       // capped von Mises: F = U V^T
       float dgam = q_trial / f_mu_prefac;
       float inst = -p_trial / K;
-      Fout = matmul(U, transpose(V));
+      Fout = matmul(P.U, transpose(P.V));
       changed = true;
       out.eps_vol = in.eps_vol + inst;
       out.eps_dev = (float)((double)in.eps_dev + (1.0 / (double)d_prefac) * (double)dgam);
@@ -863,7 +893,7 @@ struct Oracle {
       // (1.0/d_prefac) is a double in the reference; Eigen converts the product
       // to the float scalar type before scaling the vector (plasticity.cpp:232-234).
       float fac = (float)((1.0 / (double)d_prefac) * (double)dgam);
-      Fout = f_from_hencky(U, sub(h, scale(dev, fac)), V);
+      Fout = f_from_hencky(Ft, P, h, sub(h, scale(dev, fac)));
       changed = true;
       out.eps_dev = (float)((double)in.eps_dev + (1.0 / (double)d_prefac) * (double)dgam);
       out.dg = dgam / cfg.dt;
@@ -876,11 +906,11 @@ struct Oracle {
   // it, simulation_units.cpp:185).  Pradhana's correction shifts the yield
   // surface by the volume gained at the tip (p_shift = -K eps_pl_vol_pradhana),
   // resets the accumulation on every state right of the shifted tip.
-  int dpvisc(const M3& U, const V3& sig, const M3& V, const PState& in, M3& Fout, PState& out, bool& changed,
+  int dpvisc(const M3& Ft, const Principal& P, const PState& in, M3& Fout, PState& out, bool& changed,
              int& err) const {
     V3 h, dev;
     float htr, dn;
-    hencky_split(sig, h, htr, dev, dn);
+    hencky_split(P, h, htr, dev, dn);
     float p_trial = -K * htr;
     float q_trial = e_mu_prefac * dn;
     float p_tip = -cfg.q_cohesion / cfg.M;
@@ -897,7 +927,7 @@ struct Oracle {
       float inst = (p_proj - p_trial) / K;
       count = 1;
       float hv = -p_proj / (K * 3);
-      Fout = f_from_hencky(U, v3(hv, hv, hv), V);
+      Fout = f_from_hencky(Ft, P, h, v3(hv, hv, hv));
       changed = true;
       out.dg = dgam / cfg.dt;
       out.eps_dev = (float)((double)in.eps_dev + (1.0 / (double)d_prefac) * (double)dgam);
@@ -911,7 +941,7 @@ struct Oracle {
       count = 1;
       float dgam = perzyna_dgamma(q_trial, q_yield, cfg.perzyna_exp == 1, false, in.eps_dev, err);
       float fac = (float)((1.0 / (double)d_prefac) * (double)dgam);
-      Fout = f_from_hencky(U, sub(h, scale(dev, fac)), V);
+      Fout = f_from_hencky(Ft, P, h, sub(h, scale(dev, fac)));
       changed = true;
       out.eps_dev = (float)((double)in.eps_dev + (1.0 / (double)d_prefac) * (double)dgam);
       out.dg = dgam / cfg.dt;
@@ -920,10 +950,10 @@ struct Oracle {
   }
 
   // plasticity.cpp:10-27 dispatch on the model.
-  int return_map(const M3& U, const V3& sig, const M3& V, const PState& in, M3& Fout, PState& out, bool& changed,
+  int return_map(const M3& Ft, const Principal& P, const PState& in, M3& Fout, PState& out, bool& changed,
                  int& err) const {
-    if (cfg.plastic_model == MPMX_PLASTIC_VMVISC) return vmvisc(U, sig, V, in, Fout, out, changed, err);
-    if (cfg.plastic_model == MPMX_PLASTIC_DPVISC) return dpvisc(U, sig, V, in, Fout, out, changed, err);
+    if (cfg.plastic_model == MPMX_PLASTIC_VMVISC) return vmvisc(Ft, P, in, Fout, out, changed, err);
+    if (cfg.plastic_model == MPMX_PLASTIC_DPVISC) return dpvisc(Ft, P, in, Fout, out, changed, err);
     out = in;
     out.dg = 0.0f;
     changed = false;
@@ -957,15 +987,13 @@ struct Oracle {
       for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) Ft(r, c) = F[p](r, c) + prod(r, c);
       F[p] = Ft;
       if (cfg.plastic_model != MPMX_PLASTIC_NONE) {
-        M3 U, V;
-        V3 s;
-        svd3_jacobi(Ft, U, s, V);
+        const Principal P = principal_of(Ft);
         M3 Fo;
         PState in, out;
         in.eps_dev = eps_pl_dev[p]; in.eps_vol = eps_pl_vol[p]; in.prad = eps_prad[p];
         bool changed = false;
         int err = 0;
-        pc += return_map(U, s, V, in, Fo, out, changed, err);
+        pc += return_map(Ft, P, in, Fo, out, changed, err);
         if (changed) F[p] = Fo;
         eps_pl_dev[p] = out.eps_dev; eps_pl_vol[p] = out.eps_vol; eps_prad[p] = out.prad; dgamma[p] = out.dg;
         err_any |= err;
@@ -1180,14 +1208,14 @@ int orc_knife_state_at(const mpmx_knife* k, const float* frames7, int nframes, f
 }
 
 // ---- unit-level entry points for tests
-void orc_svd3(const float* A9, float* U9, float* s3, float* V9) {
-  M3 A, U, V;
-  V3 s;
+// Principal frame of F (principal_of): U, sigma, V (row-major) and lam = sigma^2 - 1.
+void orc_principal(const float* A9, float* U9, float* s3, float* V9, float* lam3) {
+  M3 A;
   for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) A(r, c) = A9[3 * r + c];
-  svd3_jacobi(A, U, s, V);
+  const Principal P = principal_of(A);
   for (int r = 0; r < 3; ++r)
-    for (int c = 0; c < 3; ++c) { U9[3 * r + c] = U(r, c); V9[3 * r + c] = V(r, c); }
-  for (int d = 0; d < 3; ++d) s3[d] = s[d];
+    for (int c = 0; c < 3; ++c) { U9[3 * r + c] = P.U(r, c); V9[3 * r + c] = P.V(r, c); }
+  for (int d = 0; d < 3; ++d) { s3[d] = P.sig[d]; lam3[d] = P.lam[d]; }
 }
 
 float orc_spline_N(float u) { return spline_N(u); }
@@ -1221,15 +1249,14 @@ int orc_return_map_state(const mpmx_config* cfg, int64_t n, const float* Ft9, co
   o.derived();
   int err_any = 0;
   for (int64_t p = 0; p < n; ++p) {
-    M3 A, U, V, Fo;
-    V3 s;
+    M3 A, Fo;
     for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) A(r, c) = Ft9[9 * p + 3 * r + c];
-    svd3_jacobi(A, U, s, V);
+    const Principal P = principal_of(A);
     Oracle::PState in, out;
     in.eps_dev = in3[3 * p]; in.eps_vol = in3[3 * p + 1]; in.prad = in3[3 * p + 2];
     bool ch = false;
     int err = 0;
-    int cnt = o.return_map(U, s, V, in, Fo, out, ch, err);
+    int cnt = o.return_map(A, P, in, Fo, out, ch, err);
     if (!ch) Fo = A;
     for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) F9[9 * p + 3 * r + c] = Fo(r, c);
     out4[4 * p] = out.eps_dev; out4[4 * p + 1] = out.eps_vol; out4[4 * p + 2] = out.prad; out4[4 * p + 3] = out.dg;

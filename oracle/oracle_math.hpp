@@ -1,19 +1,16 @@
-// oracle_math.hpp — small FP32 3-vector / 3x3 helpers and a two-sided Jacobi
-// SVD for the CPU oracle.
+// oracle_math.hpp — small FP32 3-vector / 3x3 helpers and the principal frame of
+// F (symmetric Jacobi eigen-decomposition of F^T F - I) for the CPU oracle.
 //
 // TEST INFRASTRUCTURE ONLY.  This file belongs to the oracle (a CPU restatement
 // of the reference step used to check the HIP path); nothing in the product
 // includes it.
 //
 // The reference computes every SVD with Eigen's JacobiSVD
-// (matter/src/simulation/simulation.hpp:288, plasticity.cpp:18).  Eigen is not
-// available here, so this is a restatement of Eigen's published algorithm
-// (two-sided Jacobi R-SVD: per (p,q) pair a 2x2 real Jacobi SVD — one rotation
-// making the 2x2 block symmetric, one diagonalising it — sweeps until every
-// off-diagonal is below max(tiny, 2*eps*max|diag|), then |diag| with the sign
-// folded into U, sorted descending).  Eigen's version is unpinned (survey §8c):
-// only isotropic functions of the SVD enter the step, so any accurate SVD gives
-// the same results to rounding; tests pin this one against LAPACK (numpy.linalg.svd).
+// (matter/src/simulation/simulation.hpp:288, plasticity.cpp:18) and uses only
+// isotropic functions of it: log of the singular values, U diag(.) U^T and
+// U diag(.) V^T.  Those are computed here from the eigenproblem of
+// F^T F - I (principal_of below); tests pin it against LAPACK (numpy) in
+// float64 and against the Hencky closed forms.
 #pragma once
 #include <cmath>
 #include <algorithm>
@@ -60,119 +57,93 @@ inline M3 transpose(const M3& a) {
   return r;
 }
 
-// 2x2 real Jacobi rotation (c, s) with J = [c s; -s c], applied as in Eigen's
-// JacobiRotation conventions: "left" multiplies rows p,q, "right" columns p,q.
-struct Rot { float c, s; };
+// ---------------------------------------------------------------- principal frame of F
+// The reference takes the principal Hencky strain as log of the singular values
+// of F (plasticity.cpp:18-20, simulation.hpp:288-290).  Near F = I a float
+// singular value carries ~1 ulp of 1 (6e-8) of absolute error, a relative error
+// of 1e-5..1e-3 in the small strains it encodes.  This computes the same
+// quantities from the symmetric eigenproblem of C = F^T F - I instead, with C
+// formed in double from the float F, so that its entries keep the relative
+// precision of the strain, also when F carries a rotation:
+// C = V diag(lam) V^T,  sigma_i = sqrt(1 + lam_i),  log sigma_i = log1p(lam_i) / 2,
+// U = F V diag(1 / sigma) (columns normalised).  For strongly compressed directions (sigma_i < 0.1)
+// the precision of log sigma_i falls to ~6e-8 / sigma_i^2 (that of 1 + lam_i).
+struct Principal {
+  M3 U, V;
+  V3 sig, lam;  // sigma_i, lam_i = sigma_i^2 - 1 (descending)
+};
 
-inline Rot rot_transpose(Rot r) { return Rot{r.c, -r.s}; }
-inline Rot rot_mul(Rot a, Rot b) {  // a * b
-  return Rot{a.c * b.c - a.s * b.s, a.c * b.s + a.s * b.c};
-}
-
-// Rotations are G(c,s) = [c s; -s c] acting on the index pair (p,q).
-// Symmetric Schur of the 2x2 symmetric block N = [x y; y z]: R = G(c,s) with
-// R^T N R diagonal.  The off-diagonal of R^T N R is cs(x-z) + y(c^2-s^2); with
-// t = s/c this is t^2 - 2 tau t - 1 = 0, tau = (x-z)/(2y); the smaller root is taken.
-inline Rot make_jacobi(float x, float y, float z) {
-  if (std::fabs(y) < 1e-37f) return Rot{1.0f, 0.0f};
-  float tau = (x - z) / (2.0f * y);
-  float w = std::sqrt(1.0f + tau * tau);
-  float t = (tau >= 0.0f) ? -1.0f / (tau + w) : 1.0f / (w - tau);
-  float c = 1.0f / std::sqrt(1.0f + t * t);
-  return Rot{c, t * c};
-}
-
-// Apply on the left: rows p,q of M <- J^T-style rotation as Eigen applyOnTheLeft(p,q,J):
-//   x = M(p,:), y = M(q,:):  M(p,:) = c*x + s*y ... with Eigen's convention
-//   x' = c*x + conj(s)*y ; y' = -s*x + c*y   (for real s)
-inline void apply_left(M3& m, int p, int q, Rot j) {
-  for (int k = 0; k < 3; ++k) {
-    float x = m(p, k), y = m(q, k);
-    m(p, k) = j.c * x + j.s * y;
-    m(q, k) = -j.s * x + j.c * y;
-  }
-}
-// Apply on the right: columns p,q of M: x = M(:,p), y = M(:,q):
-//   x' = c*x - s*y ; y' = s*x + c*y
-inline void apply_right(M3& m, int p, int q, Rot j) {
-  for (int k = 0; k < 3; ++k) {
-    float x = m(k, p), y = m(k, q);
-    m(k, p) = j.c * x - j.s * y;
-    m(k, q) = j.s * x + j.c * y;
-  }
-}
-
-// 2x2 real Jacobi SVD of the (p,q) block of a: returns left J1 and right J2 such that
-// J1 * B * J2 is diagonal (B = a restricted to p,q).
-inline void real_2x2_jacobi_svd(const M3& a, int p, int q, Rot& j_left, Rot& j_right) {
-  float m00 = a(p, p), m01 = a(p, q), m10 = a(q, p), m11 = a(q, q);
-  Rot rot1;
-  float t = m00 + m11;
-  float d = m10 - m01;
-  if (std::fabs(d) < 1e-37f) {
-    rot1 = Rot{1.0f, 0.0f};
-  } else {
-    float u = t / d;
-    float tmp = std::sqrt(1.0f + u * u);
-    rot1 = Rot{u / tmp, 1.0f / tmp};
-  }
-  // apply rot1 on the left of the 2x2 block
-  float n00 = rot1.c * m00 + rot1.s * m10, n01 = rot1.c * m01 + rot1.s * m11;
-  float n10 = -rot1.s * m00 + rot1.c * m10, n11 = -rot1.s * m01 + rot1.c * m11;
-  (void)n10;
-  // R^T (rot1 B) R = diag  =>  left = R^T rot1, right = R (2-D rotations commute).
-  j_right = make_jacobi(n00, n01, n11);
-  j_left = rot_mul(rot_transpose(j_right), rot1);
-}
-
-// A = U diag(s) V^T, s >= 0 sorted descending, U and V orthogonal (det may be -1).
-inline void svd3_jacobi(const M3& A, M3& U, V3& s, M3& V) {
-  M3 w = A;
-  float scalev = 0.0f;
-  for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) scalev = std::max(scalev, std::fabs(w(i, j)));
-  if (!(scalev > 0.0f) || !std::isfinite(scalev)) scalev = 1.0f;
-  for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) w(i, j) /= scalev;
-  U = eye33();
+// Cyclic Jacobi on a symmetric 3x3 (upper triangle used), until the off-diagonal
+// mass is below 1e-9 of the diagonal's (or zero); V accumulates the rotations.
+inline void sym_eig3_jacobi(float C[3][3], M3& V) {
   V = eye33();
-  const float consider_zero = 1.1754944e-38f * 2.0f;  // (min normal) * 2
-  const float precision = 2.0f * 1.1920929e-7f;      // 2 * epsilon
-  float max_diag = std::max(std::fabs(w(0, 0)), std::max(std::fabs(w(1, 1)), std::fabs(w(2, 2))));
-  bool finished = false;
-  int sweeps = 0;
-  while (!finished && sweeps < 64) {
-    finished = true;
-    ++sweeps;
-    for (int p = 1; p < 3; ++p) {
-      for (int q = 0; q < p; ++q) {
-        float threshold = std::max(consider_zero, precision * max_diag);
-        if (std::fabs(w(p, q)) > threshold || std::fabs(w(q, p)) > threshold) {
-          finished = false;
-          Rot jl, jr;
-          real_2x2_jacobi_svd(w, p, q, jl, jr);
-          apply_left(w, p, q, jl);
-          apply_right(U, p, q, rot_transpose(jl));
-          apply_right(w, p, q, jr);
-          apply_right(V, p, q, jr);
-          max_diag = std::max(max_diag, std::max(std::fabs(w(p, p)), std::fabs(w(q, q))));
+  for (int sweep = 0; sweep < 64; ++sweep) {
+    float off = std::fabs(C[0][1]) + std::fabs(C[0][2]) + std::fabs(C[1][2]);
+    float dia = std::fabs(C[0][0]) + std::fabs(C[1][1]) + std::fabs(C[2][2]);
+    if (!(off > 1e-9f * dia) || off < 1e-35f) break;
+    for (int p = 0; p < 2; ++p)
+      for (int q = p + 1; q < 3; ++q) {
+        float apq = C[p][q];
+        if (apq == 0.0f) continue;
+        // rotation zeroing C[p][q]: t = tan(angle), the smaller root of
+        // t^2 + 2 theta t - 1 = 0 with theta = (C_qq - C_pp) / (2 C_pq)
+        float theta = (C[q][q] - C[p][p]) / (2.0f * apq);
+        float t = (theta >= 0.0f ? 1.0f : -1.0f) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0f));
+        float c = 1.0f / std::sqrt(t * t + 1.0f), sn = t * c;
+        C[p][p] -= t * apq;
+        C[q][q] += t * apq;
+        C[p][q] = C[q][p] = 0.0f;
+        for (int r = 0; r < 3; ++r) {
+          if (r == p || r == q) continue;
+          float arp = C[r][p], arq = C[r][q];
+          C[r][p] = C[p][r] = c * arp - sn * arq;
+          C[r][q] = C[q][r] = sn * arp + c * arq;
+        }
+        for (int r = 0; r < 3; ++r) {
+          float vp = V(r, p), vq = V(r, q);
+          V(r, p) = c * vp - sn * vq;
+          V(r, q) = sn * vp + c * vq;
         }
       }
-    }
-  }
-  for (int i = 0; i < 3; ++i) {
-    float a = w(i, i);
-    s[i] = std::fabs(a);
-    if (a < 0.0f) for (int k = 0; k < 3; ++k) U(k, i) = -U(k, i);
-  }
-  for (int i = 0; i < 3; ++i) s[i] *= scalev;
-  // sort descending (selection, swapping columns of U and V)
-  for (int i = 0; i < 3; ++i) {
-    int imax = i;
-    for (int j = i + 1; j < 3; ++j) if (s[j] > s[imax]) imax = j;
-    if (imax != i) {
-      std::swap(s[i], s[imax]);
-      for (int k = 0; k < 3; ++k) { std::swap(U(k, i), U(k, imax)); std::swap(V(k, i), V(k, imax)); }
-    }
   }
 }
+
+inline Principal principal_of(const M3& F) {
+  float C[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      // products of floats are exact in double; the sum and the - 1 round at
+      // 2^-53, so C holds F^T F - I of the float F to ~1e-16 before rounding
+      // to float (relative precision 6e-8 of C itself, i.e. of the strain)
+      double d = ((double)F(0, i) * F(0, j) + (double)F(1, i) * F(1, j)) + (double)F(2, i) * F(2, j);
+      C[i][j] = (float)(d - (i == j ? 1.0 : 0.0));
+    }
+  Principal P;
+  sym_eig3_jacobi(C, P.V);
+  for (int i = 0; i < 3; ++i) P.lam[i] = C[i][i];
+  for (int i = 0; i < 3; ++i) {  // descending
+    int m = i;
+    for (int j = i + 1; j < 3; ++j)
+      if (P.lam[j] > P.lam[m]) m = j;
+    if (m != i) {
+      std::swap(P.lam[i], P.lam[m]);
+      for (int r = 0; r < 3; ++r) std::swap(P.V(r, i), P.V(r, m));
+    }
+  }
+  for (int i = 0; i < 3; ++i) {
+    P.sig[i] = std::sqrt(std::max(1.0f + P.lam[i], 0.0f));
+    // U_i = F v_i / |F v_i| (= F v_i / sigma_i; the norm keeps U_i unit also
+    // where 1 + lam_i has lost relative precision)
+    float w[3];
+    for (int r = 0; r < 3; ++r) w[r] = (F(r, 0) * P.V(0, i) + F(r, 1) * P.V(1, i)) + F(r, 2) * P.V(2, i);
+    float nw = std::sqrt((w[0] * w[0] + w[1] * w[1]) + w[2] * w[2]);
+    float inv = nw > 0.0f ? 1.0f / nw : 0.0f;
+    for (int r = 0; r < 3; ++r) P.U(r, i) = w[r] * inv;
+  }
+  return P;
+}
+
+// log sigma_i (the stress's unclamped log |sigma|, simulation.hpp:290)
+inline float log_sigma(float lam) { return 0.5f * std::log1p(lam); }
 
 }  // namespace orc

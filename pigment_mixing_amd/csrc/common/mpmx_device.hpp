@@ -409,78 +409,130 @@ MPMX_HD float sqrt_approx(float x) {
 #endif
 }
 
-// ---------------------------------------------------------------- 3x3 SVD
-// One-sided (Hestenes) Jacobi: rotate the columns of G = F V pairwise until they
-// are orthogonal; sigma_i = |g_i|, U_i = g_i / sigma_i.  Singular values >= 0,
-// sorted descending; U, V orthogonal (det may be -1), as Eigen's JacobiSVD
-// returns them.  Fixed sweep count (branch-uniform across a wavefront).
-// The rotation's tangent uses the hardware reciprocal and square root: a
-// slightly inexact angle only leaves a rounding-level off-diagonal that the
-// next rotation of the pair removes.  The cosine stays correctly rounded, so
-// (cs, sn = cs t) preserves the column norms -- the singular values near 1,
-// whose logarithms are the (small) Hencky strains, are that sensitive.
-MPMX_HD void svd3(const float F[9], float U[9], float sig[3], float V[9]) {
-  float g[9];
-#pragma unroll
-  for (int i = 0; i < 9; ++i) { g[i] = F[i]; V[i] = 0.0f; }
-  V[0] = V[4] = V[8] = 1.0f;
-#pragma unroll
-  for (int sweep = 0; sweep < 5; ++sweep) {
-#pragma unroll
-    for (int pair = 0; pair < 3; ++pair) {
-      const int p = pair == 2 ? 1 : 0;
-      const int q = pair == 0 ? 1 : 2;
-      float a = g[p] * g[p] + g[3 + p] * g[3 + p] + g[6 + p] * g[6 + p];
-      float b = g[q] * g[q] + g[3 + q] * g[3 + q] + g[6 + q] * g[6 + q];
-      float c = g[p] * g[q] + g[3 + p] * g[3 + q] + g[6 + p] * g[6 + q];
-      float cs = 1.0f, sn = 0.0f;
-      if (fabsf(c) > 1e-30f * sqrt_approx(a * b) && c != 0.0f) {
-        float zeta = (b - a) * rcp_approx(2.0f * c);
-        float t = copysignf(rcp_approx(fabsf(zeta) + sqrt_approx(1.0f + zeta * zeta)), zeta);
-        cs = 1.0f / sqrtf(1.0f + t * t);
-        sn = cs * t;
-      }
-#pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        float gp = g[3 * r + p], gq = g[3 * r + q];
-        g[3 * r + p] = cs * gp - sn * gq;
-        g[3 * r + q] = sn * gp + cs * gq;
-        float vp = V[3 * r + p], vq = V[3 * r + q];
-        V[3 * r + p] = cs * vp - sn * vq;
-        V[3 * r + q] = sn * vp + cs * vq;
-      }
-    }
+// ------------------------------------------------------ principal frame of F
+// The return map and the stress need F = U diag(sigma) V^T and the principal
+// Hencky strains log sigma (plasticity.cpp:18-20, simulation.hpp:288-290).
+// Near F = I a float sigma carries ~1 ulp of 1 (6e-8) of absolute error, which
+// is 1e-5..1e-3 of the small strains it encodes, so two correct FP32 SVDs
+// disagree there by that much.  Instead the strains come from the symmetric
+// eigenproblem of C = F^T F - I, formed in double so that its entries keep the
+// strain's relative precision: C = V diag(lam) V^T, sigma_i^2 = 1 + lam_i,
+// log sigma_i = log1p(lam_i) / 2, U = F V diag(1 / sigma).
+// Cyclic Jacobi, four sweeps over the pairs (0,1), (0,2), (1,2) (fixed count:
+// branch-uniform across a wavefront); each rotation zeroes C_pq with
+// t = tan(angle), the smaller root of t^2 + 2 theta t - 1 = 0,
+// theta = (C_qq - C_pp) / (2 C_pq), and updates the diagonal as
+// C_pp - t C_pq, C_qq + t C_pq.  The cosine is correctly rounded so that V
+// stays orthogonal to rounding.  lam sorted descending (as Eigen sorts sigma).
+MPMX_HD void jacobi_rot(float& app, float& aqq, float& apq, float& arp, float& arq, float* V, int p, int q) {
+  float t = 0.0f;
+  if (fabsf(apq) > 1e-30f) {   // (a smaller C_pq is already zero at this scale)
+    const float theta = (aqq - app) * rcp_approx(2.0f * apq);
+    t = copysignf(rcp_approx(fabsf(theta) + sqrt_approx(fmaf(theta, theta, 1.0f))), theta);
   }
-  float s[3];
+  const float c = 1.0f / sqrtf(fmaf(t, t, 1.0f));
+  const float sn = t * c;
+  app = fmaf(-t, apq, app);
+  aqq = fmaf(t, apq, aqq);
+  apq = 0.0f;
+  const float rp = arp, rq = arq;
+  arp = c * rp - sn * rq;
+  arq = sn * rp + c * rq;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) s[c] = sqrtf(g[c] * g[c] + g[3 + c] * g[3 + c] + g[6 + c] * g[6 + c]);
-  // sort descending (swap columns of g and V)
-#define MPMX_SWAPCOL(i, j)                                                   \
-  if (s[i] < s[j]) {                                                         \
-    float tt = s[i]; s[i] = s[j]; s[j] = tt;                                 \
-    for (int r = 0; r < 3; ++r) {                                            \
-      float t1 = g[3 * r + i]; g[3 * r + i] = g[3 * r + j]; g[3 * r + j] = t1; \
+  for (int r = 0; r < 3; ++r) {
+    const float vp = V[3 * r + p], vq = V[3 * r + q];
+    V[3 * r + p] = c * vp - sn * vq;
+    V[3 * r + q] = sn * vp + c * vq;
+  }
+}
+
+MPMX_HD void principal3(const float F[9], float U[9], float lam[3], float V[9]) {
+  // C = F^T F - I in double: the float products are exact, the sums and the
+  // - 1 round at 2^-53, so C keeps the strain's relative precision (also when
+  // F carries a rotation) until it is rounded to float
+#define MPMX_FF(i, j)                                                             \
+  fma((double)F[i], (double)F[j], fma((double)F[3 + (i)], (double)F[3 + (j)],      \
+                                      (double)F[6 + (i)] * (double)F[6 + (j)]))
+  float c00 = (float)(MPMX_FF(0, 0) - 1.0);
+  float c11 = (float)(MPMX_FF(1, 1) - 1.0);
+  float c22 = (float)(MPMX_FF(2, 2) - 1.0);
+  float c01 = (float)MPMX_FF(0, 1);
+  float c02 = (float)MPMX_FF(0, 2);
+  float c12 = (float)MPMX_FF(1, 2);
+#undef MPMX_FF
+#pragma unroll
+  for (int i = 0; i < 9; ++i) V[i] = (i % 4 == 0) ? 1.0f : 0.0f;
+#pragma unroll
+  for (int sweep = 0; sweep < 4; ++sweep) {
+    jacobi_rot(c00, c11, c01, c02, c12, V, 0, 1);   // r = 2
+    jacobi_rot(c00, c22, c02, c01, c12, V, 0, 2);   // r = 1
+    jacobi_rot(c11, c22, c12, c01, c02, V, 1, 2);   // r = 0
+  }
+  float l[3] = {c00, c11, c22};
+#define MPMX_SWAPEIG(i, j)                                                     \
+  if (l[i] < l[j]) {                                                           \
+    float tt = l[i]; l[i] = l[j]; l[j] = tt;                                   \
+    for (int r = 0; r < 3; ++r) {                                              \
       float t2 = V[3 * r + i]; V[3 * r + i] = V[3 * r + j]; V[3 * r + j] = t2; \
-    }                                                                        \
+    }                                                                          \
   }
-  MPMX_SWAPCOL(0, 1)
-  MPMX_SWAPCOL(0, 2)
-  MPMX_SWAPCOL(1, 2)
-#undef MPMX_SWAPCOL
+  MPMX_SWAPEIG(0, 1)
+  MPMX_SWAPEIG(0, 2)
+  MPMX_SWAPEIG(1, 2)
+#undef MPMX_SWAPEIG
+  // U_i = F v_i / |F v_i| (= F v_i / sigma_i; the norm keeps U_i unit also
+  // where 1 + lam_i has lost relative precision)
 #pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    float inv = s[c] > 0.0f ? 1.0f / s[c] : 0.0f;
-    U[c] = g[c] * inv; U[3 + c] = g[3 + c] * inv; U[6 + c] = g[6 + c] * inv;
+  for (int c = 0; c < 3; ++c) {
+    lam[c] = l[c];
+    float w[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) w[r] = F[3 * r] * V[c] + F[3 * r + 1] * V[3 + c] + F[3 * r + 2] * V[6 + c];
+    const float n2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    const float inv = n2 > 0.0f ? 1.0f / sqrtf(n2) : 0.0f;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) U[3 * r + c] = w[r] * inv;
   }
-  if (s[2] > 1e-30f * s[0]) {
-    float inv = 1.0f / s[2];
-    U[2] = g[2] * inv; U[5] = g[5] * inv; U[8] = g[8] * inv;
-  } else {  // rank-deficient: complete U with the cross product
-    U[2] = U[3] * U[7] - U[6] * U[4];
-    U[5] = U[6] * U[1] - U[0] * U[7];
-    U[8] = U[0] * U[4] - U[3] * U[1];
+}
+
+// Principal Hencky strain of the return map, log(max(sigma, 1e-4))
+// (plasticity.cpp:20; sigma < 1e-4 is 1 + lam < 1e-8).
+MPMX_HD float hencky_of(float lam) { return 1.0f + lam >= 1e-8f ? 0.5f * log1pf(lam) : -9.21034037f; }
+// log |sigma| of the stress (simulation.hpp:290, no clamp).
+MPMX_HD float log_sigma_of(float lam) { return 0.5f * log1pf(lam); }
+
+// F = U diag(e^h_new) V^T after the return map (plasticity.cpp:233).  While no
+// singular value was clamped (e^h_old = sigma), this is
+// Ft V diag(e^(h_new - h_old)) V^T, evaluated as Ft + Ft V diag(expm1(h_new -
+// h_old)) V^T so that the (small) correction keeps its relative precision.
+MPMX_HD void f_from_principal(const float Ft[9], const float U[9], const float V[9], const float lam[3],
+                              const float h_old[3], const float h_new[3], float Fn[9]) {
+  const bool clamped = !(1.0f + lam[0] >= 1e-8f) || !(1.0f + lam[1] >= 1e-8f) || !(1.0f + lam[2] >= 1e-8f);
+  if (clamped) {
+    const float e0 = expf(h_new[0]), e1 = expf(h_new[1]), e2 = expf(h_new[2]);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        Fn[3 * r + c] = (U[3 * r + 0] * e0) * V[3 * c + 0] + (U[3 * r + 1] * e1) * V[3 * c + 1] +
+                        (U[3 * r + 2] * e2) * V[3 * c + 2];
+    return;
   }
-  sig[0] = s[0]; sig[1] = s[1]; sig[2] = s[2];
+  const float e[3] = {expm1f(h_new[0] - h_old[0]), expm1f(h_new[1] - h_old[1]), expm1f(h_new[2] - h_old[2])};
+  float W[6];  // V diag(e) V^T, symmetric: 00 11 22 01 02 12
+  W[0] = V[0] * e[0] * V[0] + V[1] * e[1] * V[1] + V[2] * e[2] * V[2];
+  W[1] = V[3] * e[0] * V[3] + V[4] * e[1] * V[4] + V[5] * e[2] * V[5];
+  W[2] = V[6] * e[0] * V[6] + V[7] * e[1] * V[7] + V[8] * e[2] * V[8];
+  W[3] = V[0] * e[0] * V[3] + V[1] * e[1] * V[4] + V[2] * e[2] * V[5];
+  W[4] = V[0] * e[0] * V[6] + V[1] * e[1] * V[7] + V[2] * e[2] * V[8];
+  W[5] = V[3] * e[0] * V[6] + V[4] * e[1] * V[7] + V[5] * e[2] * V[8];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const float a = Ft[3 * r], b = Ft[3 * r + 1], c = Ft[3 * r + 2];
+    Fn[3 * r + 0] = a + (a * W[0] + b * W[3] + c * W[4]);
+    Fn[3 * r + 1] = b + (a * W[3] + b * W[1] + c * W[5]);
+    Fn[3 * r + 2] = c + (a * W[4] + b * W[5] + c * W[2]);
+  }
 }
 
 // ------------------------------------------------------------ return map

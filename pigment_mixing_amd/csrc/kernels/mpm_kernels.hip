@@ -136,11 +136,11 @@ __global__ void k_init_tau(DevPtrs d, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const SceneDev& sc = d.scenes[d.pscene[i]];
-  float F[9], U[9], sig[3], V[9];
+  float F[9], U[9], lam[3], V[9];
 #pragma unroll
   for (int c = 0; c < 9; ++c) F[c] = d.pa[(P_F + c) * d.cap + i];
-  svd3(F, U, sig, V);
-  float h[3] = {logf(fabsf(sig[0])), logf(fabsf(sig[1])), logf(fabsf(sig[2]))};
+  principal3(F, U, lam, V);
+  float h[3] = {log_sigma_of(lam[0]), log_sigma_of(lam[1]), log_sigma_of(lam[2])};
   float tau[6];
   kirchhoff_from_principal(sc, U, h, tau);
 #pragma unroll
@@ -519,17 +519,6 @@ __device__ __forceinline__ void harvest_pair(const DevPtrs& d, const SceneDev& s
   r[11] = dgr * sc.dt;
 }
 
-// F = U exp(h) V^T from the corrected principal strains.
-__device__ __forceinline__ void f_from_principal(const float U[9], const float h[3], const float V[9], float Fn[9]) {
-  const float e0 = expf(h[0]), e1 = expf(h[1]), e2 = expf(h[2]);
-#pragma unroll
-  for (int r = 0; r < 3; ++r)
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-      Fn[3 * r + c] = (U[3 * r + 0] * e0) * V[3 * c + 0] + (U[3 * r + 1] * e1) * V[3 * c + 1] +
-                      (U[3 * r + 2] * e2) * V[3 * c + 2];
-}
-
 // PHASE 0: the fused USL step.  MUSL (musl.cpp:6-98) splits it around a second
 // scatter: PHASE 1 gathers from the updated grid (velocity, APIC B, FLIP,
 // latent, shear), mixes the latent and sets v (AFLIP), and parks the PIC
@@ -781,10 +770,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
           for (int c = 0; c < 3; ++c)
             Ft[3 * r + c] = F[3 * r + c] + ((sc.dt * L[3 * r + 0]) * F[0 + c] + (sc.dt * L[3 * r + 1]) * F[3 + c] +
                                             (sc.dt * L[3 * r + 2]) * F[6 + c]);
-        float U[9], sig[3], V[9];
-        svd3(Ft, U, sig, V);
-        float h[3] = {logf(fmaxf(fabsf(sig[0]), 1e-4f)), logf(fmaxf(fabsf(sig[1]), 1e-4f)),
-                      logf(fmaxf(fabsf(sig[2]), 1e-4f))};
+        float U[9], lam[3], V[9];
+        principal3(Ft, U, lam, V);
+        float h[3] = {hencky_of(lam[0]), hencky_of(lam[1]), hencky_of(lam[2])};
         float eps = eps_in, dgr = 0.0f;
         uint32_t err = 0;
         const float h_trial[3] = {h[0], h[1], h[2]};
@@ -826,7 +814,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         if (plastic == 1 && valid && d.harvest) harvest_pair(d, sc, h_trial, eps_in, h, dgr);
         float Fn[9];
         if (plastic) {
-          f_from_principal(U, h, V, Fn);
+          f_from_principal(Ft, U, V, lam, h_trial, h, Fn);
         } else {
 #pragma unroll
           for (int c = 0; c < 9; ++c) Fn[c] = Ft[c];
@@ -951,10 +939,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     const SceneDev& sc = scenes[__float_as_int(r0.y)];
     const float eps_in = r0.z;
     const float Ft[9] = {r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w};
-    float U[9], sig[3], V[9];
-    svd3(Ft, U, sig, V);
-    float h[3] = {logf(fmaxf(fabsf(sig[0]), 1e-4f)), logf(fmaxf(fabsf(sig[1]), 1e-4f)),
-                  logf(fmaxf(fabsf(sig[2]), 1e-4f))};
+    float U[9], lam[3], V[9];
+    principal3(Ft, U, lam, V);
+    float h[3] = {hencky_of(lam[0]), hencky_of(lam[1]), hencky_of(lam[2])};
     const float h_trial[3] = {h[0], h[1], h[2]};
     float eps = eps_in, dgr = 0.0f;
     uint32_t err = 0;
@@ -967,7 +954,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
       if (err) atomicOr(&d.err[0], err);
       if (plastic == 1 && d.harvest) harvest_pair(d, sc, h_trial, eps_in, h, dgr);
       float Fn[9];
-      if (plastic) f_from_principal(U, h, V, Fn);
+      if (plastic) f_from_principal(Ft, U, V, lam, h_trial, h, Fn);
       else {
 #pragma unroll
         for (int c = 0; c < 9; ++c) Fn[c] = Ft[c];

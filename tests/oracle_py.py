@@ -45,7 +45,7 @@ def _load(path):
         ("orc_stats", ctypes.c_int, [H, abi.c_i64_p, abi.c_int_p, abi.c_i64_p, ctypes.POINTER(ctypes.c_double)]),
         ("orc_knife_state_at", ctypes.c_int, [ctypes.POINTER(abi.Knife), F, ctypes.c_int, ctypes.c_float,
                                               ctypes.c_double, ctypes.c_int64, F, F, F, F, F]),
-        ("orc_svd3", None, [F, F, F, F]),
+        ("orc_principal", None, [F, F, F, F, F]),
         ("orc_spline_N", ctypes.c_float, [ctypes.c_float]),
         ("orc_spline_dN", ctypes.c_float, [ctypes.c_float]),
         ("orc_knife_sdf", ctypes.c_float, [ctypes.c_int, F, ctypes.c_float, F]),
@@ -173,13 +173,15 @@ def from_scene(scene, fast=False, n_threads=1):
     return o
 
 
-def svd3(A):
+def principal(A):
+    """principal_of(F): U, sigma, V (F = U diag(sigma) V^T) and lam = sigma^2 - 1."""
     A = np.ascontiguousarray(A, np.float32).reshape(9)
     U = np.zeros(9, np.float32)
     s = np.zeros(3, np.float32)
     V = np.zeros(9, np.float32)
-    lib().orc_svd3(fp(A), fp(U), fp(s), fp(V))
-    return U.reshape(3, 3), s, V.reshape(3, 3)
+    lam = np.zeros(3, np.float32)
+    lib().orc_principal(fp(A), fp(U), fp(s), fp(V), fp(lam))
+    return U.reshape(3, 3), s, V.reshape(3, 3), lam
 
 
 def return_map(cfg, Ft, eps=None):

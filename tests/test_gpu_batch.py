@@ -5,8 +5,8 @@ A scene inside a batch must behave bit for bit as the same scene alone: the
 sort orders the particles of a cell by their previous slot (k_cellsort), so the
 scatter's FP32 sums run in the same order whatever the other scenes of the
 batch, the atomic timing or the launch shapes are.  One batched scene is also
-checked against the CPU oracle (x <= 1e-4 dx, v <= 1e-3 relative with a
-0.1 m/s floor, F <= 1e-4 relative, latents <= 1e-5)."""
+checked against the CPU oracle at the parity bars (x <= 1e-4 dx, v <= 1e-4
+relative with a 0.1 m/s floor, F <= 1e-4 relative, latents <= 1e-5)."""
 import numpy as np
 import pytest
 
@@ -70,7 +70,7 @@ def test_batched_scene_matches_oracle():
     ob = hb.download()
     o = orc.from_scene(sl[1])
     o.step(40)
-    check_particles(split(ob, hb.scene_range(1)), o.download(), scenes.DX, "batch/1 vs oracle", v_tol=1e-3)
+    check_particles(split(ob, hb.scene_range(1)), o.download(), scenes.DX, "batch/1 vs oracle")
 
 
 def test_autotune_keeps_results():
@@ -94,13 +94,11 @@ def test_batch_mixing_drucker_prager_and_von_mises_scenes():
     shares the MUSL flag): each equals its own run.  The Drucker-Prager scene
     starts from the oracle's step-25 state (1,599 particles at the tension tip,
     restarted through mpmx_upload_plastic_state on an explicit lattice origin),
-    so that the comparison runs where the map is active.  The Drucker-Prager
-    scene runs the same (volume-tracking) G2P variant in both handles and must
-    match bit for bit.  The von Mises scene runs the volume-tracking variant in
-    the batch and the plain one alone (separately compiled code of the same
-    arithmetic) and must leave the volume rows at zero; its bars are those of
-    the restart parity test: x <= 1e-5 dx, v <= 1e-4 relative (floor 0.1 m/s),
-    F <= 2e-5 relative."""
+    so that the comparison runs where the map is active.  Both scenes must
+    match their own runs bit for bit: the Drucker-Prager scene runs the same
+    (volume-tracking) G2P variant in both handles, the von Mises scene the
+    volume-tracking variant in the batch and the plain one alone (the same
+    arithmetic in two instantiations), and it leaves the volume rows at zero."""
     app = scenes.paint_app_2blob()
     o0 = orc.from_scene(app)
     o0.step(25)
@@ -132,13 +130,6 @@ def test_batch_mixing_drucker_prager_and_von_mises_scenes():
         h.step(5)
         a, b = split(ob, hb.scene_range(i)), h.download()
         np.testing.assert_array_equal(a["active"], b["active"])
-        if i == 0:
-            assert_identical(a, b, "mixed/0 (Drucker-Prager) vs alone")
-            continue
-        assert not a["eps_vol"].any() and not a["prad"].any()
-        ex = np.abs(a["x"] - b["x"]).max() / scenes.DX
-        ev = (np.linalg.norm(a["v"] - b["v"], axis=1) / np.maximum(np.linalg.norm(b["v"], axis=1), 0.1)).max()
-        eF = (np.linalg.norm((a["F"] - b["F"]).reshape(-1, 9), axis=1) /
-              np.linalg.norm(b["F"].reshape(-1, 9), axis=1)).max()
-        log(f"[batch] mixed/{i}: max|dx|/dx={ex:.2e} rel dv={ev:.2e} rel dF={eF:.2e}")
-        assert ex <= 1e-5 and ev <= 1e-4 and eF <= 2e-5
+        if i == 1:
+            assert not a["eps_vol"].any() and not a["prad"].any()
+        assert_identical(a, b, f"mixed/{i} ({'Drucker-Prager' if i == 0 else 'von Mises'}) vs alone")

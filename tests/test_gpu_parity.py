@@ -1,20 +1,20 @@
 """GPU parity of the HIP step (through the C-ABI) against the CPU oracle.
 
 Tolerances (FP32 throughout; the oracle sums in particle order, the GPU in
-sorted/atomic order, so agreement is to rounding, not bitwise):
+sorted order, so agreement is to rounding, not bitwise):
   x      |dx_p| <= 1e-4 * dx                     (dx = lattice spacing)
   v      |dv_p| <= 1e-4 * max(|v_ref|, 0.1 m/s)  (norm-wise, floor 0.1 m/s)
   F      |dF_p| <= 1e-4 * |F_ref|                (Frobenius; F ~ I)
   latent |dpig| <= 1e-5                          (absolute; latents are O(1))
   grid mass from the same particles: relative 4e-6 per node (the oracle's float
              sum of up to 216 weights carries <= 216 * 2^-24 relative error; the
-             GPU sum is exact to 2^-52 of a particle mass) — and bitwise
+             GPU sum is exact to 2^-52 of a particle mass) -- and bitwise
              independent of particle order / atomic order on the GPU.
-The two small scenes with the knife reaching the floor-contact layer amplify
-rounding differences through contact decisions (floor SlipStick, knife
-corners); their velocity bound is 1e-3 (same floor), and 2e-3 for the 200-step
-5 Pa plastic scene, where the return map's yield/no-yield decision near q = q_y
-adds a second switch; all other bounds as above.
+Both sides take the principal Hencky strains from the eigenproblem of
+F^T F - I formed in double (log sigma = log1p(lam) / 2): two different
+correct FP32 SVDs of F would disagree by 1e-5..1e-3 of the small strains
+(float sigma ~ 1 carries 6e-8), which the knife and plastic scenes amplify
+through contact and yield decisions to ~1e-3 in v after 100 steps.
 """
 import os
 import sys
@@ -137,7 +137,7 @@ def test_parity_small_no_knife_100_steps():
 def test_parity_small_elastic_knife_100_steps():
     s = small()
     h, o = run_both(s, 100)
-    check_particles(h.download(), o.download(), scenes.DX, "small/knife/100", v_tol=1e-3)
+    check_particles(h.download(), o.download(), scenes.DX, "small/knife/100")
 
 
 def test_parity_small_plastic_200_steps():
@@ -147,7 +147,7 @@ def test_parity_small_plastic_200_steps():
     ost = o.stats()
     log(f"[parity] plastic count gpu={st.plastic_count} oracle={ost['plastic_count']}")
     assert ost["plastic_count"] > 100, "scene must exercise the return map"
-    check_particles(h.download(), o.download(), scenes.DX, "small/plastic/200", v_tol=2e-3)
+    check_particles(h.download(), o.download(), scenes.DX, "small/plastic/200")
     # per-step plastic counts agree to a few boundary particles
     assert abs(st.plastic_count - ost["plastic_count"]) <= max(5, 0.01 * ost["plastic_count"])
 
@@ -173,7 +173,7 @@ def test_parity_musl_small_knife_100_steps():
     s = small()
     s["cfg"].use_musl = 1
     h, o = run_both(s, 100)
-    check_particles(h.download(), o.download(), scenes.DX, "small/musl/100", v_tol=1e-3)
+    check_particles(h.download(), o.download(), scenes.DX, "small/musl/100")
 
 
 def test_parity_musl_paint_2blob_100_steps():
@@ -189,13 +189,11 @@ def test_parity_paint_app_dpvisc_musl_spatula_100_steps():
     and the trapezoid spatula on keyframes; the initial expansion + shear field
     drives particles past the tension tip and through the Perzyna branch.
     The tip is a threshold on the trial pressure, so this scene amplifies
-    rounding: the oracle's own response to a 1-ulp change of the initial
-    positions is 7e-4 relative in F and v after 100 steps.  Per-particle parity
-    to rounding is pinned from a common state by the restart test below; this
-    one bounds the accumulated drift: x <= 1e-3 dx, F <= 2e-3 relative,
-    v median <= 1e-3 and max <= 1e-2 relative (floor 0.1 m/s), latent <= 1e-5,
-    and the populations: plastic count within 1 %, particles that reached the
-    tip within 2 %, mean eps_pl_vol within 1 %."""
+    rounding; with the principal strains taken from F^T F - I in double on both
+    sides the 100-step run still meets the parity bars (x <= 1e-4 dx, v <= 1e-4
+    relative with a 0.1 m/s floor, F <= 1e-4 relative, latent <= 1e-5), and the
+    populations agree: plastic count within 1 %, particles that reached the tip
+    within 2 %, mean eps_pl_vol within 1 %."""
     s = scenes.paint_app_2blob()
     h, o = run_both(s, 100)
     g, r = h.download(), o.download()
@@ -210,9 +208,9 @@ def test_parity_paint_app_dpvisc_musl_spatula_100_steps():
     log(f"[parity] paint_app/100: max|dx|/dx={ex.max() / dx:.2e} rel dv max {ev.max():.2e} median {np.median(ev):.2e} "
         f"max rel dF={eF.max():.2e} max|dpig|={ep:.2e} plastic gpu={st.plastic_count} oracle={ost['plastic_count']} "
         f"tip gpu={tip_g} oracle={tip_r} mean eps_vol gpu={g['eps_vol'].mean():.4e} oracle={r['eps_vol'].mean():.4e}")
-    assert ex.max() <= 1e-3 * dx
-    assert eF.max() <= 2e-3
-    assert np.median(ev) <= 1e-3 and ev.max() <= 1e-2
+    assert ex.max() <= 1e-4 * dx
+    assert eF.max() <= 1e-4
+    assert ev.max() <= 1e-4
     assert ep <= 1e-5
     assert ost["plastic_count"] > 100 and tip_r > 100
     assert abs(st.plastic_count - ost["plastic_count"]) <= max(5, 0.01 * ost["plastic_count"])
