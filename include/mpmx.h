@@ -460,6 +460,16 @@ int mpmx_closure_train(mpmx_handle* h, const float* records12, int64_t n, const 
 int mpmx_closure_eval(mpmx_handle* h, const float* trial4, int64_t n, float* learned4, float* exact4,
                       uint8_t* accepted);
 
+/* boundaryCollision (boundary_collision.cpp:5-316) of the device's grid
+ * update, evaluated on n given grid nodes: positions X3, post-force
+ * velocities v3 -> out3, against `plates` and the knife (NULL: none) at its
+ * state of step `step`, with cfg's dt and dx.  The reference applies it per
+ * node inside explicitEulerUpdate (explicit_euler_update.cpp:94) and has no
+ * separate entry; this one exists to check the device rules node by node.
+ * MPMX_ERR_DEVICE if a rule reported an invalid boundary condition. */
+int mpmx_boundary_eval(const mpmx_config* cfg, const mpmx_plate* plates, int n_plates, const mpmx_knife* knife,
+                       int64_t step, const float* X3, const float* v3, int64_t n, float* out3);
+
 /* Handle-free evaluation of the network on the device, for checks against a
  * CPU implementation: norm24 = in_mean[8] in_std[8] out_mean[4] out_std[4].
  * mpmx_mlp_forward: y4 = network(x8) in output units.

@@ -1218,6 +1218,27 @@ void orc_principal(const float* A9, float* U9, float* s3, float* V9, float* lam3
   for (int d = 0; d < 3; ++d) { s3[d] = P.sig[d]; lam3[d] = P.lam[d]; }
 }
 
+// One grid node's boundary collision (boundary_collision.cpp:5-316) against the
+// given plates and knife (its state at `step`; NULL: no knife), for tests:
+// out3 = the node velocity after the collision rules.
+int orc_boundary_collision(const mpmx_config* cfg, const mpmx_plate* plates, int np, const mpmx_knife* knife,
+                           int64_t step, const float* X3, const float* v3in, float* out3) {
+  Oracle o;
+  o.cfg = *cfg;
+  o.dx = cfg->dx;
+  o.derived();
+  if (np > 0) o.plates.assign(plates, plates + np);
+  o.has_knife = knife != nullptr;
+  if (knife) {
+    o.knife = *knife;
+    o.ks = knife_state(*knife, o.frames, o.frame_fps, cfg->dt, step);
+  }
+  V3 X = v3(X3[0], X3[1], X3[2]), v = v3(v3in[0], v3in[1], v3in[2]);
+  o.boundary_collision(X, v);
+  for (int d = 0; d < 3; ++d) out3[d] = v[d];
+  return o.exit_flag ? MPMX_ERR_DEVICE : 0;
+}
+
 float orc_spline_N(float u) { return spline_N(u); }
 float orc_spline_dN(float u) { return spline_dN(u); }
 

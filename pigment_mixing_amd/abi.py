@@ -189,6 +189,9 @@ def _sig(lib):
         ("mpmx_closure_train", ctypes.c_int, [H, c_float_p, ctypes.c_int64, ctypes.POINTER(TrainConfig),
                                               ctypes.POINTER(TrainReport), c_float_p, ctypes.c_int64]),
         ("mpmx_closure_eval", ctypes.c_int, [H, c_float_p, ctypes.c_int64, c_float_p, c_float_p, c_u8_p]),
+        ("mpmx_boundary_eval", ctypes.c_int, [ctypes.POINTER(Config), ctypes.POINTER(Plate), ctypes.c_int,
+                                              ctypes.POINTER(Knife), ctypes.c_int64, c_float_p, c_float_p,
+                                              ctypes.c_int64, c_float_p]),
         ("mpmx_mlp_forward", ctypes.c_int, [c_float_p, c_float_p, c_float_p, ctypes.c_int64, c_float_p]),
         ("mpmx_mlp_forward_noalloc", ctypes.c_int, [c_float_p, c_float_p, c_float_p, ctypes.c_int64, c_float_p]),
         ("mpmx_mlp_gradient", ctypes.c_int,
@@ -270,6 +273,21 @@ def train_config(**kw):
     for k, v in kw.items():
         setattr(tc, k, v)
     return tc
+
+
+def boundary_eval(cfg, X, v, plates=(), knife=None, step=0):
+    """Device boundaryCollision of grid nodes X (n, 3) with velocities v (n, 3)
+    against plates and the knife at its state of `step`: the new velocities."""
+    lib = load()
+    Xa, va = _f32(X, (-1, 3)), _f32(v, (-1, 3))
+    out = np.zeros_like(Xa)
+    P = (Plate * max(1, len(plates)))(*plates) if plates else None
+    K = ctypes.pointer(knife) if knife is not None else None
+    rc = lib.mpmx_boundary_eval(ctypes.byref(cfg), P, len(plates), K, step, fptr(Xa), fptr(va), Xa.shape[0],
+                                fptr(out))
+    if rc:
+        raise MpmxError(rc, "mpmx_boundary_eval")
+    return out
 
 
 MLP_NOALLOC_MAX = 65536   # MPMX_MLP_NOALLOC_MAX

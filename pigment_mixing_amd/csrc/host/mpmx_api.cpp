@@ -679,6 +679,57 @@ int mpmx_set_objects(mpmx_handle* h, const mpmx_plate* plates, int n_plates, con
   return mpmx_set_objects_scene(h, 0, plates, n_plates, knives, n_knives);
 }
 
+int mpmx_boundary_eval(const mpmx_config* cfg, const mpmx_plate* plates, int n_plates, const mpmx_knife* knife,
+                       int64_t step, const float* X3, const float* v3, int64_t n, float* out3) {
+  if (!cfg || n < 0 || (n > 0 && (!X3 || !v3 || !out3)) || step < 0 || !(cfg->dt > 0) || !(cfg->dx > 0))
+    return MPMX_ERR_ARG;
+  if (n_plates < 0 || n_plates > MAX_PLATES || (n_plates > 0 && !plates)) return MPMX_ERR_CAPACITY;
+  for (int i = 0; i < n_plates; ++i)
+    if (plates[i].bc < 0 || plates[i].bc > 2 || plates[i].plate_type < 0 || plates[i].plate_type > 5)
+      return MPMX_ERR_ARG;
+  if (knife && (knife->bc < 0 || knife->bc > 2 || knife->shape < 0 || knife->shape > 1)) return MPMX_ERR_ARG;
+  if (n == 0) return MPMX_OK;
+  Scene s;
+  s.cfg = *cfg;
+  if (n_plates > 0) s.plates.assign(plates, plates + n_plates);
+  s.has_knife = knife != nullptr;
+  if (knife) s.knife = *knife;
+  fill_scene_dev(s, 0, 0);
+  const KnifeStateDev ks = to_dev(scene_knife_state(s, step), (float)((double)step * cfg->dt));
+  hipStream_t st = nullptr;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return MPMX_ERR_HIP;
+  SceneDev* d_sc = nullptr;
+  KnifeStateDev* d_ks = nullptr;
+  float* d_buf = nullptr;
+  uint32_t* d_err = nullptr;
+  uint32_t err = 0;
+  const size_t bytes = sizeof(float) * 3 * (size_t)n;
+  int rc = MPMX_OK;
+  if (hipMalloc((void**)&d_sc, sizeof(SceneDev)) != hipSuccess || hipMalloc((void**)&d_ks, sizeof(KnifeStateDev)) != hipSuccess ||
+      hipMalloc((void**)&d_buf, 3 * bytes) != hipSuccess || hipMalloc((void**)&d_err, sizeof(uint32_t)) != hipSuccess ||
+      hipMemcpyAsync(d_sc, &s.dev, sizeof(SceneDev), hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(d_ks, &ks, sizeof(KnifeStateDev), hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(d_buf, X3, bytes, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(d_buf + 3 * n, v3, bytes, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemsetAsync(d_err, 0, sizeof(uint32_t), st) != hipSuccess) {
+    rc = MPMX_ERR_HIP;
+  } else {
+    launch_boundary_eval(d_sc, d_ks, d_buf, d_buf + 3 * n, d_buf + 6 * n, n, d_err, st);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(out3, d_buf + 6 * n, bytes, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(&err, d_err, sizeof(uint32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      rc = MPMX_ERR_HIP;
+  }
+  (void)hipFree(d_sc);
+  (void)hipFree(d_ks);
+  (void)hipFree(d_buf);
+  (void)hipFree(d_err);
+  (void)hipStreamDestroy(st);
+  if (rc == MPMX_OK && err) rc = MPMX_ERR_DEVICE;
+  return rc;
+}
+
 int mpmx_set_objects_scene(mpmx_handle* h, int scene, const mpmx_plate* plates, int n_plates,
                            const mpmx_knife* knives, int n_knives) {
   if (!h) return MPMX_ERR_ARG;

@@ -1738,6 +1738,23 @@ __global__ void k_grid_dense(DevPtrs d, int s, float* mass, float* v, float* fli
   }
 }
 
+// boundary_collision on given nodes (positions X, velocities v) with one
+// scene's plates and knife state: the grid kernel's per-node rule, evaluated
+// for checks of the device code.
+__global__ void k_boundary_eval(const SceneDev* __restrict__ sc, const KnifeStateDev* __restrict__ ks,
+                                const float* __restrict__ X, const float* __restrict__ v, float* __restrict__ out,
+                                int64_t n, uint32_t* __restrict__ err) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float Xi[3] = {X[3 * i], X[3 * i + 1], X[3 * i + 2]};
+  float vi[3] = {v[3 * i], v[3 * i + 1], v[3 * i + 2]};
+  const uint32_t e = boundary_collision(*sc, *ks, Xi, vi);
+  if (e) atomicOr(err, e);
+  out[3 * i] = vi[0];
+  out[3 * i + 1] = vi[1];
+  out[3 * i + 2] = vi[2];
+}
+
 // marks (or clears) the node blocks of the last grid update
 __global__ void k_mark_live(DevPtrs d, int clear) {
   const int parity = d.counters[CNT_GSTEP] & 1;
@@ -1825,6 +1842,11 @@ void launch_active_out(const DevPtrs& d, int64_t n, uint8_t* out, hipStream_t st
 void launch_pack_render(const DevPtrs& d, int64_t n, float* pos4, float* lat7, float* diff, hipStream_t st) {
   if (n > 0)
     hipLaunchKernelGGL(k_pack_render, dim3(blocks_for(n, 256)), dim3(256), 0, st, d, n, pos4, lat7, diff);
+}
+void launch_boundary_eval(const SceneDev* sc, const KnifeStateDev* ks, const float* X, const float* v, float* out,
+                          int64_t n, uint32_t* err, hipStream_t st) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_boundary_eval, dim3(blocks_for(n, 256)), dim3(256), 0, st, sc, ks, X, v, out, n, err);
 }
 void launch_grid_dense(const DevPtrs& d, int s, float* mass, float* v, float* flip, float* pig, float* shear,
                        hipStream_t st) {

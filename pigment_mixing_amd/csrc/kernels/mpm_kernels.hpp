@@ -148,6 +148,8 @@ void launch_grid(const DevPtrs& d, int grid, hipStream_t st);
 void launch_unsort(const DevPtrs& d, int64_t n, float* out, int first, int ncomp, hipStream_t st);
 void launch_active_out(const DevPtrs& d, int64_t n, uint8_t* out, hipStream_t st);
 void launch_pack_render(const DevPtrs& d, int64_t n, float* pos4, float* lat7, float* diff, hipStream_t st);
+void launch_boundary_eval(const SceneDev* sc, const KnifeStateDev* ks, const float* X, const float* v, float* out,
+                          int64_t n, uint32_t* err, hipStream_t st);
 void launch_grid_dense(const DevPtrs& d, int s, float* mass, float* v, float* flip, float* pig, float* shear,
                        hipStream_t st);
 

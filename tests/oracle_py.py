@@ -46,6 +46,8 @@ def _load(path):
         ("orc_knife_state_at", ctypes.c_int, [ctypes.POINTER(abi.Knife), F, ctypes.c_int, ctypes.c_float,
                                               ctypes.c_double, ctypes.c_int64, F, F, F, F, F]),
         ("orc_principal", None, [F, F, F, F, F]),
+        ("orc_boundary_collision", ctypes.c_int, [ctypes.POINTER(abi.Config), ctypes.POINTER(abi.Plate), ctypes.c_int,
+                                                  ctypes.POINTER(abi.Knife), ctypes.c_int64, F, F, F]),
         ("orc_spline_N", ctypes.c_float, [ctypes.c_float]),
         ("orc_spline_dN", ctypes.c_float, [ctypes.c_float]),
         ("orc_knife_sdf", ctypes.c_float, [ctypes.c_int, F, ctypes.c_float, F]),
@@ -182,6 +184,18 @@ def principal(A):
     lam = np.zeros(3, np.float32)
     lib().orc_principal(fp(A), fp(U), fp(s), fp(V), fp(lam))
     return U.reshape(3, 3), s, V.reshape(3, 3), lam
+
+
+def boundary_collision(cfg, X, v, plates=(), knife=None, step=0):
+    """Velocity of a grid node at X after boundaryCollision (one node)."""
+    P = (abi.Plate * max(1, len(plates)))(*plates) if plates else None
+    K = ctypes.pointer(knife) if knife is not None else None
+    X = np.ascontiguousarray(X, np.float32)
+    v = np.ascontiguousarray(v, np.float32)
+    out = np.zeros(3, np.float32)
+    rc = lib().orc_boundary_collision(ctypes.byref(cfg), P, len(plates), K, step, fp(X), fp(v), fp(out))
+    assert rc == 0
+    return out
 
 
 def return_map(cfg, Ft, eps=None):
