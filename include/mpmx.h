@@ -125,14 +125,21 @@ typedef struct {
   int   use_pradhana;        /* DPVisc volume correction (simulation.hpp:86, default 1) */
 } mpmx_config;
 
-/* Axis-aligned plate — ObjectPlate (matter/src/objects/object_plate.hpp:13-78). */
+/* Axis-aligned plate — ObjectPlate (matter/src/objects/object_plate.hpp:13-78).
+ * A plate with a velocity moves at the end of every step as ObjectPlate::move
+ * does (object_plate.hpp:136-162): during the first load_factor frames
+ * (frame = 1 / fps) at v / vmin_factor, later at v; pos_object advances along
+ * the plate's normal axis, pos_lower / pos_upper along its tangential one.
+ * vmin_factor <= 0 is taken as 1. */
 typedef struct {
   float pos_object;
   int   plate_type;          /* mpmx_plate_type */
   int   bc;                  /* mpmx_bc         */
   float friction;
   float pos_lower, pos_upper;
-  float v[3];                /* vx_object, vy_object, vz_object */
+  float v[3];                /* vx_object, vy_object, vz_object (at construction) */
+  float vmin_factor;         /* object_plate.hpp:13 (default 1)                  */
+  float load_factor;         /* object_plate.hpp:13 (default 0)                  */
 } mpmx_plate;
 
 /* Knife description + motion. */

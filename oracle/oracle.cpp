@@ -287,7 +287,8 @@ struct Oracle {
   std::vector<float> gmass, gshear;
   std::vector<L7> gpig;
   std::vector<Nbhd> nb;
-  std::vector<mpmx_plate> plates;
+  std::vector<mpmx_plate> plates;       // state during the current step
+  std::vector<mpmx_plate> plates0;      // as set (construction velocities)
   bool has_knife = false;
   mpmx_knife knife;
   std::vector<float> frames;
@@ -515,6 +516,29 @@ struct Oracle {
     V3 r = sub(X, c);
     V3 om = v3(ks.w[0], ks.w[1], ks.w[2]);
     return add(v3(ks.v[0], ks.v[1], ks.v[2]), cross(om, r));
+  }
+
+  // moveObjects (simulation.cpp:284-291) -> ObjectPlate::move
+  // (object_plate.hpp:136-162): at the end of the step, velocity = construction
+  // velocity (divided by vmin_factor before load_factor frames), then one float
+  // step of dt v along the normal (pos_object) and tangential (lower / upper) axis.
+  void move_plates() {
+    const float frame_dt = (float)(1.0 / cfg.fps);
+    for (size_t i = 0; i < plates.size(); ++i) {
+      const mpmx_plate& o = plates0[i];
+      mpmx_plate& p = plates[i];
+      float vmin = o.vmin_factor > 0 ? o.vmin_factor : 1.0f;
+      float factor = (time < o.load_factor * frame_dt) ? (float)(1.0 / vmin) : 1.0f;
+      for (int c = 0; c < 3; ++c) p.v[c] = o.v[c] * factor;
+      switch (p.plate_type) {
+        case MPMX_PLATE_LEFT: case MPMX_PLATE_RIGHT:
+          p.pos_object += cfg.dt * p.v[0]; p.pos_upper += cfg.dt * p.v[1]; p.pos_lower += cfg.dt * p.v[1]; break;
+        case MPMX_PLATE_BOTTOM: case MPMX_PLATE_TOP:
+          p.pos_object += cfg.dt * p.v[1]; p.pos_upper += cfg.dt * p.v[0]; p.pos_lower += cfg.dt * p.v[0]; break;
+        default:
+          p.pos_object += cfg.dt * p.v[2]; break;
+      }
+    }
   }
 
   // boundary_collision.cpp:5-316 (3-D branch; no MIBF)
@@ -1051,6 +1075,7 @@ struct Oracle {
     if (cfg.use_musl) musl();
     deformation_update();
     position_update();
+    move_plates();
     time += cfg.dt;
     ++step;
   }
@@ -1102,6 +1127,7 @@ int orc_upload(void* h, int64_t n, const float* x, const float* v, const float* 
 int orc_set_objects(void* h, const mpmx_plate* plates, int np, const mpmx_knife* knives, int nk) {
   Oracle* o = (Oracle*)h;
   o->plates.assign(plates, plates + np);
+  o->plates0 = o->plates;
   o->has_knife = nk > 0;
   if (nk > 0) o->knife = knives[0];
   return 0;
@@ -1228,6 +1254,11 @@ int orc_boundary_collision(const mpmx_config* cfg, const mpmx_plate* plates, int
   o.dx = cfg->dx;
   o.derived();
   if (np > 0) o.plates.assign(plates, plates + np);
+  o.plates0 = o.plates;
+  for (int64_t k = 0; k < step; ++k) {   // the plates' state during `step`
+    o.move_plates();
+    o.time += cfg->dt;
+  }
   o.has_knife = knife != nullptr;
   if (knife) {
     o.knife = *knife;

@@ -66,7 +66,7 @@ class Plate(ctypes.Structure):
     _fields_ = [
         ("pos_object", ctypes.c_float), ("plate_type", ctypes.c_int), ("bc", ctypes.c_int),
         ("friction", ctypes.c_float), ("pos_lower", ctypes.c_float), ("pos_upper", ctypes.c_float),
-        ("v", ctypes.c_float * 3),
+        ("v", ctypes.c_float * 3), ("vmin_factor", ctypes.c_float), ("load_factor", ctypes.c_float),
     ]
 
 

@@ -74,13 +74,16 @@ struct KnifeShapeDev {
   float offset, friction, pad2, pad3;
 };
 
-struct KnifeStateDev {   // per scene, per step
+struct KnifeStateDev {   // per scene, per step: the moving objects' state
   float T[12];
   float Tinv[12];
   float N[9];
   float v[3];
   float w[3];
   float time;            // simulated time at the step start (for pigment boost)
+  // plates during this step (ObjectPlate::move, object_plate.hpp:136-162):
+  // pos_object, pos_lower, pos_upper, vx, vy, vz
+  float plate[MAX_PLATES][6];
 };
 
 struct SceneDev {
@@ -345,21 +348,23 @@ MPMX_HD uint32_t boundary_collision(const SceneDev& sc, const KnifeStateDev& ks,
   }
   for (int pi = 0; pi < sc.n_plates; ++pi) {
     const PlateDev& pl = sc.plates[pi];
+    const float pos = ks.plate[pi][0], lower = ks.plate[pi][1], upper = ks.plate[pi][2];
+    const float pv[3] = {ks.plate[pi][3], ks.plate[pi][4], ks.plate[pi][5]};
     bool inside;
     switch (pl.plate_type) {
-      case 2: inside = X[1] < pl.pos_upper && X[1] > pl.pos_lower && (X[0] - pl.pos_object) <= 0.0f; break;
-      case 3: inside = X[1] < pl.pos_upper && X[1] > pl.pos_lower && (pl.pos_object - X[0]) <= 0.0f; break;
-      case 1: inside = X[0] < pl.pos_upper && X[0] > pl.pos_lower && (X[1] - pl.pos_object) <= 0.0f; break;
-      case 0: inside = X[0] < pl.pos_upper && X[0] > pl.pos_lower && (pl.pos_object - X[1]) <= 0.0f; break;
-      case 5: inside = X[2] - pl.pos_object <= 0.0f; break;
-      default: inside = pl.pos_object - X[2] <= 0.0f; break;
+      case 2: inside = X[1] < upper && X[1] > lower && (X[0] - pos) <= 0.0f; break;
+      case 3: inside = X[1] < upper && X[1] > lower && (pos - X[0]) <= 0.0f; break;
+      case 1: inside = X[0] < upper && X[0] > lower && (X[1] - pos) <= 0.0f; break;
+      case 0: inside = X[0] < upper && X[0] > lower && (pos - X[1]) <= 0.0f; break;
+      case 5: inside = X[2] - pos <= 0.0f; break;
+      default: inside = pos - X[2] <= 0.0f; break;
     }
     if (!inside) continue;
     if (influenced && pl.plate_type == 1) {  // knife priority over the floor
-      if (vi[1] < pl.v[1]) { vi[1] = pl.v[1]; vo[0] = vi[0]; vo[1] = vi[1]; vo[2] = vi[2]; }
+      if (vi[1] < pv[1]) { vi[1] = pv[1]; vo[0] = vi[0]; vo[1] = vi[1]; vo[2] = vi[2]; }
       continue;
     }
-    float r[3] = {vo[0] - pl.v[0], vo[1] - pl.v[1], vo[2] - pl.v[2]};
+    float r[3] = {vo[0] - pv[0], vo[1] - pv[1], vo[2] - pv[2]};
     int nax = (pl.plate_type == 0 || pl.plate_type == 1) ? 1 : ((pl.plate_type == 2 || pl.plate_type == 3) ? 0 : 2);
     int ta = (nax == 0) ? 1 : 0, tb = (nax == 2) ? 1 : 2;
     if (pl.bc == 0) {
@@ -386,7 +391,7 @@ MPMX_HD uint32_t boundary_collision(const SceneDev& sc, const KnifeStateDev& ks,
     } else {
       return ERR_BAD_BC;
     }
-    vi[0] = r[0] + pl.v[0]; vi[1] = r[1] + pl.v[1]; vi[2] = r[2] + pl.v[2];
+    vi[0] = r[0] + pv[0]; vi[1] = r[1] + pv[1]; vi[2] = r[2] + pv[2];
     vo[0] = vi[0]; vo[1] = vi[1]; vo[2] = vi[2];
   }
   return 0u;

@@ -23,9 +23,20 @@ namespace {
 constexpr int MAX_CHUNK = 1024;  // knife states uploaded per chunk of steps
 constexpr int MAX_SCENES = 64;   // scenes per batched handle
 
+// Plate positions and velocities during a step (ObjectPlate::move runs at the
+// end of every step, object_plate.hpp:136-162).
+struct PlateRun {
+  int64_t step = -1;           // the step whose state `cur` holds (-1: none yet)
+  float time = 0.0f;           // Simulation::time at the start of that step
+  std::vector<mpmx_plate> cur;
+};
+
 struct Scene {
   mpmx_config cfg;
-  std::vector<mpmx_plate> plates;
+  std::vector<mpmx_plate> plates;    // as set (construction state)
+  int64_t plates_step0 = 0;          // step at which they were set
+  float plates_time0 = 0.0f;
+  PlateRun prun;
   bool has_knife = false;
   mpmx_knife knife;
   std::vector<float> frames;
@@ -262,14 +273,65 @@ void fill_scene_dev(Scene& s, int node_block_base, int axis_off) {
   }
 }
 
-KnifeStateDev to_dev(const mpmx_knife_state& k, float time) {
+// ObjectPlate::move (object_plate.hpp:136-162) at the end of the step that
+// starts at r.time: the construction velocity, divided by vmin_factor during
+// the first load_factor frames, and a float step of dt v along the plate's
+// normal axis (pos_object) and its tangential one (pos_lower, pos_upper).
+void plate_move(const Scene& s, PlateRun& r) {
+  const float dt = s.cfg.dt;
+  const float frame_dt = (float)(1.0 / s.cfg.fps);
+  for (size_t i = 0; i < r.cur.size(); ++i) {
+    const mpmx_plate& o = s.plates[i];
+    mpmx_plate& p = r.cur[i];
+    const float vmin = o.vmin_factor > 0.0f ? o.vmin_factor : 1.0f;
+    const float factor = r.time < o.load_factor * frame_dt ? (float)(1.0 / vmin) : 1.0f;
+    for (int c = 0; c < 3; ++c) p.v[c] = o.v[c] * factor;
+    if (p.plate_type == MPMX_PLATE_LEFT || p.plate_type == MPMX_PLATE_RIGHT) {
+      p.pos_object += dt * p.v[0];
+      p.pos_upper += dt * p.v[1];
+      p.pos_lower += dt * p.v[1];
+    } else if (p.plate_type == MPMX_PLATE_BOTTOM || p.plate_type == MPMX_PLATE_TOP) {
+      p.pos_object += dt * p.v[1];
+      p.pos_upper += dt * p.v[0];
+      p.pos_lower += dt * p.v[0];
+    } else {
+      p.pos_object += dt * p.v[2];
+    }
+  }
+  r.time += dt;
+  ++r.step;
+}
+
+// Plates during step `step`, replayed from the step at which they were set.
+const std::vector<mpmx_plate>& plates_at(Scene& s, int64_t step) {
+  PlateRun& r = s.prun;
+  if (r.step < 0 || r.step > step || r.cur.size() != s.plates.size()) {
+    r.step = s.plates_step0;
+    r.time = s.plates_time0;
+    r.cur = s.plates;
+  }
+  while (r.step < step) plate_move(s, r);
+  return r.cur;
+}
+
+KnifeStateDev to_dev(const mpmx_knife_state& k, float time, const std::vector<mpmx_plate>& plates) {
   KnifeStateDev d;
+  std::memset(&d, 0, sizeof d);
   std::memcpy(d.T, k.T, sizeof d.T);
   std::memcpy(d.Tinv, k.Tinv, sizeof d.Tinv);
   std::memcpy(d.N, k.Nmat, sizeof d.N);
   std::memcpy(d.v, k.v, sizeof d.v);
   std::memcpy(d.w, k.w, sizeof d.w);
   d.time = time;
+  for (size_t i = 0; i < plates.size() && i < (size_t)MAX_PLATES; ++i) {
+    const mpmx_plate& p = plates[i];
+    d.plate[i][0] = p.pos_object;
+    d.plate[i][1] = p.pos_lower;
+    d.plate[i][2] = p.pos_upper;
+    d.plate[i][3] = p.v[0];
+    d.plate[i][4] = p.v[1];
+    d.plate[i][5] = p.v[2];
+  }
   return d;
 }
 
@@ -296,7 +358,7 @@ int upload_kstates(mpmx_handle* h, int64_t g0, int n, float t0) {
       } else {
         k = scene_knife_state(sc, g0 + i);
       }
-      h->h_kstate[(size_t)i * ns + s] = to_dev(k, t);
+      h->h_kstate[(size_t)i * ns + s] = to_dev(k, t, plates_at(sc, g0 + i));
     }
     t += h->scenes[0].cfg.dt;  // Simulation::time accumulates in float
   }
@@ -695,7 +757,9 @@ int mpmx_boundary_eval(const mpmx_config* cfg, const mpmx_plate* plates, int n_p
   s.has_knife = knife != nullptr;
   if (knife) s.knife = *knife;
   fill_scene_dev(s, 0, 0);
-  const KnifeStateDev ks = to_dev(scene_knife_state(s, step), (float)((double)step * cfg->dt));
+  float t_step = 0.0f;
+  for (int64_t i = 0; i < step; ++i) t_step += cfg->dt;   // Simulation::time accumulates in float
+  const KnifeStateDev ks = to_dev(scene_knife_state(s, step), t_step, plates_at(s, step));
   hipStream_t st = nullptr;
   if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return MPMX_ERR_HIP;
   SceneDev* d_sc = nullptr;
@@ -743,6 +807,10 @@ int mpmx_set_objects_scene(mpmx_handle* h, int scene, const mpmx_plate* plates, 
     return fail(h, MPMX_ERR_ARG, "set_objects: invalid knife bc/shape");
   Scene& s = h->scenes[scene];
   s.plates.assign(plates, plates + n_plates);
+  // plates set on a running handle start from the given state at the current step
+  s.plates_step0 = h->prepared ? h->step : 0;
+  s.plates_time0 = h->prepared ? h->time_f : 0.0f;
+  s.prun = PlateRun{};
   s.has_knife = n_knives > 0;
   if (s.has_knife) s.knife = knives[0];
   if (h->prepared) {
@@ -1065,13 +1133,13 @@ int download_rows(mpmx_handle* h, int first, int ncomp, float* out) {
 }
 
 // Checkpoint file: little-endian binary.
-//   char[8] "MPMXCKP1"; u32 sizeof(mpmx_config); mpmx_config (origin_mode 1, the
+//   char[8] "MPMXCKP2"; u32 sizeof(mpmx_config); mpmx_config (origin_mode 1, the
 //   lattice's origin and node counts); i64 n, step; f64 time; f32 time_f; i32 frame;
 //   i32 n_plates; mpmx_plate[n_plates]; i32 has_knife; mpmx_knife (if any);
 //   i32 n_frames; f32 fps; f32[7 n_frames];
 //   f32 x[3n] v[3n] F[9n] C[9n] pig[7n] diff[n] eps_pl_dev[n] delta_gamma[n]
 //   eps_pl_vol[n] eps_pl_vol_pradhana[n] tau[6n]; u8 active[n]   (upload order)
-constexpr char CKPT_MAGIC[8] = {'M', 'P', 'M', 'X', 'C', 'K', 'P', '1'};
+constexpr char CKPT_MAGIC[8] = {'M', 'P', 'M', 'X', 'C', 'K', 'P', '2'};
 
 struct FileCloser {
   FILE* f;

@@ -62,6 +62,7 @@ struct ObjectPlate {
   T friction;
   T pos_lower = -1e15f, pos_upper = 1e15f;
   T vx_object = 0, vy_object = 0, vz_object = 0;
+  T vmin_factor = 1, load_factor = 0;   // object_plate.hpp:13
 };
 
 // Knife (ObjectSpatula's role, object_spatula.hpp): shape, BC and motion.
@@ -144,6 +145,8 @@ class Simulation {
       q.pos_lower = p.pos_lower;
       q.pos_upper = p.pos_upper;
       q.v[0] = p.vx_object; q.v[1] = p.vy_object; q.v[2] = p.vz_object;
+      q.vmin_factor = p.vmin_factor;
+      q.load_factor = p.load_factor;
       pl.push_back(q);
     }
     check(mpmx_set_objects(h_, pl.data(), (int)pl.size(), knife ? &knife->spec : nullptr, knife ? 1 : 0),

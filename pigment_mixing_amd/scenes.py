@@ -75,6 +75,7 @@ def floor_plate(mu=1.0):
     p.pos_lower = -1e15
     p.pos_upper = 1e15
     p.v[:] = [0.0, 0.0, 0.0]
+    p.vmin_factor, p.load_factor = 1.0, 0.0
     return p
 
 

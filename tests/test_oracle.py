@@ -295,40 +295,6 @@ def test_free_fall_and_translation():
     np.testing.assert_allclose(out["F"], np.broadcast_to(np.eye(3), out["F"].shape), atol=1e-6)
 
 
-@pytest.mark.slow
-def test_coulomb_friction_incline():
-    """Port of CoulombFrictionTest (matter/tests/tests.cpp:100-306): an elastic block on a
-    SlipFree plate under gravity tilted 24 deg with mu = tan 15 deg slides with mean
-    displacement 0.5 g t^2 (sin th - mu cos th).  Reduced: 0.15 s instead of 5 s."""
-    th = math.radians(24)
-    mu = math.tan(math.radians(15))
-    dx = 0.01
-    s = scenes.paint_2blob(cells=(8, 3, 4), dims=(48, 12, 12), dx=dx, knife=False)
-    x = s["x"].copy()
-    x[:, 1] += 0.5 * dx
-    c = s["cfg"]
-    c.plastic_model = abi.PLASTIC_NONE
-    c.E = 1e6
-    c.rho = 1000.0
-    c.flip_ratio = -0.95
-    c.gravity[:] = [9.81 * math.sin(th), -9.81 * math.cos(th), 0.0]
-    c.dt = 2e-4
-    c.grid_reference_point[:] = [0, 0, 0]
-    p = scenes.floor_plate(mu)
-    p.bc = abi.BC_SLIPFREE
-    o = orc.Oracle(c)
-    o.upload(x, pig=s["pig"])
-    o.set_objects([p], None)
-    o.prepare()
-    nsteps = 750
-    o.step(nsteps)
-    out = o.download()
-    t = nsteps * c.dt
-    ref = 0.5 * 9.81 * t * t * (math.sin(th) - mu * math.cos(th))
-    got = (out["x"][:, 0] - x[:, 0]).mean()
-    assert abs(got - ref) / ref < 2e-2, (got, ref)
-
-
 def test_musl_free_fall_translation_and_rigid_rotation():
     """MUSL (musl.cpp:6-98) leaves affine velocity fields and free fall exact:
     the second scatter of v + B (x_i - x_p) reproduces the same affine grid
