@@ -225,6 +225,67 @@ def test_spatula_sdf_sign():
     assert abs(d - (0.5 - 0.07)) < 1e-6             # straight above the blade: |y| - halfThickness
 
 
+def _spatula_outline(hw, hl, off, n=20001):
+    """Boundary of the spatula's (|x|, z) cross-section as dense points: the
+    slanted side from (b1, -he) to (b2, he), the top semicircle of radius b2
+    about (0, he) and the bottom half-ellipse of radii (b1, 1.5 b1) about
+    (0, -he) (object_spatula.hpp:40-82), b1 = hw + off, b2 = b1 / 4, he = hl + off."""
+    b1, he = hw + off, hl + off
+    b2 = 0.25 * b1
+    t = np.linspace(0.0, 1.0, n)
+    side = np.stack([b1 + (b2 - b1) * t, -he + 2 * he * t], 1)
+    a = np.linspace(0.0, 0.5 * np.pi, n)
+    top = np.stack([b2 * np.cos(a), he + b2 * np.sin(a)], 1)
+    bot = np.stack([b1 * np.cos(a), -he - 1.5 * b1 * np.sin(a)], 1)
+    return b1, b2, he, side, top, bot
+
+
+@pytest.mark.parametrize("dims", [(0.368, 0.07, 0.736, 0.05), (0.006, 0.0015, 0.008, 0.0005)])
+def test_spatula_sdf_surface_and_distances(dims):
+    """sdSpatula (object_spatula.hpp:40-82) against the shape it describes:
+    its zero level set is the outline (sign = inside test of the exact
+    cross-section, extruded by halfThickness in y) everywhere; beside the
+    slanted side and above the rounded tip it is the Euclidean distance
+    (brute force over the densely sampled outline), also combined with the
+    extrusion as sqrt(d_2d^2 + d_y^2)."""
+    hw, ht, hl, off = dims
+    half = np.array([hw, ht, hl], np.float32)
+    b1, b2, he, side, top, bot = _spatula_outline(hw, hl, off)
+    sdf = lambda q: orc.lib().orc_knife_sdf(1, orc.fp(half), off, orc.fp(np.asarray(q, np.float32)))
+
+    def inside2d(px, pz):
+        if pz > he:
+            return px * px + (pz - he) ** 2 <= b2 * b2
+        if pz < -he:
+            return (px / b1) ** 2 + ((pz + he) / (1.5 * b1)) ** 2 <= 1.0
+        return px <= b1 - (b1 - b2) * (pz + he) / (2 * he)
+
+    r = rng(11)
+    L = 2.5 * (he + 1.5 * b1)
+    n_side = n_top = 0
+    for _ in range(3000):
+        q = np.array([r.uniform(-2 * b1, 2 * b1), r.uniform(-3 * ht, 3 * ht), r.uniform(-L / 2, L / 2)])
+        q = q.astype(np.float32).astype(np.float64)
+        px, py, pz = abs(q[0]), q[1], q[2]
+        d = sdf(q)
+        pts = np.concatenate([side, top, bot])
+        dist = np.hypot(pts[:, 0] - px, pts[:, 1] - pz)
+        k = int(np.argmin(dist))
+        d2_exact = dist[k] * (-1 if inside2d(px, pz) else 1)
+        dy = abs(py) - ht
+        if abs(d2_exact) > 1e-5 * L and abs(dy) > 1e-5 * L:
+            assert (d < 0) == (inside2d(px, pz) and dy < 0), (q, d)
+        outside2d = not inside2d(px, pz)
+        on_side = k < len(side) and 0 < k < len(side) - 1
+        on_top = len(side) <= k < len(side) + len(top) and pz > he
+        if outside2d and (on_side and -he < pz < he or on_top):
+            ext = np.hypot(max(d2_exact, 0.0), max(dy, 0.0)) + min(max(d2_exact, dy), 0.0)
+            assert abs(d - ext) <= 2e-4 * b1 + 1e-6 * L, (q, d, ext)
+            n_side += on_side
+            n_top += on_top
+    assert n_side > 50 and n_top > 50
+
+
 # ----------------------------------------------------------------- knife kinematics
 def test_knife_kinematics_linear_and_circular():
     k = scenes.slab_knife(abi.MOTION_LINEAR, (0.1, 0.02, 0.03), velocity=(0.1, 0.0, 0.0))
@@ -298,8 +359,9 @@ def test_free_fall_and_translation():
 def test_musl_free_fall_translation_and_rigid_rotation():
     """MUSL (musl.cpp:6-98) leaves affine velocity fields and free fall exact:
     the second scatter of v + B (x_i - x_p) reproduces the same affine grid
-    field, so F stays I under translation, the rigid-rotation field keeps its
-    velocities, and free fall adds dt*g per step as under USL."""
+    field, so F stays I under translation, a rigid rotation keeps its velocities
+    and energy under USL and MUSL, and free fall adds dt*g per step as under
+    USL; a non-affine shear makes MUSL differ from USL."""
     dt = 1e-5
     s = _mini_scene()
     s["plates"] = []
@@ -336,3 +398,26 @@ def test_musl_free_fall_translation_and_rigid_rotation():
         o.step(2)
         res.append(o.download()["F"])
     assert np.abs(res[0] - res[1]).max() > 0
+    # rigid rotation v = W (x - c) with the matching APIC matrix Bmat = W dx^2/4:
+    # the transfers carry the affine field exactly, so over two steps (USL and
+    # MUSL) each particle keeps its velocity to O(omega dt) = 2e-5, the kinetic
+    # energy is kept, and F stays a rotation to O((omega dt)^2)
+    dx = s["cfg"].dx
+    omega = 2.0
+    W = np.array([[0.0, -omega, 0.0], [omega, 0.0, 0.0], [0.0, 0.0, 0.0]], np.float32)
+    xc = s["x"].mean(0)
+    vr = ((s["x"] - xc) @ W.T).astype(np.float32)
+    Cr = np.broadcast_to(W * (dx * dx / 4.0), (s["x"].shape[0], 3, 3)).astype(np.float32).copy()
+    for musl in (0, 1):
+        s["cfg"].use_musl = musl
+        o = orc.Oracle(s["cfg"])
+        o.upload(s["x"], v=vr, C=Cr, pig=s["pig"])
+        o.prepare()
+        o.step(2)
+        out = o.download()
+        vmax = np.linalg.norm(vr, axis=1).max()
+        assert np.abs(out["v"] - vr).max() <= 5e-5 * vmax + 1e-7, musl
+        e0, e1 = (vr.astype(np.float64) ** 2).sum(), (out["v"].astype(np.float64) ** 2).sum()
+        assert abs(e1 - e0) <= 1e-4 * e0, musl
+        FtF = np.einsum("nji,njk->nik", out["F"].astype(np.float64), out["F"].astype(np.float64))
+        assert np.abs(FtF - np.eye(3)).max() <= 1e-6, musl
