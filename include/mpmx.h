@@ -535,6 +535,54 @@ int mpmx_sample_level_sets(const mpmx_level_set* objects, int32_t n_objects, flo
                            uint32_t seed, float ppc, float* xyz, int32_t* object, int64_t cap, int64_t* n_out,
                            float* dx_out);
 
+/* ---- the paint app's blob scene (host code).
+ *
+ * One model of the blob database (simulation.hpp:272-279): its level set as a
+ * dense grid (offset and scale are ignored; the models share one voxel size,
+ * with origins on that voxel lattice), its volume (m^3, ascending over the
+ * database) and its calibration factor. */
+typedef struct mpmx_blob_model {
+  mpmx_level_set shape;
+  float volume;
+  float calibration;
+} mpmx_blob_model;
+
+/* Where mpmx_blob_scene put blob i: the database pair (pair, pair + 1) it
+ * morphs and the blend t, the target volume, the packing radius, the offset
+ * applied to the morphed level set and its placed bounding box. */
+typedef struct mpmx_blob_placement {
+  int32_t pair;
+  float t;
+  float target_volume;
+  float radius;
+  float offset[3];
+  float bbox_min[3], bbox_max[3];
+} mpmx_blob_placement;
+
+/* Simulation::blobs (simulation_blobs.cpp:7-150): per colour ratio (clamped to
+ * [0.1, 0.9]) a target volume between the database's smallest and largest,
+ * the bracketing pair (getMorphPair, :152-163), t by 15 bisection steps on the
+ * voxel-counted volume times the blended calibration factor
+ * (findTForExactVolume, :165-218), the models blended as (1 - t) A + t B over
+ * the union of their extents (values outside a model's extent: 3 voxels,
+ * positive); the blobs placed touching in x-z (n = 1..4 by the reference's
+ * closed forms, a row beyond; packing radius = half the wider x-z extent -
+ * 0.055) and centred; then sampled as mpmx_sample_level_sets does (the
+ * reference calls it with radius 0.01, 30 attempts, seed 42, ppc 27).
+ * object[p] = the blob (colour) index of particle p.  placement (n entries)
+ * may be NULL.  Two-call protocol on cap as mpmx_sample_level_sets. */
+int mpmx_blob_scene(const mpmx_blob_model* db, int32_t n_db, const float* ratios, int32_t n, float radius,
+                    int32_t attempts, uint32_t seed, float ppc, float* xyz, int32_t* object, int64_t cap,
+                    int64_t* n_out, float* dx_out, mpmx_blob_placement* placement);
+
+/* ObjectSpatula::loadAnimation (object_spatula.hpp:235-250): the animation
+ * file is an array of FrameData {float pos[3]; float quat[4] (x, y, z, w)}
+ * (object_spatula.hpp:7-10), 28 bytes each, played at 60 fps; frames7
+ * receives them as the 7-float keyframes mpmx_set_knife_keyframes takes.
+ * *n_out = whole records in the file; MPMX_ERR_CAPACITY (nothing written)
+ * when more than cap; MPMX_ERR_IO when the file cannot be read. */
+int mpmx_load_keyframes(const char* path, float* frames7, int64_t cap, int64_t* n_out);
+
 #ifdef __cplusplus
 }
 #endif

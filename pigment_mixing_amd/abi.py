@@ -32,6 +32,17 @@ class LevelSet(ctypes.Structure):
                 ("origin", ctypes.c_float * 3), ("offset", ctypes.c_float * 3), ("scale", ctypes.c_float)]
 
 
+class BlobModel(ctypes.Structure):
+    """mpmx_blob_model: one model of the blob database."""
+    _fields_ = [("shape", LevelSet), ("volume", ctypes.c_float), ("calibration", ctypes.c_float)]
+
+
+class BlobPlacement(ctypes.Structure):
+    _fields_ = [("pair", ctypes.c_int32), ("t", ctypes.c_float), ("target_volume", ctypes.c_float),
+                ("radius", ctypes.c_float), ("offset", ctypes.c_float * 3), ("bbox_min", ctypes.c_float * 3),
+                ("bbox_max", ctypes.c_float * 3)]
+
+
 class Config(ctypes.Structure):
     _fields_ = [
         ("E", ctypes.c_float), ("nu", ctypes.c_float), ("rho", ctypes.c_float),
@@ -192,6 +203,11 @@ def _sig(lib):
         ("mpmx_boundary_eval", ctypes.c_int, [ctypes.POINTER(Config), ctypes.POINTER(Plate), ctypes.c_int,
                                               ctypes.POINTER(Knife), ctypes.c_int64, c_float_p, c_float_p,
                                               ctypes.c_int64, c_float_p]),
+        ("mpmx_blob_scene", ctypes.c_int,
+         [ctypes.POINTER(BlobModel), ctypes.c_int32, c_float_p, ctypes.c_int32, ctypes.c_float, ctypes.c_int32,
+          ctypes.c_uint32, ctypes.c_float, c_float_p, ctypes.POINTER(ctypes.c_int32), ctypes.c_int64, c_i64_p,
+          c_float_p, ctypes.POINTER(BlobPlacement)]),
+        ("mpmx_load_keyframes", ctypes.c_int, [ctypes.c_char_p, c_float_p, ctypes.c_int64, c_i64_p]),
         ("mpmx_mlp_forward", ctypes.c_int, [c_float_p, c_float_p, c_float_p, ctypes.c_int64, c_float_p]),
         ("mpmx_mlp_forward_noalloc", ctypes.c_int, [c_float_p, c_float_p, c_float_p, ctypes.c_int64, c_float_p]),
         ("mpmx_mlp_gradient", ctypes.c_int,
@@ -288,6 +304,53 @@ def boundary_eval(cfg, X, v, plates=(), knife=None, step=0):
     if rc:
         raise MpmxError(rc, "mpmx_boundary_eval")
     return out
+
+
+def blob_scene(models, ratios, radius=0.01, attempts=30, seed=42, ppc=27.0):
+    """The paint app's blob scene (Simulation::blobs): models are (phi[nx, ny, nz],
+    voxel, origin3, volume, calibration) tuples of the blob database (ascending
+    volume); returns (x (N, 3), blob index (N,), dx, placements)."""
+    lib = load()
+    keep, db = [], (BlobModel * len(models))()
+    for i, (phi, voxel, origin, volume, calib) in enumerate(models):
+        a = np.ascontiguousarray(phi, np.float32)
+        keep.append(a)
+        db[i].shape.phi = fptr(a)
+        db[i].shape.dims[:] = list(a.shape)
+        db[i].shape.voxel = voxel
+        db[i].shape.origin[:] = [float(v) for v in origin]
+        db[i].shape.offset[:] = [0.0, 0.0, 0.0]
+        db[i].shape.scale = 1.0
+        db[i].volume = volume
+        db[i].calibration = calib
+    r = _f32(ratios)
+    place = (BlobPlacement * len(r))()
+    n, dx = ctypes.c_int64(0), ctypes.c_float(0.0)
+    args = (db, len(models), fptr(r), len(r), radius, attempts, seed, ppc)
+    rc = lib.mpmx_blob_scene(*args, None, None, 0, ctypes.byref(n), ctypes.byref(dx), place)
+    if rc not in (0, ERR_CAPACITY):
+        raise MpmxError(rc, "mpmx_blob_scene")
+    x = np.zeros((n.value, 3), np.float32)
+    obj = np.zeros(n.value, np.int32)
+    rc = lib.mpmx_blob_scene(*args, fptr(x), obj.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n.value,
+                             ctypes.byref(n), ctypes.byref(dx), place)
+    if rc:
+        raise MpmxError(rc, "mpmx_blob_scene")
+    return x, obj, float(dx.value), [place[i] for i in range(len(r))]
+
+
+def load_keyframes(path):
+    """Frames {pos[3], quat_xyzw[4]} of a spatula animation file (n, 7)."""
+    lib = load()
+    n = ctypes.c_int64(0)
+    rc = lib.mpmx_load_keyframes(os.fsencode(path), None, 0, ctypes.byref(n))
+    if rc not in (0, ERR_CAPACITY):
+        raise MpmxError(rc, "mpmx_load_keyframes")
+    fr = np.zeros((n.value, 7), np.float32)
+    rc = lib.mpmx_load_keyframes(os.fsencode(path), fptr(fr), n.value, ctypes.byref(n))
+    if rc:
+        raise MpmxError(rc, "mpmx_load_keyframes")
+    return fr
 
 
 MLP_NOALLOC_MAX = 65536   # MPMX_MLP_NOALLOC_MAX

@@ -28,8 +28,8 @@ SOURCES = [
     "host/closure.cpp",
     "host/kinematics.cpp",
     "host/scene.cpp",
-    "host/batch.cpp",
     "host/sampling.cpp",
+    "host/blobs.cpp",
 ]
 
 ARCH = os.environ.get("MPMX_ARCH", "gfx950")
