@@ -19,6 +19,7 @@ OBJ = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libmpmx.so")
 DEMO = os.path.join(OBJ, "simulation_api_demo")
+ADAPTER_DEMO = os.path.join(OBJ, "adapter_style_demo")
 COPY_PROBE = os.path.join(OBJ, "hbm_copy")   # HBM-counter calibration (plain copy)
 
 SOURCES = [
@@ -88,10 +89,13 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
-    # C++ driver of the Simulation facade (used by a GPU test)
-    demo_src = os.path.join(REPO, "tests", "cpp", "simulation_api_demo.cpp")
-    if os.path.exists(demo_src):
-        cmd = ["g++", "-std=c++17", "-O2", f"-I{os.path.join(REPO, 'include')}", demo_src, "-o", DEMO,
+    # C++ drivers of the Simulation facade (used by GPU tests): the facade's own
+    # API, and a host written as the reference's renderer adapter is
+    for name, exe in (("simulation_api_demo.cpp", DEMO), ("adapter_style_demo.cpp", ADAPTER_DEMO)):
+        demo_src = os.path.join(REPO, "tests", "cpp", name)
+        if not os.path.exists(demo_src):
+            continue
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", f"-I{os.path.join(REPO, 'include')}", demo_src, "-o", exe,
                f"-L{LIBDIR}", "-lmpmx", f"-Wl,-rpath,{LIBDIR}", "-Wl,-rpath,$ORIGIN/../lib"]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:

@@ -3,14 +3,21 @@
 //
 // It keeps the lifecycle the renderer adapter drives
 // (raytracing/src/mpm/MPMIntegrationSim.cpp:16-116):
-//   initializeBasic(name) -> setupScene(...) -> prepareSimulation() ->
-//   loop { step(); if (frameFinished()) read particles; } until isFinished()
-// and the public state it reads (Np, dx, particles.x / pigments /
-// diffusion_factor / active, getGridBoundaries, getSpatulaObject).  Particle
-// arrays are a host mirror refreshed by syncParticles() (one device->host copy);
-// the device state is authoritative.  Header-only; link with libmpmx.so.
+//   initializeBasic(name) -> setupScene(fps, ratios, pigments, anim_path) ->
+//   prepareSimulation() -> loop { step(); if (frameFinished()) read particles; }
+//   until isFinished()
+// and the public state it reads: Np, dx, dim, particles.x[i](d) /
+// pigments[i](c) / diffusion_factor[i] / active[i], getGridBoundaries(),
+// getSpatulaObject()->invTransform.matrix()(r, c) and half sizes,
+// pigment_D_edge0 / pigment_D_edge1.  The vector types accept the element
+// access forms the adapter uses on the reference's Eigen types (v(i), M(r, c),
+// data()).  Particle arrays are a host mirror refreshed after every completed
+// frame and by syncParticles() (one device->host copy); the device state is
+// authoritative.  Header-only; link with libmpmx.so.  matter_compat.hpp puts
+// these names at global scope, where the reference declares them.
 #pragma once
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <memory>
@@ -23,9 +30,52 @@
 namespace matter_amd {
 
 typedef float T;
-typedef std::array<float, 3> TV;
-typedef std::array<float, 9> TM;          // row-major 3x3
-typedef std::array<float, 7> Latent;
+
+// 3-vector with the reference's TV access forms (Eigen: v(i), v[i], size()).
+struct TV {
+  float a[3] = {0.0f, 0.0f, 0.0f};
+  TV() = default;
+  TV(float x, float y, float z) : a{x, y, z} {}
+  float& operator()(int i) { return a[i]; }
+  float operator()(int i) const { return a[i]; }
+  float& operator[](int i) { return a[i]; }
+  float operator[](int i) const { return a[i]; }
+  int size() const { return 3; }
+  float* data() { return a; }
+  const float* data() const { return a; }
+  static TV Zero() { return TV(); }
+  static TV Ones() { return TV(1.0f, 1.0f, 1.0f); }
+  TV operator+(const TV& o) const { return TV(a[0] + o.a[0], a[1] + o.a[1], a[2] + o.a[2]); }
+  TV operator-(const TV& o) const { return TV(a[0] - o.a[0], a[1] - o.a[1], a[2] - o.a[2]); }
+  TV operator*(float s) const { return TV(a[0] * s, a[1] * s, a[2] * s); }
+};
+
+// 3x3 matrix, row-major: M(r, c) as the reference's TM, M[i] = flat element i.
+struct TM {
+  float a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  float& operator()(int r, int c) { return a[3 * r + c]; }
+  float operator()(int r, int c) const { return a[3 * r + c]; }
+  float& operator[](int i) { return a[i]; }
+  float operator[](int i) const { return a[i]; }
+  static TM Identity() {
+    TM m;
+    m.a[0] = m.a[4] = m.a[8] = 1.0f;
+    return m;
+  }
+};
+
+// The 7-float Mixbox latent (Eigen::Matrix<float, 7, 1> in the reference).
+struct Latent {
+  float a[7] = {0, 0, 0, 0, 0, 0, 0};
+  float& operator()(int i) { return a[i]; }
+  float operator()(int i) const { return a[i]; }
+  float& operator[](int i) { return a[i]; }
+  float operator[](int i) const { return a[i]; }
+  float* data() { return a; }
+  const float* data() const { return a; }
+  int size() const { return 7; }
+  static Latent Zero() { return Latent(); }
+};
 
 enum class BC { NoSlip = MPMX_BC_NOSLIP, SlipStick = MPMX_BC_SLIPSTICK, SlipFree = MPMX_BC_SLIPFREE };
 enum class PlateType { top = 0, bottom = 1, left = 2, right = 3, front = 4, back = 5 };
@@ -39,18 +89,20 @@ struct Particles {
   std::vector<T> eps_pl_dev, delta_gamma, diffusion_factor;
   std::vector<T> eps_pl_vol, eps_pl_vol_pradhana;
   std::vector<Latent> pigments;
+  Particles() = default;
+  explicit Particles(size_t n) { resize(n); }
   void resize(size_t n) {
     active.assign(n, 1);
-    x.assign(n, TV{0, 0, 0});
-    v.assign(n, TV{0, 0, 0});
-    F.assign(n, TM{1, 0, 0, 0, 1, 0, 0, 0, 1});
-    Bmat.assign(n, TM{});
+    x.assign(n, TV());
+    v.assign(n, TV());
+    F.assign(n, TM::Identity());
+    Bmat.assign(n, TM());
     eps_pl_dev.assign(n, 0);
     eps_pl_vol.assign(n, 0);
     eps_pl_vol_pradhana.assign(n, 0);
     delta_gamma.assign(n, 0);
     diffusion_factor.assign(n, 0);
-    pigments.assign(n, Latent{});
+    pigments.assign(n, Latent());
   }
 };
 
@@ -65,11 +117,104 @@ struct ObjectPlate {
   T vmin_factor = 1, load_factor = 0;   // object_plate.hpp:13
 };
 
-// Knife (ObjectSpatula's role, object_spatula.hpp): shape, BC and motion.
-struct ObjectKnife {
-  mpmx_knife spec;
-  mpmx_knife_state state;   // state of the current step (invTransform etc.)
+// A 4x4 affine matrix read as M(r, c) (Eigen's Transform::matrix()).
+struct Mat4 {
+  float a[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+  float operator()(int r, int c) const { return a[4 * r + c]; }
+  float& operator()(int r, int c) { return a[4 * r + c]; }
 };
+struct Transform3 {
+  Mat4 m;
+  const Mat4& matrix() const { return m; }
+  TV translation() const { return TV(m(0, 3), m(1, 3), m(2, 3)); }
+};
+
+// ObjectSpatula's FrameData record (object_spatula.hpp:7-10).
+struct FrameData {
+  float pos[3];
+  float quat[4];   // x, y, z, w
+};
+
+// The knife: ObjectSpatula's public state (object_spatula.hpp:252-270) for the
+// spatula of the paint app, or any mpmx_knife (slab or trapezoid, any motion).
+struct ObjectSpatula {
+  Transform3 transform, invTransform;   // of the current step
+  T halfWidth = 0.368f, halfLength = 0.736f, halfThickness = 0.07f, offset = 0.05f;   // :20-23
+  T vx_object = 0, vy_object = 0, vz_object = 0;
+  TV angularVelocity;
+  std::vector<FrameData> animation_data;
+  std::string animation_path;
+  const float animation_fps = 60.0f;
+  BC bc = BC::SlipFree;
+  T friction = 0;
+  mpmx_knife spec{};        // shape, bc and motion handed to the library
+  mpmx_knife_state state{};
+
+  // Pose of the current step from the library's knife state (3x4 row-major).
+  void setState(const mpmx_knife_state& s) {
+    state = s;
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 4; ++c) {
+        transform.m(r, c) = s.T[4 * r + c];
+        invTransform.m(r, c) = s.Tinv[4 * r + c];
+      }
+    vx_object = s.v[0];
+    vy_object = s.v[1];
+    vz_object = s.v[2];
+    angularVelocity = TV(s.w[0], s.w[1], s.w[2]);
+  }
+};
+typedef ObjectSpatula ObjectKnife;
+
+// One model of the blob database (simulation.hpp:272-279): a dense
+// signed-distance grid (world point origin + (i, j, k) voxel), its volume and
+// calibration factor.  The reference's models are .vdb files; here the caller
+// hands in their densified grids, or keeps the default stand-ins.
+struct BlobModel {
+  std::string path;
+  std::vector<float> phi;
+  int dims[3] = {0, 0, 0};
+  float voxel = 0.03f;
+  float origin[3] = {0, 0, 0};
+  float volume = 0;
+  float calibration_factor = 1;
+};
+
+// Default database: spheres of the reference models' volumes (0.0056 ..
+// 0.053 m^3) resting on y = 0, on a 0.03 m lattice, each with the calibration
+// factor that makes its voxel-counted volume the nominal one.
+inline std::vector<BlobModel> default_blob_database() {
+  const float vols[6] = {0.0056f, 0.0151f, 0.0246f, 0.0341f, 0.0436f, 0.053f};
+  const float vx = 0.03f;
+  std::vector<BlobModel> db;
+  const float rmax = std::cbrt(3.0f * vols[5] / (4.0f * 3.14159265f));
+  const int m = (int)std::ceil((rmax + 4 * vx) / vx);
+  for (int b = 0; b < 6; ++b) {
+    BlobModel M;
+    M.path = "sphere_" + std::to_string(b + 1);
+    const float r = std::cbrt(3.0f * vols[b] / (4.0f * 3.14159265f));
+    const int ny = (int)std::ceil((2 * r + 4 * vx) / vx) + 2;
+    M.dims[0] = M.dims[2] = 2 * m + 1;
+    M.dims[1] = ny;
+    M.voxel = vx;
+    M.origin[0] = M.origin[2] = -m * vx;
+    M.origin[1] = -2 * vx;
+    M.phi.resize((size_t)M.dims[0] * M.dims[1] * M.dims[2]);
+    size_t count = 0, p = 0;
+    for (int i = 0; i < M.dims[0]; ++i)
+      for (int j = 0; j < M.dims[1]; ++j)
+        for (int k = 0; k < M.dims[2]; ++k) {
+          const float x = M.origin[0] + i * vx, y = M.origin[1] + j * vx - r, z = M.origin[2] + k * vx;
+          const float d = std::sqrt(x * x + y * y + z * z) - r;
+          M.phi[p++] = d;
+          count += d <= 0.0f;
+        }
+    M.volume = vols[b];
+    M.calibration_factor = vols[b] / ((float)count * vx * vx * vx);
+    db.push_back(std::move(M));
+  }
+  return db;
+}
 
 class Simulation {
  public:
@@ -81,19 +226,27 @@ class Simulation {
   Simulation& operator=(const Simulation&) = delete;
 
   // ---- public members (simulation.hpp:30-126)
+  const int dim = 3;
   int exit = 0;
   unsigned int end_frame = 10;
   unsigned int frame = 0;
   bool is_initialized = false;
   unsigned int Np = 0;
   T dx = 0.01f;
+  T dt = 0;                       // fixed step; 0: chosen by prepareSimulation
+  T fps = 60;
   T particle_mass = 0, particle_volume = 0;
+  T pigment_D_max = 0, pigment_D_edge0 = 0, pigment_D_edge1 = 0;      // pigment_config.json
+  T start_boost_time = 0, end_boost_time = 0, boost_factor = 1;
   mpmx_config cfg;
   Particles particles;
   std::vector<ObjectPlate> plates;
-  std::unique_ptr<ObjectKnife> knife;
+  std::unique_ptr<ObjectSpatula> knife;
+  std::vector<BlobModel> blobDatabase = default_blob_database();
   std::string sim_name;
-  std::string directory = "output/";   // simulation.hpp:205
+  std::string directory = "output/";                                   // simulation.hpp:205
+  std::string sim_config_path = "../matter/sim_config.json";            // simulation_units.cpp:51
+  std::string pigment_config_path = "../matter/pigment_config.json";    // simulation_units.cpp:32
 
   // ---- lifecycle (simulation_units.cpp:9-245)
   void initializeBasic(const std::string& name) {
@@ -103,12 +256,100 @@ class Simulation {
 
   // Load the two JSON files the reference reads (simulation_units.cpp:32-80).
   int loadConfig(const std::string& sim_config_json, const std::string& pigment_config_json) {
-    return mpmx_config_load_json(&cfg, sim_config_json.c_str(), pigment_config_json.c_str());
+    const int rc = mpmx_config_load_json(&cfg, sim_config_json.c_str(), pigment_config_json.c_str());
+    pull_pigment_members();
+    return rc;
   }
 
-  // Paint scene: box blobs (one per latent) side by side along x, the floor
-  // (SlipStick, mu = 1; simulation_units.cpp:117) and a knife.
-  void setupScene(float fps_value, const std::vector<mpmx_box>& blobs, const ObjectKnife* knife_in) {
+  // Simulation::setupScene (simulation_units.cpp:28-131): the two JSON files,
+  // the paint app's settings, the blobs of the colour ratios
+  // (Simulation::blobs, mpmx_blob_scene: radius 0.01, ppc 27, seed 42), the
+  // floor (SlipStick, mu 1), the spatula (SlipFree, mu 0) animated by the
+  // file at spatula_anim_path, and the Drucker-Prager/Perzyna material with
+  // Pradhana's correction.  Pigments may be any 7-vector type with v(c)
+  // (Eigen::Matrix<float, 7, 1> as the adapter builds them, or Latent).
+  template <class Pigment>
+  void setupScene(float fps_value, const std::vector<float>& colorRatios, const std::vector<Pigment>& pigments,
+                  const std::string& spatula_anim_path) {
+    loadConfig(sim_config_path, pigment_config_path);
+    end_frame = 1800;
+    fps = fps_value;
+    cfg.fps = fps_value;
+    cfg.gravity[0] = 0.0f;
+    cfg.gravity[1] = -9.81f;
+    cfg.gravity[2] = 0.0f;
+    cfg.use_particle_boundaries = 1;
+    for (int d = 0; d < 3; ++d) {
+      cfg.particle_boundary_min[d] = -2.0f;
+      cfg.particle_boundary_max[d] = 2.0f;
+      cfg.grid_reference_point[d] = 0.0f;
+    }
+    // blobs (simulation_blobs.cpp:7-150)
+    std::vector<mpmx_blob_model> db(blobDatabase.size());
+    for (size_t i = 0; i < db.size(); ++i) {
+      const BlobModel& M = blobDatabase[i];
+      mpmx_level_set& L = db[i].shape;
+      L.phi = M.phi.data();
+      for (int d = 0; d < 3; ++d) {
+        L.dims[d] = M.dims[d];
+        L.origin[d] = M.origin[d];
+        L.offset[d] = 0.0f;
+      }
+      L.voxel = M.voxel;
+      L.scale = 1.0f;
+      db[i].volume = M.volume;
+      db[i].calibration = M.calibration_factor;
+    }
+    const float ppc = 27.0f;
+    int64_t n = 0;
+    float dxs = 0;
+    const int rc0 = mpmx_blob_scene(db.data(), (int)db.size(), colorRatios.data(), (int)colorRatios.size(), 0.01f, 30,
+                                    42, ppc, nullptr, nullptr, 0, &n, &dxs, nullptr);
+    if (rc0 != MPMX_ERR_CAPACITY) check(rc0 == MPMX_OK ? MPMX_ERR_ARG : rc0, "blobs (no particles)");
+    std::vector<float> x(3 * n);
+    std::vector<int32_t> obj(n);
+    check(mpmx_blob_scene(db.data(), (int)db.size(), colorRatios.data(), (int)colorRatios.size(), 0.01f, 30, 42, ppc,
+                          x.data(), obj.data(), n, &n, &dxs, nullptr),
+          "blobs");
+    Np = (unsigned)n;
+    dx = dxs;
+    cfg.ppc = ppc;
+    particles.resize(Np);
+    for (unsigned p = 0; p < Np; ++p) {
+      particles.x[p] = TV(x[3 * p], x[3 * p + 1], x[3 * p + 2]);
+      const auto& pg = pigments[(size_t)obj[p] % pigments.size()];
+      for (int c = 0; c < 7; ++c) particles.pigments[p][c] = pg(c);
+    }
+    // floor and spatula (simulation_units.cpp:116-123)
+    plates.clear();
+    plates.push_back(ObjectPlate{0.0f, PlateType::bottom, BC::SlipStick, 1.0f});
+    knife.reset(new ObjectSpatula());
+    knife->animation_path = spatula_anim_path;
+    int64_t nf = 0;
+    if (mpmx_load_keyframes(spatula_anim_path.c_str(), nullptr, 0, &nf) == MPMX_ERR_CAPACITY && nf > 0) {
+      knife->animation_data.resize((size_t)nf);
+      check(mpmx_load_keyframes(spatula_anim_path.c_str(), knife->animation_data[0].pos, nf, &nf), "animation");
+    }
+    mpmx_knife& k = knife->spec;
+    k = mpmx_knife{};
+    k.shape = MPMX_KNIFE_SPATULA;
+    k.half[0] = knife->halfWidth;
+    k.half[1] = knife->halfThickness;
+    k.half[2] = knife->halfLength;
+    k.offset = knife->offset;
+    k.bc = (int)BC::SlipFree;
+    k.friction = 0.0f;
+    k.motion = knife->animation_data.empty() ? MPMX_MOTION_STATIC : MPMX_MOTION_KEYFRAMES;
+    // material (simulation_units.cpp:126-130)
+    cfg.plastic_model = MPMX_PLASTIC_DPVISC;
+    cfg.use_pradhana = 1;
+    cfg.q_prefac = 1.0f / std::sqrt(2.0f);
+  }
+
+  // The box-blob scene of the benchmarks: box blobs (one per latent) with
+  // seeded jitter, the floor (SlipStick, mu = 1) and an optional knife.
+  void setupScene(float fps_value, const std::vector<mpmx_box>& blobs, const ObjectSpatula* knife_in) {
+    fps = fps_value;
     cfg.fps = fps_value;
     cfg.dx = dx;
     int64_t n = 0;
@@ -118,26 +359,36 @@ class Simulation {
     Np = (unsigned)n;
     particles.resize(Np);
     for (unsigned p = 0; p < Np; ++p) {
-      particles.x[p] = TV{x[3 * p], x[3 * p + 1], x[3 * p + 2]};
+      particles.x[p] = TV(x[3 * p], x[3 * p + 1], x[3 * p + 2]);
       for (int c = 0; c < 7; ++c) particles.pigments[p][c] = pig[7 * p + c];
     }
     plates.clear();
     plates.push_back(ObjectPlate{0.0f, PlateType::bottom, BC::SlipStick, 1.0f});
-    if (knife_in) knife.reset(new ObjectKnife(*knife_in));
+    if (knife_in) knife.reset(new ObjectSpatula(*knife_in));
+    pull_pigment_members();
   }
 
   // Simulation::prepareSimulation (simulation_units.cpp:133-215): derived
-  // constants, lattice, first scatter — here on the device.
+  // constants, MUSL on (:186), lattice, first scatter — here on the device.
+  // A step dt of 0 takes the reference's elastic bound dx / sqrt(E / rho)
+  // (update_dt.cpp:5-71 at cfl_elastic 1), lowered to cfl dx / v where v is
+  // the knife's fastest speed in its animation; the step stays fixed.
   void prepareSimulation() {
     if (!is_initialized) throw std::runtime_error("Simulation not initialized");
     cfg.dx = dx;
+    cfg.fps = fps;
+    cfg.use_musl = 1;
+    push_pigment_members();
+    if (!(dt > 0)) dt = default_dt();
+    cfg.dt = dt;
+    if (Np > 0 && cfg.use_particle_boundaries && cfg.grid_dims[0] == 0) lattice_from_boundaries();
     if (h_) mpmx_destroy(h_);
     h_ = nullptr;
     check(mpmx_create(&cfg, &h_), "create");
     upload();
     std::vector<mpmx_plate> pl;
     for (const ObjectPlate& p : plates) {
-      mpmx_plate q;
+      mpmx_plate q{};
       q.pos_object = p.pos_object;
       q.plate_type = (int)p.plate_type;
       q.bc = (int)p.bc;
@@ -151,6 +402,10 @@ class Simulation {
     }
     check(mpmx_set_objects(h_, pl.data(), (int)pl.size(), knife ? &knife->spec : nullptr, knife ? 1 : 0),
           "set_objects");
+    if (knife && knife->spec.motion == MPMX_MOTION_KEYFRAMES)
+      check(mpmx_set_knife_keyframes(h_, 0, knife->animation_data[0].pos, (int)knife->animation_data.size(),
+                                     knife->animation_fps),
+            "keyframes");
     check(mpmx_prepare(h_), "prepare");
     particle_volume = dx * dx * dx / cfg.ppc;
     particle_mass = cfg.rho * particle_volume;
@@ -163,6 +418,11 @@ class Simulation {
     }
     frame = 0;
     time_ = 0;
+    if (knife) {
+      mpmx_knife_state s;
+      check(mpmx_knife_state_at(h_, 0, 0, &s), "knife state");
+      knife->setState(s);
+    }
   }
 
   // Simulation::step (simulation_units.cpp:217-233): one advanceStep on the device.
@@ -171,13 +431,23 @@ class Simulation {
     if (rc == MPMX_ERR_DEVICE) exit = 1;  // reference exit flag
     else check(rc, "step");
     time_ += cfg.dt;
-    if (knife) mpmx_knife_state_at(h_, 0, stepsDone(), &knife->state);
+    if (knife) {
+      mpmx_knife_state s;
+      check(mpmx_knife_state_at(h_, 0, stepsDone(), &s), "knife state");
+      knife->setState(s);
+    }
   }
 
+  // Simulation::frameFinished (simulation_units.cpp:235-241); the host mirror
+  // of the particles is refreshed at every finished frame, as the adapter
+  // reads it then (MPMIntegrationSim.cpp:40-42, 100-116).
   bool frameFinished() {
     int f = 0;
     check(mpmx_frame_finished(h_, &f), "frame_finished");
-    if (f) frame++;
+    if (f) {
+      frame++;
+      syncParticles();
+    }
     return f != 0;
   }
   bool isFinished() const { return frame >= end_frame; }
@@ -231,7 +501,7 @@ class Simulation {
   std::pair<std::vector<T>, std::vector<T>> getGridBoundaries() const {
     return {std::vector<T>{low_[0], low_[1], low_[2]}, std::vector<T>{high_[0], high_[1], high_[2]}};
   }
-  const ObjectKnife* getSpatulaObject() const { return knife.get(); }
+  ObjectSpatula* getSpatulaObject() const { return knife.get(); }
   mpmx_handle* handle() const { return h_; }
   double time() const { return time_; }
 
@@ -240,6 +510,53 @@ class Simulation {
   T low_[3] = {0, 0, 0}, high_[3] = {0, 0, 0};
   double time_ = 0;
 
+  void pull_pigment_members() {
+    pigment_D_max = cfg.D_max;
+    pigment_D_edge0 = cfg.D_edge0;
+    pigment_D_edge1 = cfg.D_edge1;
+    start_boost_time = cfg.start_boost_time;
+    end_boost_time = cfg.end_boost_time;
+    boost_factor = cfg.boost_factor;
+  }
+  void push_pigment_members() {
+    cfg.D_max = pigment_D_max;
+    cfg.D_edge0 = pigment_D_edge0;
+    cfg.D_edge1 = pigment_D_edge1;
+    cfg.start_boost_time = start_boost_time;
+    cfg.end_boost_time = end_boost_time;
+    cfg.boost_factor = boost_factor;
+  }
+  T default_dt() const {
+    T d = dx / std::sqrt(cfg.E / cfg.rho);
+    if (knife && knife->animation_data.size() > 1) {
+      float vmax = 0;
+      for (size_t i = 1; i < knife->animation_data.size(); ++i) {
+        const float* a = knife->animation_data[i - 1].pos;
+        const float* b = knife->animation_data[i].pos;
+        const float s = std::sqrt((b[0] - a[0]) * (b[0] - a[0]) + (b[1] - a[1]) * (b[1] - a[1]) +
+                                  (b[2] - a[2]) * (b[2] - a[2])) * knife->animation_fps;
+        vmax = std::max(vmax, s);
+      }
+      if (vmax > 1e-10f && cfg.cfl > 0) d = std::min(d, cfg.cfl * dx / vmax);
+    }
+    return d;
+  }
+  // A lattice on the reference's node positions (particle bbox minimum - 2 dx,
+  // remesh.cpp:114-137, shifted by whole dx) that reaches the particle
+  // boundaries: the reference re-meshes as the paint spreads; this lattice
+  // is fixed, large enough for wherever a live particle can go.
+  void lattice_from_boundaries() {
+    float lo[3] = {1e30f, 1e30f, 1e30f};
+    for (unsigned p = 0; p < Np; ++p)
+      for (int d = 0; d < 3; ++d) lo[d] = std::min(lo[d], particles.x[p][d]);
+    cfg.origin_mode = 1;
+    for (int d = 0; d < 3; ++d) {
+      const float o = lo[d] - 2 * dx;
+      const int below = std::max(0, (int)std::ceil((o - (cfg.particle_boundary_min[d] - 2 * dx)) / dx));
+      cfg.grid_origin[d] = o - below * dx;
+      cfg.grid_dims[d] = (int)std::ceil((cfg.particle_boundary_max[d] + 2 * dx - cfg.grid_origin[d]) / dx) + 1;
+    }
+  }
   int64_t stepsDone() const {
     mpmx_stats s;
     mpmx_stats_get(h_, &s);

@@ -18,7 +18,7 @@ int main(int argc, char** argv) {
   // BASELINE material on top of the reference's files
   sim.cfg.E = 1e5f; sim.cfg.nu = 0.3f; sim.cfg.rho = 1500.0f; sim.cfg.flip_ratio = -0.5f;
   sim.cfg.plastic_model = MPMX_PLASTIC_VMVISC; sim.cfg.q_max = sim.cfg.q_min = 50.0f;
-  sim.cfg.perzyna_visc = 0.2f; sim.cfg.gravity[1] = -9.81f; sim.cfg.dt = 1e-5f;
+  sim.cfg.perzyna_visc = 0.2f; sim.cfg.gravity[1] = -9.81f; sim.dt = 1e-5f;
   sim.cfg.grid_dims[0] = 32; sim.cfg.grid_dims[1] = 16; sim.cfg.grid_dims[2] = 24;
   sim.dx = 3.125e-3f;
   std::vector<mpmx_box> blobs(2);
