@@ -7,17 +7,25 @@ Perzyna (yield 50 Pa, viscosity 10 Pa s), slab knife on a 40 mm circle at
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 
-Prints ONE JSON line.  Two runs of the same scene from the same initial state:
+Prints ONE JSON line.  Each material is run twice from the same developed
+state: an untimed pre-roll of the plain step (--preroll steps, 300 by default,
+by which the blobs have settled into plastic flow) is saved as a checkpoint and
+both timed runs start from it:
   plain   : exact von Mises/Perzyna return map;
   learned : the learned closure (BF16 MLP on MFMA, refined mode: one Newton step
             from the predicted plastic increment, accepted when the yield
             residual is within 1e-4 q_y, else the exact map), its network
-            trained on the GPU on pairs harvested from the plain run.
+            trained on the GPU on the pairs harvested in the pre-roll's last
+            --harvest-steps steps.
+The benchmark's material is the closed-form Perzyna map (perzyna_exp 1); a
+second pair of runs (newton_material) uses perzyna_exp 2, whose exact map
+iterates Newton's method.
 `value` is the learned run's live particles x K / wall seconds of K steps
 (graph-launched, inputs resident in HBM, synchronised before and after); the
-plain run's figure is under `plain`.  `learned` reports the held-out principal
-Kirchhoff-stress error of accepted predictions against the exact map (bar
-1e-3) and the acceptance / fallback rates.
+plain run's figure is under `plain`.  `learned` reports the queued candidates
+per step, the held-out principal Kirchhoff-stress error of accepted
+predictions against the exact map (bar 1e-3) and the acceptance / fallback
+rates.
 `roofline` is for the dominant kernel of the learned step (by time), its mean
 duration measured with HIP events on the library's stream over a second run of
 K steps; every kernel's figure is listed under roofline.kernels.
@@ -218,6 +226,104 @@ def principal_tau(hp, E, nu):
     return 2 * mu * hp + lam * hp.sum(1, keepdims=True)
 
 
+def checkpoint_with_closure(src, dst, mode, tol):
+    """Copy of a checkpoint file whose stored configuration selects the
+    learned closure: the file holds "MPMXCKP2", u32 sizeof(mpmx_config), then
+    the mpmx_config (include/mpmx.h), whose closure fields are rewritten."""
+    import struct
+    from pigment_mixing_amd import abi
+    data = bytearray(open(src, "rb").read())
+    assert data[:8] == b"MPMXCKP2" and struct.unpack_from("<I", data, 8)[0] == __import__("ctypes").sizeof(abi.Config)
+    struct.pack_into("<i", data, 12 + abi.Config.closure.offset, mode)
+    struct.pack_into("<f", data, 12 + abi.Config.closure_tol.offset, tol)
+    with open(dst, "wb") as f:
+        f.write(data)
+
+
+def plain_vs_learned(scene, args, label, cap, tmpdir):
+    """One material on the 1M-particle scene.  An untimed pre-roll of the plain
+    step develops the plastic flow (the last `harvest_steps` of it harvest the
+    closure's training pairs) and is saved as a checkpoint; the plain run
+    continues from it, the learned run is that checkpoint reloaded with the
+    learned closure selected, so both are timed from the same developed state."""
+    from pigment_mixing_amd import abi, scenes
+    cfg = scene["cfg"]
+    h = scenes.make_handle(scene)
+    h.step(args.preroll - args.harvest_steps)
+    h.harvest_begin(cap)
+    h.step(args.harvest_steps)
+    rec = h.harvest_end(cap)
+    ck = os.path.join(tmpdir, f"preroll_{label}.ckpt")
+    ckl = os.path.join(tmpdir, f"preroll_{label}_learned.ckpt")
+    h.save_checkpoint(ck)
+    pre = h.stats()
+    h.close()
+    print(f"[bench] {label}: pre-roll {args.preroll} steps, {pre.plastic_count} plastic particles in its last step, "
+          f"{rec.shape[0]} pairs harvested over its last {args.harvest_steps}", file=sys.stderr, flush=True)
+    # ---- plain, from the pre-rolled state
+    hp = abi.Handle.load_checkpoint(ck)
+    wall_p, st_p, shape_p = timed_run(hp, args.steps, args.warmup)
+    value_p = st_p.n_active * args.steps / wall_p
+    newton_p = st_p.newton_iters
+    kt_p = hp.profile(args.steps)
+    hp.close()
+    # ---- training on the GPU
+    rng = np.random.default_rng(0)
+    order = rng.permutation(rec.shape[0])
+    n_hold = max(1, min(rec.shape[0] // 10, 1_000_000))
+    hold, train = rec[order[:n_hold]], rec[order[n_hold:]]
+    del rec
+    checkpoint_with_closure(ck, ckl, 2, 1e-4)        # MPMX_CLOSURE_LEARNED_REFINED
+    os.remove(ck)
+    hl = abi.Handle.load_checkpoint(ckl)
+    os.remove(ckl)
+    rep, _ = hl.closure_train(train, steps=args.train_steps, batch=8192, lr=2e-3, lr_final=5e-5, seed=7,
+                              report_every=500)
+    print(f"[bench] {label}: trained {rep.steps} steps on {train.shape[0]} pairs in {rep.ms:.0f} ms, loss "
+          f"{rep.loss_first:.3e} -> {rep.loss_full:.3e}", file=sys.stderr, flush=True)
+    trial = np.concatenate([hold[:, 0:3], np.zeros((hold.shape[0], 1), np.float32)], axis=1)
+    lrn, ex, acc = hl.closure_eval(trial)
+    te = principal_tau(ex[:, 0:3].astype(np.float64), cfg.E, cfg.nu)
+    err_max = err_med = None
+    if acc.any():
+        tl = principal_tau(lrn[acc, 0:3].astype(np.float64), cfg.E, cfg.nu)
+        rel = np.linalg.norm(tl - te[acc], axis=1) / np.linalg.norm(te[acc], axis=1)
+        err_max, err_med = float(rel.max()), float(np.median(rel))
+    # ---- learned, from the same pre-rolled state
+    wall, st_last, shape_l = timed_run(hl, args.steps, args.warmup)
+    value = st_last.n_active * args.steps / wall
+    newton_l = st_last.newton_iters
+    kt = hl.profile(args.steps)
+    hl.close()
+    print(f"[bench] {label}: plain {value_p:.4g}, learned {value:.4g} particle-steps/s ({value / value_p:.3f}x); "
+          f"last step plastic {st_last.plastic_count} accepted {st_last.learned_count} fallback "
+          f"{st_last.fallback_count}; held-out stress err max {err_max}; kernels ms plain g2p={kt_p.ms_g2p:.4f} "
+          f"p2g={kt_p.ms_p2g:.4f} grid={kt_p.ms_grid:.4f} sort={kt_p.ms_sort:.4f}, learned g2p={kt.ms_g2p:.4f} "
+          f"closure={kt.ms_closure:.4f} ({kt.queued_per_step:.0f} queued)", file=sys.stderr, flush=True)
+    return {
+        "plain": {"value": value_p, "ms_per_step": wall_p / args.steps * 1e3, "wg_per_cu": shape_p,
+                  "newton_iters_per_step": newton_p / max(1, args.steps),
+                  "roofline": kernel_report(kt_p, learned=False, traffic=True)},
+        "learned": {
+            "value": value, "ms_per_step": wall / args.steps * 1e3,
+            "mode": "refined", "closure_tol": 1e-4, "wg_per_cu": shape_l,
+            "speedup_vs_plain": value / value_p,
+            "newton_iters_per_step": newton_l / max(1, args.steps),
+            "candidates_per_step": kt.queued_per_step,
+            "last_step": {"plastic": st_last.plastic_count, "accepted": st_last.learned_count,
+                          "fallback": st_last.fallback_count},
+            "fallback_rate_last_step": st_last.fallback_count / max(1, st_last.plastic_count),
+            "held_out": {"pairs": int(hold.shape[0]), "accepted_frac": float(acc.mean()),
+                         "stress_rel_err_max": err_max, "stress_rel_err_median": err_med, "bar": 1e-3},
+            "training": {"pairs": int(train.shape[0]), "steps": int(rep.steps), "batch": 8192,
+                         "ms": rep.ms, "loss_first": rep.loss_first, "loss_full": rep.loss_full},
+            "roofline": kernel_report(kt, learned=True, traffic=True),
+        },
+        "preroll_steps": args.preroll,
+        "plastic_at_preroll_end": pre.plastic_count,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -225,8 +331,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=5)
-    ap.add_argument("--harvest-steps", type=int, default=40)
-    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--harvest-steps", type=int, default=60)
+    ap.add_argument("--harvest-cap", type=int, default=50_000_000)
+    ap.add_argument("--preroll", type=int, default=300,
+                    help="untimed plain steps before both timed runs (plastic flow developed)")
+    ap.add_argument("--no-newton", action="store_true", help="skip the perzyna_exp = 2 material")
+    ap.add_argument("--train-steps", type=int, default=6000)
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched run")
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--no-large", action="store_true", help="skip the 8M-particle six-blob scene")
@@ -245,62 +355,24 @@ def main():
         print(json.dumps({"large": run_large(args, 6_000_000)}), flush=True)
         return
 
+    import tempfile
+    tmpdir = tempfile.mkdtemp(prefix="mpmx_bench_")
     scene = scenes.paint_4blob()
     n = scene["x"].shape[0]
     cfg = scene["cfg"]
     print(f"[bench] scene {scene['name']} particles={n}", file=sys.stderr, flush=True)
-
-    # ---- plain (exact return map)
-    h = scenes.make_handle(scene)
-    wall_p, st_p, shape_p = timed_run(h, args.steps, args.warmup)
-    value_p = st_p.n_active * args.steps / wall_p
-    kt_p = h.profile(args.steps)
-    rf_p = kernel_report(kt_p, learned=False, traffic=True)
-    print(f"[bench] plain: {args.steps} steps in {wall_p:.3f}s -> {value_p:.4g} particle-steps/s; kernels ms "
-          f"g2p={kt_p.ms_g2p:.4f} sort={kt_p.ms_sort:.4f} p2g={kt_p.ms_p2g:.4f} grid={kt_p.ms_grid:.4f}",
-          file=sys.stderr, flush=True)
-
-    # ---- harvest (trial, corrected) pairs from the plain run, train on the GPU
-    cap = 6_000_000
-    h.harvest_begin(cap)
-    h.step(args.harvest_steps)
-    rec = h.harvest_end(cap)
-    h.close()
-    rng = np.random.default_rng(0)
-    order = rng.permutation(rec.shape[0])
-    n_hold = max(1, rec.shape[0] // 10)
-    hold, train = rec[order[:n_hold]], rec[order[n_hold:]]
-    cfg.closure = 2          # MPMX_CLOSURE_LEARNED_REFINED
-    cfg.closure_tol = 1e-4
-    hl = scenes.make_handle(scene)
-    cfg.closure = 0
-    rep, _ = hl.closure_train(train, steps=args.train_steps, batch=8192, lr=2e-3, lr_final=5e-5, seed=7,
-                              report_every=500)
-    print(f"[bench] harvested {rec.shape[0]} pairs over {args.harvest_steps} steps; trained {rep.steps} steps "
-          f"in {rep.ms:.0f} ms, loss {rep.loss_first:.3e} -> {rep.loss_full:.3e}", file=sys.stderr, flush=True)
-    trial = np.concatenate([hold[:, 0:3], np.zeros((hold.shape[0], 1), np.float32)], axis=1)
-    lrn, ex, acc = hl.closure_eval(trial)
-    te = principal_tau(ex[:, 0:3].astype(np.float64), cfg.E, cfg.nu)
-    if acc.any():
-        tl = principal_tau(lrn[acc, 0:3].astype(np.float64), cfg.E, cfg.nu)
-        rel = np.linalg.norm(tl - te[acc], axis=1) / np.linalg.norm(te[acc], axis=1)
-        err_max, err_med = float(rel.max()), float(np.median(rel))
-    else:
-        err_max = err_med = None
-
-    # ---- learned closure run (same scene, same steps)
-    wall, st, shape_l = timed_run(hl, args.steps, args.warmup)
-    value = st.n_active * args.steps / wall
-    print(f"[bench] learned: {args.steps} steps in {wall:.3f}s -> {value:.4g} particle-steps/s; last step plastic "
-          f"{st.plastic_count} accepted {st.learned_count} fallback {st.fallback_count}; held-out stress err max "
-          f"{err_max}", file=sys.stderr, flush=True)
-    kt = hl.profile(args.steps)
-    rf = kernel_report(kt, learned=True, traffic=True)
-    print(f"[bench] learned kernels ms: g2p={kt.ms_g2p:.4f} closure={kt.ms_closure:.4f} "
-          f"({kt.queued_per_step:.0f} queued) sort={kt.ms_sort:.4f} p2g={kt.ms_p2g:.4f} "
-          f"grid={kt.ms_grid:.4f} step={kt.ms_step:.4f} active_nodes={kt.active_nodes_per_step:.0f}",
-          file=sys.stderr, flush=True)
-    st_last = hl.stats()
+    cap = args.harvest_cap
+    # the benchmark's material (perzyna_exp 1: the closed-form map, plasticity.cpp:187-193)
+    vm = plain_vs_learned(scene, args, "vmvisc_n1", cap, tmpdir)
+    # the Newton material (perzyna_exp 2: plasticity.cpp:194-230), where the exact map iterates
+    newton = None
+    if not args.no_newton:
+        sn = scenes.paint_4blob()
+        sn["cfg"].perzyna_exp = 2.0
+        newton = plain_vs_learned(sn, args, "vmvisc_n2", cap, tmpdir)
+    os.rmdir(tmpdir)
+    value = vm["learned"]["value"]
+    wall = vm["learned"]["ms_per_step"] * 1e-3 * args.steps
 
     # ---- 64 independent scenes in one handle (one graph per step pair), autotuned
     batch = None
@@ -324,7 +396,7 @@ def main():
 
     large = None
     if not args.no_large:
-        large = run_large(args, cap)
+        large = run_large(args, 6_000_000)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -351,8 +423,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded box blobs, Mixbox latents of palette colours); closure trained on pairs "
-                "harvested from the plain run of the same scene",
+        "data": "synthetic (seeded box blobs, Mixbox latents of palette colours); both runs timed after the same "
+                f"{args.preroll}-step plain pre-roll (checkpoint), closure trained on pairs harvested in its last "
+                f"{args.harvest_steps} steps",
         "config": {
             "workload": scene["name"],
             "particles": n,
@@ -363,19 +436,11 @@ def main():
             "knife": "slab 60x2x30 mm tilted 30 deg, circular sweep r=40 mm at 0.15 m/s",
             "parallelism": "single GPU",
         },
-        "plain": {"value": value_p, "ms_per_step": wall_p / args.steps * 1e3, "wg_per_cu": shape_p, "roofline": rf_p},
-        "learned": {
-            "mode": "refined", "closure_tol": 1e-4, "wg_per_cu": shape_l,
-            "speedup_vs_plain": value / value_p,
-            "last_step": {"plastic": st_last.plastic_count, "accepted": st_last.learned_count,
-                          "fallback": st_last.fallback_count},
-            "fallback_rate_last_step": st_last.fallback_count / max(1, st_last.plastic_count),
-            "held_out": {"pairs": int(hold.shape[0]), "accepted_frac": float(acc.mean()),
-                         "stress_rel_err_max": err_max, "stress_rel_err_median": err_med, "bar": 1e-3},
-            "training": {"pairs": int(train.shape[0]), "steps": int(rep.steps), "batch": 8192,
-                         "ms": rep.ms, "loss_first": rep.loss_first, "loss_full": rep.loss_full},
-        },
-        "roofline": rf,
+        "preroll_steps": args.preroll,
+        "plain": vm["plain"],
+        "learned": vm["learned"],
+        "newton_material": newton,
+        "roofline": vm["learned"]["roofline"],
         "batch": batch,
         "large": large,
         "cpu_baseline": cpu,
