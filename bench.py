@@ -67,10 +67,12 @@ def step_bytes(np_live, n_active):
     return 356.0 * np_live + 232.0 * n_active
 
 
-# Learned closure, per queued candidate: G2P writes a 48-byte record instead of
-# F, eps, dgamma and stress (68 bytes); k_closure reads the record and writes
-# those 68 bytes.  Network flops per candidate: 2 * (8*64 + 3*64*64 + 64*4).
-CLOSURE_BYTES = 48.0 + 68.0
+# Learned closure, per queued candidate: G2P writes a 96-byte record (slot,
+# scene, eps, trial F and its principal frame) instead of F, eps, dgamma and
+# stress (68 bytes); k_closure reads the record and writes those 68 bytes.
+# Network flops per candidate: 2 * (8*64 + 3*64*64 + 64*4).
+CLOSURE_REC = 96.0
+CLOSURE_BYTES = CLOSURE_REC + 68.0
 CLOSURE_FLOPS = 2.0 * (8 * 64 + 3 * 64 * 64 + 64 * 4)
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
@@ -146,10 +148,10 @@ def kernel_report(kt, learned=False, traffic=False):
     kb = kernel_bytes(kt.particles_per_step, n_act)
     kms = {"k_g2p": kt.ms_g2p, "k_p2g": kt.ms_p2g, "k_grid": kt.ms_grid}
     q = kt.queued_per_step if kt.ms_closure > 0 else 0.0
-    kb["k_g2p"] -= 20.0 * q
+    kb["k_g2p"] += (CLOSURE_REC - 68.0) * q
     kgbs = {k: kb[k] / (kms[k] * 1e-3) / 1e9 for k in kb}
     dom = max(kms, key=kms.get)
-    step_gbs = (step_bytes(kt.particles_per_step, n_act) + 96.0 * q) / (kt.ms_step * 1e-3) / 1e9
+    step_gbs = (step_bytes(kt.particles_per_step, n_act) + 2 * CLOSURE_REC * q) / (kt.ms_step * 1e-3) / 1e9
     closure = None
     if kt.ms_closure > 0:
         tfl = q * CLOSURE_FLOPS / (kt.ms_closure * 1e-3) / 1e12

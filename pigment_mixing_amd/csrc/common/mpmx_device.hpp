@@ -397,8 +397,8 @@ MPMX_HD uint32_t boundary_collision(const SceneDev& sc, const KnifeStateDev& ks,
   return 0u;
 }
 
-// Hardware reciprocal and square root on the device (v_rcp_f32, v_sqrt_f32,
-// about 1 ulp); IEEE forms on the host.
+// Hardware reciprocal, square root and reciprocal square root on the device
+// (v_rcp_f32, v_sqrt_f32, v_rsq_f32, about 1 ulp); IEEE forms on the host.
 MPMX_HD float rcp_approx(float x) {
 #ifdef __HIP_DEVICE_COMPILE__
   return __builtin_amdgcn_rcpf(x);
@@ -411,6 +411,13 @@ MPMX_HD float sqrt_approx(float x) {
   return __builtin_amdgcn_sqrtf(x);
 #else
   return sqrtf(x);
+#endif
+}
+MPMX_HD float rsq_approx(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_amdgcn_rsqf(x);
+#else
+  return 1.0f / sqrtf(x);
 #endif
 }
 
@@ -427,15 +434,19 @@ MPMX_HD float sqrt_approx(float x) {
 // branch-uniform across a wavefront); each rotation zeroes C_pq with
 // t = tan(angle), the smaller root of t^2 + 2 theta t - 1 = 0,
 // theta = (C_qq - C_pp) / (2 C_pq), and updates the diagonal as
-// C_pp - t C_pq, C_qq + t C_pq.  The cosine is correctly rounded so that V
-// stays orthogonal to rounding.  lam sorted descending (as Eigen sorts sigma).
+// C_pp - t C_pq, C_qq + t C_pq.  The eigenvalues depend on t alone; the
+// cosine 1 / sqrt(1 + t^2) comes from the hardware reciprocal square root
+// (1 ulp), so each rotation is orthogonal to ~1e-7 and V's columns stay unit
+// to ~1e-6 after the twelve rotations (V enters F only through the small
+// plastic correction, U is normalised on its own).  lam sorted descending (as
+// Eigen sorts sigma).
 MPMX_HD void jacobi_rot(float& app, float& aqq, float& apq, float& arp, float& arq, float* V, int p, int q) {
   float t = 0.0f;
   if (fabsf(apq) > 1e-30f) {   // (a smaller C_pq is already zero at this scale)
     const float theta = (aqq - app) * rcp_approx(2.0f * apq);
     t = copysignf(rcp_approx(fabsf(theta) + sqrt_approx(fmaf(theta, theta, 1.0f))), theta);
   }
-  const float c = 1.0f / sqrtf(fmaf(t, t, 1.0f));
+  const float c = rsq_approx(fmaf(t, t, 1.0f));
   const float sn = t * c;
   app = fmaf(-t, apq, app);
   aqq = fmaf(t, apq, aqq);
@@ -448,6 +459,21 @@ MPMX_HD void jacobi_rot(float& app, float& aqq, float& apq, float& arp, float& a
     const float vp = V[3 * r + p], vq = V[3 * r + q];
     V[3 * r + p] = c * vp - sn * vq;
     V[3 * r + q] = sn * vp + c * vq;
+  }
+}
+
+// U_i = F v_i / |F v_i| (= F v_i / sigma_i; the norm keeps U_i unit also where
+// 1 + lam_i has lost relative precision; zero for a null column).
+MPMX_HD void principal_u(const float F[9], const float V[9], float U[9]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    float w[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) w[r] = F[3 * r] * V[c] + F[3 * r + 1] * V[3 + c] + F[3 * r + 2] * V[6 + c];
+    const float n2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    const float inv = n2 > 1e-30f ? rsq_approx(n2) : 0.0f;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) U[3 * r + c] = w[r] * inv;
   }
 }
 
@@ -485,19 +511,9 @@ MPMX_HD void principal3(const float F[9], float U[9], float lam[3], float V[9]) 
   MPMX_SWAPEIG(0, 2)
   MPMX_SWAPEIG(1, 2)
 #undef MPMX_SWAPEIG
-  // U_i = F v_i / |F v_i| (= F v_i / sigma_i; the norm keeps U_i unit also
-  // where 1 + lam_i has lost relative precision)
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    lam[c] = l[c];
-    float w[3];
-#pragma unroll
-    for (int r = 0; r < 3; ++r) w[r] = F[3 * r] * V[c] + F[3 * r + 1] * V[3 + c] + F[3 * r + 2] * V[6 + c];
-    const float n2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
-    const float inv = n2 > 0.0f ? 1.0f / sqrtf(n2) : 0.0f;
-#pragma unroll
-    for (int r = 0; r < 3; ++r) U[3 * r + c] = w[r] * inv;
-  }
+  for (int c = 0; c < 3; ++c) lam[c] = l[c];
+  principal_u(F, V, U);
 }
 
 // Principal Hencky strain of the return map, log(max(sigma, 1e-4))
@@ -548,22 +564,32 @@ MPMX_HD void f_from_principal(const float Ft[9], const float U[9], const float V
 // Perzyna consistency residual r = q (dt / (visc dgam + dt))^n - q_y(eps + dgam/d)
 // at equivalent stress q (plasticity.cpp:196-229), and its derivative along the
 // flow rule q = q_tr - f dgam.
+// x^n of the Perzyna rate factor (plasticity.cpp:187-229 std::pow(tmp, n)):
+// exact for the exponents 0, 1 and 2 (x^2 rounded once, as a correctly
+// rounded pow returns it), powf otherwise.
+MPMX_HD float perzyna_pow(float x, float n) {
+  if (n == 0.0f) return 1.0f;
+  if (n == 1.0f) return x;
+  if (n == 2.0f) return x * x;
+  return powf(x, n);
+}
+
 MPMX_HD float perzyna_residual(const SceneDev& sc, float q, float dgam, float eps_in) {
   const float tm = sc.perzyna_visc * dgam + sc.dt;
   const float tmp = sc.dt / tm;
-  const float tmp1 = sc.perzyna_exp == 1.0f ? tmp : powf(tmp, sc.perzyna_exp);
+  const float tmp1 = perzyna_pow(tmp, sc.perzyna_exp);
   const float qy = sc.q_min + (sc.q_max - sc.q_min) * expf(-sc.xi * (eps_in + sc.inv_d_prefac * dgam));
   return q * tmp1 - qy;
 }
 MPMX_HD float perzyna_residual_slope(const SceneDev& sc, float q_trial, float dgam, float eps_in) {
   const float tm = sc.perzyna_visc * dgam + sc.dt;
   const float tmp = sc.dt / tm;
-  const float tmp1 = sc.perzyna_exp == 1.0f ? tmp : powf(tmp, sc.perzyna_exp);
+  const float tmp1 = perzyna_pow(tmp, sc.perzyna_exp);
   const float ex = expf(-sc.xi * (eps_in + sc.inv_d_prefac * dgam));
   const float qyd = -sc.xi * sc.inv_d_prefac * (sc.q_max - sc.q_min) * ex;
   return -sc.f_mu_prefac * tmp1 +
          (q_trial - sc.f_mu_prefac * dgam) * sc.perzyna_exp *
-             (sc.perzyna_exp == 1.0f ? 1.0f : powf(tmp, sc.perzyna_exp - 1.0f)) * (-sc.perzyna_visc * sc.dt) /
+             perzyna_pow(tmp, sc.perzyna_exp - 1.0f) * (-sc.perzyna_visc * sc.dt) /
              (tm * tm) -
          qyd;
 }
@@ -587,7 +613,7 @@ MPMX_HD float perzyna_dgamma(const SceneDev& sc, float q_trial, float q_yield, b
     if (it == max_iter - 1) err |= ERR_NEWTON;
     float tm = sc.perzyna_visc * dgam + sc.dt;
     float tmp = sc.dt / tm;
-    float tmp1 = powf(tmp, sc.perzyna_exp);
+    float tmp1 = perzyna_pow(tmp, sc.perzyna_exp);
     float qyd = 0.0f;
     if (harden) {
       float ex = expf(-sc.xi * (eps + sc.inv_d_prefac * dgam));
@@ -597,7 +623,7 @@ MPMX_HD float perzyna_dgamma(const SceneDev& sc, float q_trial, float q_yield, b
     float res = (q_trial - sc.f_mu_prefac * dgam) * tmp1 - q_yield;
     if (fabsf(res) < 1e-2f) break;
     float rd = -sc.f_mu_prefac * tmp1 +
-               (q_trial - sc.f_mu_prefac * dgam) * sc.perzyna_exp * powf(tmp, sc.perzyna_exp - 1.0f) *
+               (q_trial - sc.f_mu_prefac * dgam) * sc.perzyna_exp * perzyna_pow(tmp, sc.perzyna_exp - 1.0f) *
                    (-sc.perzyna_visc * sc.dt) / (tm * tm) - qyd;
     if (fabsf(rd) < 1e-14f) err |= ERR_NEWTON;
     dgam -= res / rd;

@@ -2,14 +2,23 @@
 //
 // MLP 8 -> 64 -> 64 -> 64 -> 64 -> 4, SiLU on the hidden layers, BF16 weights
 // and activations, FP32 accumulation and biases.  One wavefront evaluates 64
-// samples at once (row = sample) with v_mfma_f32_16x16x16_bf16:
-//   A operand (16x16, M x K): lane l holds A[l % 16][4 (l / 16) + i], i = 0..3
-//   B operand (16x16, K x N): lane l holds B[4 (l / 16) + i][l % 16]
+// samples at once (row = sample).  The 64-wide products run on gfx950's
+// v_mfma_f32_16x16x32_bf16 (K = 32 per instruction); the first layer (8
+// inputs padded to 16) on v_mfma_f32_16x16x16_bf16:
+//   A operand (16 x K, M x K): lane l holds A[l % 16][k(l / 16, i)]
+//   B operand (K x 16, K x N): lane l holds B[k(l / 16, i)][l % 16]
 //   C/D       (16x16, M x N): lane l holds D[4 (l / 16) + i][l % 16]
+// with i = 0..3 (K = 16) or 0..7 (K = 32).  The product sums A[m][k] B[k][n]
+// over the K slots, so any assignment of logical k to (lane group, slot) gives
+// the same result as long as A and B use the same one.  For K = 32 this code
+// assigns slot i of group g to k = 4 g + i (i < 4) and k = 16 + 4 g + (i - 4)
+// (i >= 4): each operand is then two of the K = 16 fragments (k-blocks 2m and
+// 2m + 1) side by side, so weights, activations and the D -> B hand-over below
+// keep their K = 16 layout.
 // Weights are stored W[out][in] row-major, which is exactly the B operand
-// layout of Z = A W^T (four consecutive `in` values per lane).  Rows are padded
-// (MLP_S0 / MLP_S bf16) so that the 16 rows a wavefront reads at once fall in
-// different LDS banks.
+// layout of Z = A W^T (four consecutive `in` values per lane and fragment).
+// Rows are padded (MLP_S0 / MLP_S bf16) so that the 16 rows a wavefront reads
+// at once fall in different LDS banks.
 //
 // Rounding points (the CPU oracle of the network rounds at the same places):
 // normalised inputs -> BF16; weights -> BF16; each hidden layer's
@@ -47,6 +56,8 @@ typedef short mlp_bf16x4 __attribute__((ext_vector_type(4)));
 typedef float mlp_f32x4 __attribute__((ext_vector_type(4)));
 typedef float mlp_f32x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 mlp_bf16x2v __attribute__((ext_vector_type(2)));
+typedef short mlp_s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 mlp_bf16x8 __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
   // round to nearest even (inputs here are finite)
@@ -80,6 +91,24 @@ __device__ __forceinline__ float silu_grad(float z) {
 
 __device__ __forceinline__ mlp_f32x4 mfma16(mlp_bf16x4 a, mlp_bf16x4 b, mlp_f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+// K = 32: operands are the K = 16 fragments of k-blocks 2m (lo) and 2m + 1 (hi)
+__device__ __forceinline__ mlp_f32x4 mfma32(mlp_bf16x4 a_lo, mlp_bf16x4 a_hi, mlp_bf16x4 b_lo, mlp_bf16x4 b_hi,
+                                            mlp_f32x4 c) {
+  const mlp_s16x8 a = __builtin_shufflevector(a_lo, a_hi, 0, 1, 2, 3, 4, 5, 6, 7);
+  const mlp_s16x8 b = __builtin_shufflevector(b_lo, b_hi, 0, 1, 2, 3, 4, 5, 6, 7);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mlp_bf16x8, a), __builtin_bit_cast(mlp_bf16x8, b),
+                                                 c, 0, 0, 0);
+}
+// Z += A B over K = 64 from LDS rows: a_row / b_row point at the lane's row of
+// the A (M x K) and B^T (N x K) tiles, bf16, consecutive k.
+__device__ __forceinline__ mlp_f32x4 mfma_k64(const uint16_t* a_row, const uint16_t* b_row, mlp_f32x4 c) {
+  const int g = (threadIdx.x & 63) >> 4;
+#pragma unroll
+  for (int m = 0; m < 2; ++m)
+    c = mfma32(ld_bf16x4(a_row + 32 * m + 4 * g), ld_bf16x4(a_row + 32 * m + 16 + 4 * g),
+               ld_bf16x4(b_row + 32 * m + 4 * g), ld_bf16x4(b_row + 32 * m + 16 + 4 * g), c);
+  return c;
 }
 
 struct MlpNet {
@@ -125,10 +154,17 @@ __device__ __forceinline__ void mlp_layer_t(const uint16_t* W, int WS, const flo
     mlp_f32x4 acc[OB];
 #pragma unroll
     for (int ob = 0; ob < OB; ++ob) acc[ob] = bz[ob];   // Z = b + sum_k W A
+    if constexpr (KB == 4) {
 #pragma unroll
-    for (int kb = 0; kb < KB; ++kb)
+      for (int m = 0; m < 2; ++m)
 #pragma unroll
-      for (int ob = 0; ob < OB; ++ob) acc[ob] = mfma16(wf[ob][kb], act[sb][kb], acc[ob]);
+        for (int ob = 0; ob < OB; ++ob)
+          acc[ob] = mfma32(wf[ob][2 * m], wf[ob][2 * m + 1], act[sb][2 * m], act[sb][2 * m + 1], acc[ob]);
+    } else {
+      static_assert(KB == 1, "64-wide (K = 32 pairs) or one 16-wide k-block");
+#pragma unroll
+      for (int ob = 0; ob < OB; ++ob) acc[ob] = mfma16(wf[ob][0], act[sb][0], acc[ob]);
+    }
     if constexpr (HIDDEN) {
 #pragma unroll
       for (int ob = 0; ob < OB; ++ob) {

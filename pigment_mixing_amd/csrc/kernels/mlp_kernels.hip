@@ -115,13 +115,8 @@ __device__ __forceinline__ void grad_w(const uint16_t* dzt, const uint16_t* at, 
   for (int mb = 0; mb < NBo; ++mb) {
 #pragma unroll 1
     for (int nb = 0; nb < NBi; ++nb) {
-      mlp_f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-      for (int kb = 0; kb < 4; ++kb) {
-        const mlp_bf16x4 a = ld_bf16x4(dzt + (mb * 16 + r16) * MLP_S + kb * 16 + 4 * g);
-        const mlp_bf16x4 b = ld_bf16x4(at + (nb * 16 + r16) * MLP_S + kb * 16 + 4 * g);
-        acc = mfma16(a, b, acc);
-      }
+      const mlp_f32x4 acc = mfma_k64(dzt + (mb * 16 + r16) * MLP_S, at + (nb * 16 + r16) * MLP_S,
+                                     mlp_f32x4{0.0f, 0.0f, 0.0f, 0.0f});
       const int k = nb * 16 + r16;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
@@ -177,9 +172,11 @@ __global__ void __launch_bounds__(64) k_mlp_grad(const uint8_t* __restrict__ blo
 #pragma unroll 1
       for (int nb = 0; nb < NB; ++nb) {
         mlp_f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        for (int kb = 0; kb < KB; ++kb) {
-          const mlp_bf16x4 a = ld_bf16x4(in + (rb * 16 + r16) * MLP_S + kb * 16 + 4 * g);
-          const mlp_bf16x4 b = ld_bf16x4(W[l] + (nb * 16 + r16) * WS + kb * 16 + 4 * g);
+        if (KB == 4) {
+          acc = mfma_k64(in + (rb * 16 + r16) * MLP_S, W[l] + (nb * 16 + r16) * WS, acc);
+        } else {
+          const mlp_bf16x4 a = ld_bf16x4(in + (rb * 16 + r16) * MLP_S + 4 * g);
+          const mlp_bf16x4 b = ld_bf16x4(W[l] + (nb * 16 + r16) * WS + 4 * g);
           acc = mfma16(a, b, acc);
         }
         const int col = nb * 16 + r16;
@@ -260,11 +257,15 @@ __global__ void __launch_bounds__(64) k_mlp_grad(const uint8_t* __restrict__ blo
       mlp_f32x4 acc[4];
 #pragma unroll
       for (int nb = 0; nb < 4; ++nb) acc[nb] = mlp_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-      for (int kb = 0; kb < KB; ++kb) {
-        const mlp_bf16x4 a = ld_bf16x4(S.dz + (rb * 16 + r16) * MLP_S + kb * 16 + 4 * g);
+      if (KB == 4) {
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) acc[nb] = mfma_k64(S.dz + (rb * 16 + r16) * MLP_S, wtl + (nb * 16 + r16) * WTS,
+                                                          acc[nb]);
+      } else {
+        const mlp_bf16x4 a = ld_bf16x4(S.dz + (rb * 16 + r16) * MLP_S + 4 * g);
 #pragma unroll
         for (int nb = 0; nb < 4; ++nb) {
-          const mlp_bf16x4 b = ld_bf16x4(wtl + (nb * 16 + r16) * WTS + kb * 16 + 4 * g);
+          const mlp_bf16x4 b = ld_bf16x4(wtl + (nb * 16 + r16) * WTS + 4 * g);
           acc[nb] = mfma16(a, b, acc[nb]);
         }
       }

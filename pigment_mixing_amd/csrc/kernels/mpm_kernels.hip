@@ -790,9 +790,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             deferred = deferred && slot < d.cq_cap;   // a full queue maps in place
             if (deferred) {
               float4* r = d.cq + CQ_REC * slot;
-              r[0] = make_float4(__int_as_float((int)j), __int_as_float(s), eps_in, Ft[0]);
-              r[1] = make_float4(Ft[1], Ft[2], Ft[3], Ft[4]);
-              r[2] = make_float4(Ft[5], Ft[6], Ft[7], Ft[8]);
+              r[0] = make_float4(__int_as_float((int)j), __int_as_float(s), eps_in, lam[0]);
+              r[1] = make_float4(lam[1], lam[2], Ft[0], Ft[1]);
+              r[2] = make_float4(Ft[2], Ft[3], Ft[4], Ft[5]);
+              r[3] = make_float4(Ft[6], Ft[7], Ft[8], V[0]);
+              r[4] = make_float4(V[1], V[2], V[3], V[4]);
+              r[5] = make_float4(V[5], V[6], V[7], V[8]);
             }
           }
         }
@@ -896,12 +899,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 }
 
 // The learned closure over the queued candidates (one record per lane, 64 per
-// wavefront on MFMA): SVD of the trial F (the same routine as G2P), learned map
+// wavefront on MFMA): the trial F's principal frame as G2P found it, learned map
 // with the exact map as per-lane fallback, then F = U exp(h') V^T, eps_pl_dev,
 // dgamma / dt and the Kirchhoff stress into the particle's output slot.  The
 // queue's order (wavefront atomics) does not change any result: every record
 // is mapped on its own (the network's rows are independent).
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_closure(DevPtrs d, const SceneDev* __restrict__ scenes,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_closure(DevPtrs d, const SceneDev* __restrict__ scenes,
                                                  const uint8_t* __restrict__ mlp_blob) {
   __shared__ __align__(16) G2PLearnedLds lrn;
   __shared__ unsigned int plastic_wg, fallback_wg, learned_wg, newton_wg;
@@ -934,13 +937,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
       if (q >= pre[k]) slot = start[k] + (q - pre[k]);
     const bool valid = q0 < n && slot < d.cq_cap;      // G2P mapped in place past the capacity
     const float4* r = d.cq + CQ_REC * min((int64_t)slot, d.cq_cap - 1);
-    const float4 r0 = r[0], r1 = r[1], r2 = r[2];
+    const float4 r0 = r[0];
+    const float2 r1 = *reinterpret_cast<const float2*>(r + 1);
     const int64_t j = __float_as_int(r0.x);
     const SceneDev& sc = scenes[__float_as_int(r0.y)];
     const float eps_in = r0.z;
-    const float Ft[9] = {r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w};
-    float U[9], lam[3], V[9];
-    principal3(Ft, U, lam, V);
+    const float lam[3] = {r0.w, r1.x, r1.y};
     float h[3] = {hencky_of(lam[0]), hencky_of(lam[1]), hencky_of(lam[2])};
     const float h_trial[3] = {h[0], h[1], h[2]};
     float eps = eps_in, dgr = 0.0f;
@@ -948,6 +950,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     int newton = 0, plastic = 0;
     bool cand = false;
     const bool ok = learned_return_map(sc, lrn, h, eps, dgr, valid, cand);
+    // the trial F and V only after the network (its LDS fences keep these loads
+    // below it), so that they do not hold registers across it
+    const float4 r1b = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5];
+    const float Ft[9] = {r1b.z, r1b.w, r2.x, r2.y, r2.z, r2.w, r3.x, r3.y, r3.z};
+    const float V[9] = {r3.w, r4.x, r4.y, r4.z, r4.w, r5.x, r5.y, r5.z, r5.w};
+    float U[9];
+    principal_u(Ft, V, U);
     if (ok) plastic = 1;
     else if (sc.plastic_model == 5) plastic = vmvisc_principal(sc, h, eps, dgr, err, &newton);
     if (valid) {

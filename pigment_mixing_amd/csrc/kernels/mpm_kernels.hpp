@@ -87,7 +87,11 @@ struct DevPtrs {
   int tile_cap;            // listed blocks with a tile; later ones scatter with atomics
   int track_vol;           // some scene carries the eps_pl_vol / pradhana rows (G2P variant)
 };
-constexpr int CQ_REC = 3;              // float4 per closure-queue record: {slot, scene, eps, F0} {F1..F4} {F5..F8}
+// float4 per closure-queue record: the slot, scene, eps_pl_dev, the trial F and
+// its principal frame from G2P (lam, V), so that the closure does not decompose
+// F again: {slot, scene, eps, lam0} {lam1, lam2, F0, F1} {F2..F5} {F6, F7, F8, V0}
+// {V1..V4} {V5..V8}
+constexpr int CQ_REC = 6;
 // The queue is split into one segment per XCD walker: segment x starts at the
 // first particle slot of XCD x's contiguous share of the block list and can
 // hold every particle of that share, so it never overflows.

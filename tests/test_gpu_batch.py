@@ -133,3 +133,49 @@ def test_batch_mixing_drucker_prager_and_von_mises_scenes():
         if i == 1:
             assert not a["eps_vol"].any() and not a["prad"].any()
         assert_identical(a, b, f"mixed/{i} ({'Drucker-Prager' if i == 0 else 'von Mises'}) vs alone")
+
+
+def test_batch_64_scenes_full_workload():
+    """The batched workload at its own size: 64 scenes of 20k-400k particles on
+    64^3-160^3 lattices (uneven rheology and knife speed), one handle.  After 40
+    steps the smallest scene matches the CPU oracle at the parity bars; after 100
+    steps no device error is set, every particle is live and inside its own
+    scene's lattice, scene 0's node mass equals its particles' mass (the
+    float-coordinate partition of unity, as the large-scene test), and the
+    smallest and largest scenes equal their own single-scene runs bit for bit."""
+    sl = scenes.batch_scenes(64)
+    sizes = np.array([sc["x"].shape[0] for sc in sl])
+    lo, hi = int(sizes.argmin()), int(sizes.argmax())
+    log(f"[batch64] {sizes.sum()} particles, sizes {sizes.min()}..{sizes.max()}, lattices "
+        f"{min(sc['cfg'].grid_dims[0] for sc in sl)}..{max(sc['cfg'].grid_dims[0] for sc in sl)}")
+    assert sizes.min() >= 15_000 and sizes.max() <= 450_000
+    hb = scenes.make_batch_handle(sl)
+    hb.step(40)
+    ob = hb.download()
+    o = orc.from_scene(sl[lo])
+    o.step(40)
+    check_particles(split(ob, hb.scene_range(lo)), o.download(), scenes.DX, f"batch64/{lo} vs oracle")
+    del o, ob
+    m0 = hb.grid()["mass"].sum(dtype=np.float64)
+    hb.step(60)
+    st = hb.stats()
+    assert st.device_error == 0 and st.stray_count == 0
+    assert st.n_active == sizes.sum(), f"{sizes.sum() - st.n_active} particles deactivated"
+    ob = hb.download()
+    assert np.isfinite(ob["x"]).all() and np.isfinite(ob["v"]).all() and np.isfinite(ob["F"]).all()
+    for i, sc in enumerate(sl):
+        a, n = hb.scene_range(i)
+        x = ob["x"][a:a + n].astype(np.float64)
+        org = sc["x"].min(0).astype(np.float64) - 2 * sc["cfg"].dx     # the lattice origin rule (bbox_min - 2dx)
+        ext = (np.asarray(sc["cfg"].grid_dims[:], np.float64) - 1) * sc["cfg"].dx
+        assert (x >= org).all() and (x <= org + ext).all(), f"scene {i} left its lattice"
+    m1 = hb.grid()["mass"].sum(dtype=np.float64)
+    np.testing.assert_allclose(m1, m0, rtol=1e-6)
+    mp = sl[0]["cfg"].rho * sl[0]["cfg"].dx ** 3 / sl[0]["cfg"].ppc
+    np.testing.assert_allclose(m0, sizes[0] * mp, rtol=2e-5)
+    for i in (lo, hi):
+        h1 = scenes.make_handle(sl[i])
+        h1.step(100)
+        assert_identical(split(ob, hb.scene_range(i)), h1.download(), f"batch64/{i} vs alone")
+        h1.close()
+    hb.close()
