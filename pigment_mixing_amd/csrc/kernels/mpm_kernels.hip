@@ -432,10 +432,10 @@ struct TrialState {
 __device__ __forceinline__ TrialState trial_state(const SceneDev& sc, const float h[3], float eps) {
   TrialState t;
   t.htr = (h[0] + h[1]) + h[2];
-  const float m = t.htr / 3.0f;
+  const float m = fdiv(t.htr, 3.0f);
   t.nv[0] = h[0] - m; t.nv[1] = h[1] - m; t.nv[2] = h[2] - m;
   t.dn = sqrtf(t.nv[0] * t.nv[0] + t.nv[1] * t.nv[1] + t.nv[2] * t.nv[2]);
-  if (t.dn > 0.0f) { const float inv = 1.0f / t.dn; t.nv[0] *= inv; t.nv[1] *= inv; t.nv[2] *= inv; }
+  if (t.dn > 0.0f) { const float inv = rcp_approx(t.dn); t.nv[0] *= inv; t.nv[1] *= inv; t.nv[2] *= inv; }
   t.q_trial = sc.e_mu_prefac * t.dn;
   t.q_yield = sc.q_min + (sc.q_max - sc.q_min) * expf(-sc.xi * eps);
   const float p_trial = -sc.K * t.htr;
@@ -473,7 +473,7 @@ __device__ bool learned_return_map(const SceneDev& sc, G2PLearnedLds& lrn, float
     // strains by the flow rule
     const float r = perzyna_residual(sc, q_trial - sc.f_mu_prefac * dg, dg, eps);
     const float rd = perzyna_residual_slope(sc, q_trial, dg, eps);
-    if (rd != 0.0f) dg = fmaxf(dg - r / rd, 0.0f);
+    if (rd != 0.0f) dg = fmaxf(dg - fdiv(r, rd), 0.0f);
     const float fr = sc.inv_d_prefac * dg;
     hp[0] = h[0] - fr * nv[0]; hp[1] = h[1] - fr * nv[1]; hp[2] = h[2] - fr * nv[2];
   } else {
@@ -495,7 +495,7 @@ __device__ bool learned_return_map(const SceneDev& sc, G2PLearnedLds& lrn, float
   if (!(ry <= lim && rv <= lim && rf <= lim)) return false;
   h[0] = hp[0]; h[1] = hp[1]; h[2] = hp[2];
   eps = eps + sc.inv_d_prefac * dg;
-  dg_rate = dg / sc.dt;
+  dg_rate = fdiv(dg, sc.dt);
   return true;
 }
 
@@ -730,14 +730,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             sn += sv * sv;
           }
         const float diff_new = sqrtf(sn);
-        const float tt = fmaxf(0.0f, fminf(1.0f, (gs - sc.D_edge0) / (sc.D_edge1 - sc.D_edge0)));
+        const float tt = fmaxf(0.0f, fminf(1.0f, fdiv(gs - sc.D_edge0, sc.D_edge1 - sc.D_edge0)));
         const float mixf = tt * tt * (3.0f - 2.0f * tt);
         const float smf = fminf(fmaxf(dynD * mixf * sc.dt, 0.0f), 1.0f);
         if constexpr (PHASE != 2) {
 #pragma unroll
           for (int c = 0; c < NPIG; ++c) pig[c] = (1.0f - smf) * pig[c] + smf * gp[c];
           const float ps = ((pig[0] + pig[1]) + pig[2]) + pig[3];
-          if (ps > 1e-6f) { pig[0] /= ps; pig[1] /= ps; pig[2] /= ps; pig[3] /= ps; }
+          if (ps > 1e-6f) {
+            const float ip = rcp_approx(ps);
+            pig[0] *= ip; pig[1] *= ip; pig[2] *= ip; pig[3] *= ip;
+          }
         }
         if constexpr (PHASE == 1) {
           // first half of MUSL (musl.cpp:13-26): particle velocity update; park
@@ -904,7 +907,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 // dgamma / dt and the Kirchhoff stress into the particle's output slot.  The
 // queue's order (wavefront atomics) does not change any result: every record
 // is mapped on its own (the network's rows are independent).
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_closure(DevPtrs d, const SceneDev* __restrict__ scenes,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_closure(DevPtrs d, const SceneDev* __restrict__ scenes,
                                                  const uint8_t* __restrict__ mlp_blob) {
   __shared__ __align__(16) G2PLearnedLds lrn;
   __shared__ unsigned int plastic_wg, fallback_wg, learned_wg, newton_wg;
