@@ -30,28 +30,13 @@ namespace mpmx {
 
 constexpr int BLK = 4;            // nodes per block edge
 constexpr int BLK3 = 64;          // nodes per block
-constexpr int ACC_CH = 14;        // mom3, force3, pig7, shear1 (float accumulators)
+constexpr int ACC_CH = 15;        // mom3, force3, pig7, shear1, mass1 (float accumulators)
 constexpr int GV_CH = 15;         // v3, flip3, pig7, shear1, mass1
 constexpr int NPIG = 7;
 constexpr int MAX_PLATES = 4;
-// Node mass (sum of weights) is accumulated in u64 fixed point, 2^52 per unit
-// weight: integer adds are order-independent, so grid masses are bitwise
-// reproducible whatever the atomic order; a node holds up to 4096 units.
-constexpr double MASS_FX_SCALE = 4503599627370496.0;        // 2^52
-constexpr double MASS_FX_INV = 1.0 / 4503599627370496.0;
-// Fixed-point mass of a weight w in [0, 1] already scaled by 2^20
-// (t = w 2^20): rn(w 2^52) = hi 2^32 + lo with hi = floor(t) and
-// lo = rn(frac(t) 2^32), in FP32.  The scalings by powers of two and the
-// fraction are exact (truncation is the floor for t >= 0), so the only
-// rounding is the round-to-nearest-even of lo, which is the rounding of
-// w 2^52 itself (hi carries no fraction): bit-identical to rn((double)w 2^52).
-__device__ __forceinline__ void mass_fx_scaled(float t, unsigned& hi, unsigned& lo) {
-  hi = (unsigned)t;
-  lo = (unsigned)rintf(__builtin_amdgcn_fractf(t) * 4294967296.0f);
-}
 
 // channel indices
-enum { A_MOM = 0, A_FRC = 3, A_PIG = 6, A_SHR = 13 };
+enum { A_MOM = 0, A_FRC = 3, A_PIG = 6, A_SHR = 13, A_MASS = 14 };
 enum { G_V = 0, G_FLIP = 3, G_PIG = 6, G_SHR = 13, G_MASS = 14 };
 
 // device error bits (the reference's `exit = 1` causes)

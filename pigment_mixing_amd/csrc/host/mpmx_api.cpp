@@ -412,7 +412,7 @@ void enqueue_step(mpmx_handle* h, const DevPtrs& d) {
   enqueue_g2p(h, d);
   launch_scan(d, h->nkeys, 1, h->stream);
   launch_cellscan(d, grid_grid(h), h->stream);
-  launch_perm(d, h->np, grid_grid(h), h->stream);
+  launch_perm(d, h->np, grid_grid(h), 1, h->stream);
   launch_p2g(d, grid_p2g(h), 1, h->stream);
   launch_grid(d, grid_grid(h), h->stream);
 }
@@ -905,6 +905,7 @@ int mpmx_prepare(mpmx_handle* h) {
   ALLOC(d.key_out, int, n);
   ALLOC(d.rank_out, int, n);
   ALLOC(d.cell_out, int, n);
+  ALLOC(d.okey_out, int, n);
   ALLOC(d.perm, int, n);
   ALLOC(d.kslot, int, n);
   ALLOC(d.cmaj, int, n);
@@ -916,7 +917,6 @@ int mpmx_prepare(mpmx_handle* h) {
   // per block); blocks beyond it scatter with atomics
   d.tile_cap = (int)std::min<int64_t>(h->nkeys, std::max<int64_t>(1024, n / 128));
   ALLOC(d.tile_acc, float, (size_t)d.tile_cap * ACC_CH * PT3);
-  ALLOC(d.tile_mass, unsigned long long, (size_t)d.tile_cap * PT3);
   ALLOC(d.list_pos, int, h->nkeys);
   if (learned(h)) {
     // closure queue: every live particle can be a candidate in one step
@@ -931,7 +931,6 @@ int mpmx_prepare(mpmx_handle* h) {
   ALLOC(d.scan_partials, unsigned long long, h->nchunks);
   int* bsc; ALLOC(bsc, int, h->nkeys); d.blk_scene = bsc;
   ALLOC(d.acc, float, (size_t)nb_total * ACC_CH * BLK3);
-  ALLOC(d.massfx, unsigned long long, (size_t)nb_total * BLK3);
   ALLOC(d.gv, float, (size_t)nb_total * GV_STRIDE * BLK3);
   ALLOC(d.nb_flag, int, nb_total);
   ALLOC(d.nb_list, int, (size_t)2 * nb_total);
@@ -975,7 +974,7 @@ int mpmx_prepare(mpmx_handle* h) {
   launch_keys(d, n, h->stream);
   launch_scan(d, h->nkeys, 0, h->stream);
   launch_cellscan(d, grid_grid(h), h->stream);
-  launch_perm(d, n, grid_grid(h), h->stream);
+  launch_perm(d, n, grid_grid(h), 0, h->stream);
   launch_p2g(d, grid_p2g(h), 0, h->stream);
   launch_grid(d, grid_grid(h), h->stream);
   HIPCHK(h, hipGetLastError());
@@ -1541,7 +1540,7 @@ int mpmx_profile_steps(mpmx_handle* h, int nsteps, mpmx_kernel_times* out) {
       HIPCHK(h, hipEventRecord(ev[1], h->stream));
       launch_scan(d, h->nkeys, 1, h->stream);
       launch_cellscan(d, grid_grid(h), h->stream);
-      launch_perm(d, h->np, grid_grid(h), h->stream);
+      launch_perm(d, h->np, grid_grid(h), 1, h->stream);
       HIPCHK(h, hipEventRecord(ev[2], h->stream));
       launch_p2g(d, grid_p2g(h), 1, h->stream);
       HIPCHK(h, hipEventRecord(ev[3], h->stream));

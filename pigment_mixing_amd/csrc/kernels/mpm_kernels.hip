@@ -12,7 +12,7 @@
 //   k_cellscan per block: offsets of the 64 cells inside the block, and the
 //             rank-major slot of every (cell, rank).
 //   k_perm    particles to cell-major positions by their counting ranks;
-//   k_cellsort orders each cell by previous slot (deterministic) and writes the
+//   k_cellsort orders each cell by particle position (canonical) and writes the
 //             permutation slot <- particle: block-major, rank-major in a block.
 //   k_p2g     lane = cell, wave = (block, channel group): each lane walks its
 //             cell's particles and accumulates their 27-node contributions (mass,
@@ -85,6 +85,22 @@ __device__ __forceinline__ int particle_key(const SceneDev& sc, const float x[3]
   return node_block_of(sc, b0, b1, b2);
 }
 
+// Order key of a particle inside its cell (k_cellsort): its position in the
+// cell, 10 bits per axis from the same scaled coordinate that bins it
+// (stencil_base), x in the high bits.  Monotone in each coordinate, so ranking
+// by the key and, where keys are equal, by the full position gives a total
+// order that depends on the positions (and ids) alone.
+__device__ __forceinline__ uint32_t cell_order_key(const SceneDev& sc, const float x[3]) {
+  uint32_t k = 0u;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float f = __fsub_rn(__fmul_rn(__fsub_rn(x[a], sc.origin[a]), sc.one_over_dx), 0.5f);
+    const int q = (int)((f - floorf(f)) * 1024.0f);
+    k = (k << 10) | (uint32_t)min(max(q, 0), 1023);
+  }
+  return k;
+}
+
 __device__ __forceinline__ void block_coords(const SceneDev& sc, int key, int& bi, int& bj, int& bk) {
   const int lb = key - sc.node_block_base;
   bi = lb / (sc.bdims[1] * sc.bdims[2]);
@@ -118,7 +134,8 @@ __device__ __forceinline__ XcdRange xcd_range(int nblk) {
 // a wavefront (one atomic per distinct key); cell keys mostly do not (the
 // lanes walk a rank-major block layout, one particle per cell and rank), so
 // each live lane takes its cell rank with its own atomic.
-__device__ __forceinline__ void count_for_sort(const DevPtrs& d, bool valid, int key, int cell, int64_t j) {
+__device__ __forceinline__ void count_for_sort(const DevPtrs& d, bool valid, int key, int cell, int64_t j,
+                                               uint32_t okey = 0u) {
   const bool live = valid && cell >= 0;
   const int rb = wave_rank(valid, key, d.count_next);
   const int rc = live ? atomicAdd(&d.cell_cnt[key * BLK3 + cell], 1) : 0;
@@ -126,6 +143,7 @@ __device__ __forceinline__ void count_for_sort(const DevPtrs& d, bool valid, int
     d.key_out[j] = key;
     d.cell_out[j] = live ? cell : -1;
     d.rank_out[j] = live ? rc : rb;
+    if (live) d.okey_out[j] = okey;
   }
 }
 
@@ -153,10 +171,12 @@ __global__ void k_keys(DevPtrs d, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n;
   int key = d.dead_key, cell = -1;
+  uint32_t okey = 0u;
   if (valid) {
     const SceneDev& sc = d.scenes[d.pscene[i]];
     const float x[3] = {d.pa[0 * d.cap + i], d.pa[1 * d.cap + i], d.pa[2 * d.cap + i]};
     if (d.pa_id[i] >= 0 && d.active_in[i]) key = particle_key(sc, x, d.dead_key, cell);
+    okey = cell_order_key(sc, x);
     if (key == d.dead_key) {
       // inactive from the start: v = 0 (position_update.cpp:222-228 convention)
       d.pa[3 * d.cap + i] = 0.0f;
@@ -164,7 +184,7 @@ __global__ void k_keys(DevPtrs d, int64_t n) {
       d.pa[5 * d.cap + i] = 0.0f;
     }
   }
-  count_for_sort(d, valid, key, cell, i);
+  count_for_sort(d, valid, key, cell, i, okey);
 }
 
 // ------------------------------------------------------------------ scans
@@ -329,24 +349,42 @@ __global__ void k_perm(DevPtrs d, int64_t n) {
 
 // ------------------------------------------------------------------ k_cellsort
 // Per listed block (one wavefront, lane = cell): the particles of a cell in
-// ascending order of their previous slot.  The counting ranks depend on the
-// timing of the atomics; ordering by previous slot makes the order inside a
-// cell, and with it every FP32 sum of the scatter, a function of the previous
-// order alone.  By induction from the upload order, two runs of a scene are
-// bit-identical, and a scene inside a batch (whose particles keep their
-// relative order) sorts exactly as the same scene alone.  Previous slots are
-// distinct, so the ranks are a permutation of 0..c-1; rank r of the cell goes
-// to its rank-major slot kslot[cell start + r].  Cells of up to CS_LDS
-// particles are ranked from an LDS column per lane, larger ones from global
-// memory (same arithmetic).
+// ascending order of their position: the order key (cell_order_key, written
+// by the key pass), then on equal keys x, y, z and the particle id.  The
+// counting ranks depend on the timing of the atomics; ordering by position
+// makes the order inside a cell, and with it every FP32 sum of the scatter
+// (mass, momentum, force, latents, shear), a function of the particles' state
+// alone: independent of the upload order, of the layout the previous step
+// left, of the other scenes of a batch and of the launch shapes.  Positions
+// and ids come from the side the sort orders.  Ranks are a permutation of
+// 0..c-1; rank r of the cell goes to its rank-major slot kslot[cell start +
+// r].  Cells of up to CS_LDS particles are ranked from an LDS column of keys
+// per lane, larger ones from global memory (same order).
 constexpr int CS_LDS = 32;
-__global__ void __launch_bounds__(256) k_cellsort(DevPtrs d) {
-  __shared__ int col[4][CS_LDS][64];
+// float -> unsigned with the same order (negative values reversed below the
+// positive ones); -0 sorts just below +0
+__device__ __forceinline__ uint32_t ord_key(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+// slot a before slot b among particles with equal order keys: x, y, z, id
+__device__ __forceinline__ bool pos_before(const float* __restrict__ X, const int* __restrict__ ids, int64_t cap,
+                                           int a, int b) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const uint32_t ka = ord_key(X[(P_X + c) * cap + a]), kb = ord_key(X[(P_X + c) * cap + b]);
+    if (ka != kb) return ka < kb;
+  }
+  return a != b && ids[a] < ids[b];
+}
+__global__ void __launch_bounds__(256) k_cellsort(DevPtrs d, const float* __restrict__ X, const int* __restrict__ ids) {
+  __shared__ uint32_t col[4][CS_LDS][64];
   const int nblk = d.counters[CNT_PBLOCKS];
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int nwaves = (gridDim.x * blockDim.x) >> 6;
   const int w = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
+  const int64_t cap = d.cap;
   for (int e = wave; e < nblk; e += nwaves) {
     const int c = d.cell_num[e * BLK3 + lane];
     const int base = d.pdesc[e].y + d.cell_off[e * BLK3 + lane];
@@ -354,18 +392,33 @@ __global__ void __launch_bounds__(256) k_cellsort(DevPtrs d) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) cmax = max(cmax, __shfl_xor(cmax, off));
     if (cmax <= CS_LDS) {
-      for (int r = 0; r < c; ++r) col[w][r][lane] = d.cmaj[base + r];
+      for (int r = 0; r < c; ++r) col[w][r][lane] = (uint32_t)d.okey_out[d.cmaj[base + r]];
       for (int r = 0; r < c; ++r) {
-        const int v = col[w][r][lane];
+        const uint32_t kr = col[w][r][lane];
         int rk = 0;
-        for (int s = 0; s < c; ++s) rk += col[w][s][lane] < v ? 1 : 0;
+        bool tie = false;
+        for (int s = 0; s < c; ++s) {
+          const uint32_t ks = col[w][s][lane];
+          rk += ks < kr ? 1 : 0;
+          tie |= s != r && ks == kr;
+        }
+        const int v = d.cmaj[base + r];
+        if (tie) {   // equal order keys (particles within 1/1024 cell of each other): full positions decide
+          for (int s = 0; s < c; ++s)
+            if (s != r && col[w][s][lane] == kr) rk += pos_before(X, ids, cap, d.cmaj[base + s], v) ? 1 : 0;
+        }
         d.perm[d.kslot[base + rk]] = v;
       }
     } else {
       for (int r = 0; r < c; ++r) {
         const int v = d.cmaj[base + r];
+        const uint32_t kr = (uint32_t)d.okey_out[v];
         int rk = 0;
-        for (int s = 0; s < c; ++s) rk += d.cmaj[base + s] < v ? 1 : 0;
+        for (int s = 0; s < c; ++s) {
+          const int vs = d.cmaj[base + s];
+          const uint32_t ks = (uint32_t)d.okey_out[vs];
+          rk += (ks < kr || (ks == kr && pos_before(X, ids, cap, vs, v))) ? 1 : 0;
+        }
         d.perm[d.kslot[base + rk]] = v;
       }
     }
@@ -600,6 +653,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
       // Every lane computes; a lane past the block's count recomputes the block's
       // last particle and stores nothing.
       const int64_t j = start + (valid ? t : cnt - 1);
+      uint32_t okey = 0u;
       int newkey = d.dead_key, newcell = -1, plastic = 0;
       int newton = 0;
       {
@@ -858,6 +912,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
           if (alive) {
             newkey = particle_key(sc, xn, d.dead_key, newcell);
             if (newkey == d.dead_key) alive = false;
+            okey = cell_order_key(sc, xn);
           }
           if (!alive) { vn[0] = vn[1] = vn[2] = 0.0f; }
         }
@@ -887,7 +942,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
           d.pb_id[j] = id;
         }
       }
-      count_for_sort(d, valid, newkey, newcell, j);
+      count_for_sort(d, valid, newkey, newcell, j, okey);
       const uint64_t pm = __ballot(valid && plastic == 1);
       if (__lane_id() == 0 && pm) atomicAdd(&plastic_wg, (unsigned)__popcll(pm));
       if (newton) atomicAdd(&newton_wg, (unsigned)newton);
@@ -1044,8 +1099,8 @@ __global__ void __launch_bounds__(256) k_closure_eval(const SceneDev* __restrict
 // ------------------------------------------------------------------ k_p2g
 // Work item = (particle block of the new sort, channel group); one wavefront
 // per item, lane = cell of the block (4x4x4 bins):
-//   0: mass (u64 fixed point) + latent 6 + shear intensity
-//   1: APIC momentum    2: Kirchhoff force    3: latent 0..5
+//   0: latent 6 + shear intensity + mass (sum of weights)
+//   1: APIC momentum    2: Kirchhoff force    3: latent 0..2    4: latent 3..5
 // Each lane walks its cell's particles (block-sorted, so the stencil base is
 // the cell for all of them) and accumulates the 27 stencil nodes x the group's
 // channels in registers.  The 64 lanes' stencils then merge into the block's
@@ -1053,13 +1108,18 @@ __global__ void __launch_bounds__(256) k_closure_eval(const SceneDev* __restrict
 // every tile node ends with exactly one owning lane; the tile leaves through a
 // small LDS staging area (see p2g_group).  Items are independent, so there is
 // no workgroup-wide phase: many small waves per CU overlap their particle
-// loads with other waves' arithmetic.  Fewer, wider groups evaluate the spline
-// weights and walk the particles fewer times (six 3-channel groups: 146 us,
-// five: 137 us, four: 133 us on the 1M-particle scene).
-constexpr int P2G_GROUPS = 4;
-// Accumulator channels of group G: mass (u64 as two words) + latent 6 + shear |
-// momentum 3 | force 3 | latent 0..5 | (MUSL) momentum 3.
-__host__ __device__ constexpr int p2g_nch(int G) { return G == 0 ? 4 : (G == 3 ? 6 : 3); }
+// loads with other waves' arithmetic.  Five groups of three FP32 channels keep
+// the accumulators at 81 registers, so that three wavefronts fit per SIMD
+// (168 VGPRs); four groups (a six-channel latent group, 250 VGPRs, two
+// wavefronts per SIMD) measured 131 us against 110 us on the 1M-particle
+// scene.
+constexpr int P2G_GROUPS = 5;
+// Accumulator channels of group G (three each): latent 6 + shear + mass |
+// momentum | force | latent 0..2 | latent 3..5 | (MUSL) momentum.  Every
+// channel is an FP32 sum in a fixed order (the particles of a cell in the
+// sort's position order, then the fixed merge and tile order), so it is
+// reproducible bit for bit.
+__host__ __device__ constexpr int p2g_nch(int) { return 3; }
 
 // Per-item inputs shared by the channel-group bodies.
 struct P2GItem {
@@ -1079,7 +1139,7 @@ struct P2GItem {
 // half's output (slot j of the sort).
 template <int G>
 struct P2GLoad {
-  static constexpr int NQ = G == 0 ? 2 : (G == 1 || G == 6 ? 12 : 6);
+  static constexpr int NQ = G == 0 ? 2 : (G == 1 || G == 6 ? 12 : (G == 2 ? 6 : 3));
   float x[3];
   float q[NQ > 0 ? NQ : 1];
 };
@@ -1114,9 +1174,9 @@ __device__ __forceinline__ void p2g_load(const float* __restrict__ P, const floa
   } else if constexpr (G == 2) {
 #pragma unroll
     for (int c = 0; c < 6; ++c) o.q[c] = ld_slot(P, cap, P_TAU + c, src);
-  } else if constexpr (G == 3) {
+  } else if constexpr (G == 3 || G == 4) {
 #pragma unroll
-    for (int c = 0; c < 6; ++c) o.q[c] = ld_slot(P, cap, P_PIG + c, src);
+    for (int c = 0; c < 3; ++c) o.q[c] = ld_slot(P, cap, P_PIG + 3 * (G - 3) + c, src);
   } else if constexpr (G == 0) {
     o.q[0] = ld_slot(P, cap, P_PIG + 6, src);
     o.q[1] = ld_slot(P, cap, P_DIFF, src);
@@ -1184,26 +1244,10 @@ __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[2
 #pragma unroll
     for (int o = 0; o < 3; ++o) N[o] = act ? N[o] : 0.0f;
     if constexpr (G == 0) {
-      // t = w 2^20 with the power of two folded into the x weights: exact for
-      // every weight whose fixed-point value is nonzero (an intermediate
-      // product below the normal range means w < 2^-126, which rounds to 0
-      // either way).  The reference's cut w > 1e-25 needs no test here: such
-      // weights give hi = 0 and lo = rn(< 2^-40) = 0.
-      float Nx20[3];
-#pragma unroll
-      for (int o = 0; o < 3; ++o) Nx20[o] = N[o] * 1048576.0f;
-#pragma unroll
-      for (int n = 0; n < 27; ++n) {
-        const float t = (Nx20[n / 9] * N[3 + (n / 3) % 3]) * N[6 + n % 3];
-        // 64-bit add as add with carry: low word, then high word + carry
-        unsigned hi, lo;
-        mass_fx_scaled(t, hi, lo);
-        const unsigned lo_acc = __float_as_uint(acc[n][0]) + lo;
-        const unsigned hi_acc = __float_as_uint(acc[n][1]) + hi + (lo_acc < lo ? 1u : 0u);
-        acc[n][0] = __uint_as_float(lo_acc);
-        acc[n][1] = __uint_as_float(hi_acc);
-      }
-      // latent 6 and shear intensity: weighted scalars, q * Nx folded per offset
+      // latent 6 and shear intensity: weighted scalars, q * Nx folded per
+      // offset; mass: the weight in the reference's product order (Nx Ny) Nz
+      // and with its cut w > 1e-25 (p2g.cpp:36-43), which decides whether a
+      // node becomes active
       float fq[2][3];
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
@@ -1214,8 +1258,10 @@ __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[2
 #pragma unroll
       for (int n = 0; n < 27; ++n) {
         const float nyz = N[3 + (n / 3) % 3] * N[6 + n % 3];
-        acc[n][2] += fq[0][n / 9] * nyz;
-        acc[n][3] += fq[1][n / 9] * nyz;
+        acc[n][0] += fq[0][n / 9] * nyz;
+        acc[n][1] += fq[1][n / 9] * nyz;
+        const float w = (N[n / 9] * N[3 + (n / 3) % 3]) * N[6 + n % 3];
+        acc[n][2] += w > 1e-25f ? w : 0.0f;
       }
     } else {
       float NyNz[9];
@@ -1375,7 +1421,7 @@ __device__ __forceinline__ bool slot_node(int xs, int ys, int zs, int ci, int cj
 template <int G>
 __device__ __forceinline__ void p2g_group(const P2GItem& it, const DevPtrs& d, const SceneDev& sc, int e, int B0i,
                                           int B0j, int B0k, int ci, int cj, int ck, unsigned& touched,
-                                          unsigned long long* stage, const int (&rd)[4]) {
+                                          float* stage_f, const int (&rd)[4]) {
   const bool tiled = e < d.tile_cap;
   const int lane = threadIdx.x;
   float acc[27][p2g_nch(G)];
@@ -1384,49 +1430,27 @@ __device__ __forceinline__ void p2g_group(const P2GItem& it, const DevPtrs& d, c
 #pragma unroll
     for (int c = 0; c < p2g_nch(G); ++c) acc[n][c] = 0.0f;
   p2g_accumulate<G>(it, acc);
-  if constexpr (G == 0) {
-    unsigned long long m[27];
-#pragma unroll
-    for (int n = 0; n < 27; ++n)
-      m[n] = ((unsigned long long)__float_as_uint(acc[n][1]) << 32) | (unsigned long long)__float_as_uint(acc[n][0]);
-    p2g_merge(m, ci, cj, ck);
-    if (tiled) {
-#pragma unroll
-      for (int n = 0; n < 27; ++n) stage[n * 64 + lane] = m[n];
-    }
-#pragma unroll
-    for (int n = 0; n < 27; ++n) {
-      int tx, ty, tz;
-      const bool own = slot_node(n / 9, (n / 3) % 3, n % 3, ci, cj, ck, tx, ty, tz);
-      if (own && m[n] != 0ull) {
-        if (!tiled) {
-          const int gi = B0i + tx, gj = B0j + ty, gk = B0k + tz;
-          atomicAdd(&d.massfx[(size_t)node_block_of(sc, gi, gj, gk) * BLK3 + node_local(gi, gj, gk)], m[n]);
-        }
-        touched |= 1u << ((((tx >> 2) * 2) + (ty >> 2)) * 2 + (tz >> 2));
-      }
-    }
-    if (tiled) {
-      __syncthreads();
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-        if (lane + 64 * r < PT3) d.tile_mass[(size_t)e * PT3 + lane + 64 * r] = stage[rd[r]];
-      __syncthreads();
-    }
-  }
   {
-    // float channels: acc channel c goes to grid channel ch0 + c (group 0's
-    // channels 2, 3 are latent 6 and shear: A_PIG + 6, A_SHR = A_PIG + 7)
-    constexpr int CF = G == 0 ? 2 : 0;
-    constexpr int ch0 = (G == 1 || G == 6) ? A_MOM : (G == 2 ? A_FRC : (G == 3 ? A_PIG : A_PIG + 6 - CF));
+    // acc channel c goes to grid channel ch0 + c (group 0: latent 6, shear,
+    // mass = A_PIG + 6, A_SHR, A_MASS)
+    constexpr int ch0 = (G == 1 || G == 6) ? A_MOM : (G == 2 ? A_FRC : (G == 0 ? A_PIG + 6 : A_PIG + 3 * (G - 3)));
+    static_assert(A_SHR == A_PIG + 7 && A_MASS == A_SHR + 1, "group 0's channels are contiguous");
     constexpr int nch = p2g_nch(G);
-    float* stage_f = reinterpret_cast<float*>(stage);
 #pragma unroll
-    for (int c = CF; c < nch; ++c) {
+    for (int c = 0; c < nch; ++c) {
       float v[27];
 #pragma unroll
       for (int n = 0; n < 27; ++n) v[n] = acc[n][c];
       p2g_merge(v, ci, cj, ck);
+      if (G == 0 && c == 2) {
+        // node blocks receiving mass from this block
+#pragma unroll
+        for (int n = 0; n < 27; ++n) {
+          int tx, ty, tz;
+          if (slot_node(n / 9, (n / 3) % 3, n % 3, ci, cj, ck, tx, ty, tz) && v[n] != 0.0f)
+            touched |= 1u << ((((tx >> 2) * 2) + (ty >> 2)) * 2 + (tz >> 2));
+        }
+      }
       if (tiled) {
 #pragma unroll
         for (int n = 0; n < 27; ++n) stage_f[n * 64 + lane] = v[n];
@@ -1453,14 +1477,13 @@ __device__ __forceinline__ void p2g_group(const P2GItem& it, const DevPtrs& d, c
 
 // MUSL: only the momentum of group 6 (MUSL's second scatter), no activation.
 template <bool MUSL>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_p2g(DevPtrs d, const SceneDev* __restrict__ scenes,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_p2g(DevPtrs d, const SceneDev* __restrict__ scenes,
                                             const float* __restrict__ axis, const float* __restrict__ P,
                                             const int* __restrict__ perm, const int4* __restrict__ pdesc,
                                             const int* __restrict__ cell_num, const float* __restrict__ PB) {
   constexpr int NG = MUSL ? 1 : P2G_GROUPS;
   // tile staging area of this workgroup's one wavefront: 27 slots x 64 lanes
-  // (8-byte words for the mass group, 4-byte for the others)
-  __shared__ unsigned long long stage[MUSL ? 27 * 32 : 27 * 64];
+  __shared__ float stage[27 * 64];
   const int nblk = d.counters[CNT_PBLOCKS];
   const int parity = d.counters[CNT_GSTEP] & 1;
   const int lane = threadIdx.x;
@@ -1522,7 +1545,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
         case 0: p2g_group<0>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched, stage, rd); break;
         case 1: p2g_group<1>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched, stage, rd); break;
         case 2: p2g_group<2>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched, stage, rd); break;
-        default: p2g_group<3>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched, stage, rd); break;
+        case 3: p2g_group<3>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched, stage, rd); break;
+        default: p2g_group<4>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched, stage, rd); break;
       }
     }
     if (group == 0) {
@@ -1567,7 +1591,6 @@ __global__ void __launch_bounds__(256) k_grid(DevPtrs d) {
     // stencils reach it (the block itself and its lower neighbours).  When
     // every listed block had a tile, nothing was scattered: the accumulators
     // are still zero and are neither read nor cleared.
-    unsigned long long mfx = scattered ? d.massfx[(size_t)nb * BLK3 + lane] : 0ull;
     float a[ACC_CH];
 #pragma unroll
     for (int c = 0; c < ACC_CH; ++c) a[c] = scattered ? acc[c * BLK3 + lane] : 0.0f;
@@ -1590,15 +1613,14 @@ __global__ void __launch_bounds__(256) k_grid(DevPtrs d) {
       const int tx = li + 4 * ox, ty = lj + 4 * oy, tz = lk + 4 * oz;
       if (tx >= PT || ty >= PT || tz >= PT) continue;
       const int t = tile_pos(tx, ty, tz);   // this block's group: one contiguous run
-      mfx += d.tile_mass[(size_t)e * PT3 + t];
 #pragma unroll
       for (int c = 0; c < ACC_CH; ++c) a[c] += d.tile_acc[((size_t)e * ACC_CH + c) * PT3 + t];
     }
     float out[GV_STRIDE];
 #pragma unroll
     for (int c = 0; c < GV_STRIDE; ++c) out[c] = 0.0f;
-    if (mfx != 0ull && gi < sc.dims[0] && gj < sc.dims[1] && gk < sc.dims[2]) {
-      const float mw = (float)((double)mfx * MASS_FX_INV);  // sum of weights (= m_i / m_p)
+    const float mw = a[A_MASS];   // sum of weights (= m_i / m_p, p2g.cpp:94)
+    if (mw != 0.0f && gi < sc.dims[0] && gj < sc.dims[1] && gk < sc.dims[2]) {
       float vold[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) vold[c] = a[A_MOM + c] / mw;
@@ -1627,7 +1649,7 @@ __global__ void __launch_bounds__(256) k_grid(DevPtrs d) {
       out[G_SHR] = shear;
       out[G_MASS] = mass;
     }
-    const uint64_t act = __ballot(mfx != 0ull);
+    const uint64_t act = __ballot(mw != 0.0f);
     if (lane == 0 && act) atomicAdd(&active_wg, (unsigned)__popcll(act));
     float4* g = reinterpret_cast<float4*>(d.gv + ((size_t)nb * BLK3 + lane) * GV_STRIDE);
     g[0] = make_float4(out[0], out[1], out[2], out[3]);
@@ -1637,7 +1659,6 @@ __global__ void __launch_bounds__(256) k_grid(DevPtrs d) {
     if (scattered) {
 #pragma unroll
       for (int c = 0; c < ACC_CH; ++c) acc[c * BLK3 + lane] = 0.0f;
-      d.massfx[(size_t)nb * BLK3 + lane] = 0ull;
     }
     if (lane == 0) d.nb_flag[nb] = 0;
   }
@@ -1793,10 +1814,11 @@ int scan_chunks(int nkeys) { return (nkeys + SCAN_CHUNK - 1) / SCAN_CHUNK; }
 void launch_cellscan(const DevPtrs& d, int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_cellscan, dim3(grid), dim3(256), 0, st, d);
 }
-void launch_perm(const DevPtrs& d, int64_t n, int grid, hipStream_t st) {
+void launch_perm(const DevPtrs& d, int64_t n, int grid, int src_is_b, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_perm, dim3(blocks_for(n, 256)), dim3(256), 0, st, d, n);
-  hipLaunchKernelGGL(k_cellsort, dim3(grid), dim3(256), 0, st, d);
+  hipLaunchKernelGGL(k_cellsort, dim3(grid), dim3(256), 0, st, d, (const float*)(src_is_b ? d.pb : d.pa),
+                     (const int*)(src_is_b ? d.pb_id : d.pa_id));
 }
 void launch_g2p(const DevPtrs& d, int grid, int learned, int phase, hipStream_t st) {
   const SceneDev* sc = d.scenes;

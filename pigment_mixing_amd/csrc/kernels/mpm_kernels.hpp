@@ -50,6 +50,7 @@ struct DevPtrs {
   int* key_out;
   int* rank_out;
   int* cell_out;           // cell (0..63) of the particle in its block, -1 if dead
+  int* okey_out;           // order key inside the cell (cell_order_key), live particles
   int* perm;
   int* kslot;              // cell-major position (block start + cell offset + rank) -> rank-major slot
   int* cmaj;               // live particles at cell-major positions by counting rank (k_perm -> k_cellsort)
@@ -58,7 +59,6 @@ struct DevPtrs {
   int* cell_num;           // per (listed block, cell) counts of the current sort, list order
   int4* pdesc;             // per listed block: key, first slot, particle count, scene
   float* tile_acc;         // per listed block (< tile_cap): merged 6x6x6 node tile, [ACC_CH][216]
-  unsigned long long* tile_mass;  // per listed block (< tile_cap): [216] fixed-point mass
   int* list_pos;           // block key -> position in pb_list (listed blocks only)
   int* count_next;
   int* count_cur;
@@ -67,7 +67,6 @@ struct DevPtrs {
   unsigned long long* scan_partials;
   const int* blk_scene;
   float* acc;
-  unsigned long long* massfx;
   float* gv;
   int* nb_flag;
   int* nb_list;
@@ -135,8 +134,9 @@ void launch_keys(const DevPtrs& d, int64_t n, hipStream_t st);
 void launch_scan(const DevPtrs& d, int nkeys, int advance, hipStream_t st);
 int scan_chunks(int nkeys);
 void launch_cellscan(const DevPtrs& d, int grid, hipStream_t st);
-// k_perm + k_cellsort (grid: workgroups of the per-block pass)
-void launch_perm(const DevPtrs& d, int64_t n, int grid, hipStream_t st);
+// k_perm + k_cellsort (grid: workgroups of the per-block pass; src_is_b: the
+// sorted slots index the B side's rows and ids, else the A side's)
+void launch_perm(const DevPtrs& d, int64_t n, int grid, int src_is_b, hipStream_t st);
 // phase 0: fused USL G2P (+ dead-particle copy); MUSL: phase 1 (first half),
 // launch_musl (second scatter + grid velocity), phase 2 (second half + copy).
 // learned (phases 0, 2): queue the closure's candidates for launch_closure.

@@ -2,7 +2,7 @@
 same kernels and graph) and of the launch-shape autotuner.
 
 A scene inside a batch must behave bit for bit as the same scene alone: the
-sort orders the particles of a cell by their previous slot (k_cellsort), so the
+sort orders the particles of a cell by position (k_cellsort), so the
 scatter's FP32 sums run in the same order whatever the other scenes of the
 batch, the atomic timing or the launch shapes are.  One batched scene is also
 checked against the CPU oracle at the parity bars (x <= 1e-4 dx, v <= 1e-4
