@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -914,8 +915,11 @@ int mpmx_prepare(mpmx_handle* h) {
   ALLOC(d.cell_num, int, (size_t)h->nkeys * BLK3);
   ALLOC(d.pdesc, int4, h->nkeys);
   // merged P2G tiles: room for the listed blocks of dense paint (~355 particles
-  // per block); blocks beyond it scatter with atomics
-  d.tile_cap = (int)std::min<int64_t>(h->nkeys, std::max<int64_t>(1024, n / 128));
+  // per block); blocks beyond it scatter with atomics.  MPMX_TILE_CAP lowers
+  // the count (tests of the atomic path on small scenes).
+  int64_t tiles = std::max<int64_t>(1024, n / 128);
+  if (const char* env = std::getenv("MPMX_TILE_CAP")) tiles = std::max<int64_t>(1, std::atoll(env));
+  d.tile_cap = (int)std::min<int64_t>(h->nkeys, tiles);
   ALLOC(d.tile_acc, float, (size_t)d.tile_cap * ACC_CH * PT3);
   ALLOC(d.list_pos, int, h->nkeys);
   if (learned(h)) {

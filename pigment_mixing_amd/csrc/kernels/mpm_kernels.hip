@@ -249,10 +249,12 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_down(DevPtrs d, int nkeys
     if (k < nkeys) {
       d.offsets[k] = (int)(run >> 32);
       d.count_cur[k] = (int)(items[j] >> 32);
+      // list position of every key (INT_MAX for an unlisted one), so that a
+      // lookup needs no count check first
+      d.list_pos[k] = (items[j] & 1ull) ? (int)(run & 0xffffffffull) : 0x7fffffff;
       if (items[j] & 1ull) {
         const int e = (int)(run & 0xffffffffull);
         d.pb_list[e] = k;
-        d.list_pos[k] = e;
         d.pdesc[e] = make_int4(k, (int)(run >> 32), (int)(items[j] >> 32), d.blk_scene[k]);
       }
       d.count_next[k] = 0;
@@ -358,9 +360,13 @@ __global__ void k_perm(DevPtrs d, int64_t n) {
 // left, of the other scenes of a batch and of the launch shapes.  Positions
 // and ids come from the side the sort orders.  Ranks are a permutation of
 // 0..c-1; rank r of the cell goes to its rank-major slot kslot[cell start +
-// r].  Cells of up to CS_LDS particles are ranked from an LDS column of keys
-// per lane, larger ones from global memory (same order).
-constexpr int CS_LDS = 32;
+// r].  One 256-thread workgroup per listed block (grid-stride over the list;
+// the launch gives most blocks a workgroup of their own), one thread per particle:
+// the block's order keys and slots are staged in LDS in cell-major order
+// (blocks of up to CS_CAP particles; larger ones read them from global
+// memory, same order), and each thread ranks its particle against the others
+// of its cell.
+constexpr int CS_CAP = 2048;
 // float -> unsigned with the same order (negative values reversed below the
 // positive ones); -0 sorts just below +0
 __device__ __forceinline__ uint32_t ord_key(float f) {
@@ -378,49 +384,51 @@ __device__ __forceinline__ bool pos_before(const float* __restrict__ X, const in
   return a != b && ids[a] < ids[b];
 }
 __global__ void __launch_bounds__(256) k_cellsort(DevPtrs d, const float* __restrict__ X, const int* __restrict__ ids) {
-  __shared__ uint32_t col[4][CS_LDS][64];
+  __shared__ uint32_t skey[CS_CAP];
+  __shared__ int sslot[CS_CAP];
+  __shared__ int coff[BLK3 + 1];
   const int nblk = d.counters[CNT_PBLOCKS];
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int nwaves = (gridDim.x * blockDim.x) >> 6;
-  const int w = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
   const int64_t cap = d.cap;
-  for (int e = wave; e < nblk; e += nwaves) {
-    const int c = d.cell_num[e * BLK3 + lane];
-    const int base = d.pdesc[e].y + d.cell_off[e * BLK3 + lane];
-    int cmax = c;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) cmax = max(cmax, __shfl_xor(cmax, off));
-    if (cmax <= CS_LDS) {
-      for (int r = 0; r < c; ++r) col[w][r][lane] = (uint32_t)d.okey_out[d.cmaj[base + r]];
-      for (int r = 0; r < c; ++r) {
-        const uint32_t kr = col[w][r][lane];
-        int rk = 0;
-        bool tie = false;
-        for (int s = 0; s < c; ++s) {
-          const uint32_t ks = col[w][s][lane];
-          rk += ks < kr ? 1 : 0;
-          tie |= s != r && ks == kr;
-        }
-        const int v = d.cmaj[base + r];
-        if (tie) {   // equal order keys (particles within 1/1024 cell of each other): full positions decide
-          for (int s = 0; s < c; ++s)
-            if (s != r && col[w][s][lane] == kr) rk += pos_before(X, ids, cap, d.cmaj[base + s], v) ? 1 : 0;
-        }
-        d.perm[d.kslot[base + rk]] = v;
+  for (int e = blockIdx.x; e < nblk; e += gridDim.x) {
+    const int4 ds = d.pdesc[e];
+    const int start = ds.y, cnt = ds.z;
+    __syncthreads();   // the previous block is done with the LDS
+    if (threadIdx.x < BLK3) coff[threadIdx.x] = d.cell_off[e * BLK3 + threadIdx.x];
+    if (threadIdx.x == 0) coff[BLK3] = cnt;
+    const bool staged = cnt <= CS_CAP;
+    if (staged) {
+      for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+        const int v = d.cmaj[start + t];
+        sslot[t] = v;
+        skey[t] = (uint32_t)d.okey_out[v];
       }
-    } else {
-      for (int r = 0; r < c; ++r) {
-        const int v = d.cmaj[base + r];
-        const uint32_t kr = (uint32_t)d.okey_out[v];
-        int rk = 0;
-        for (int s = 0; s < c; ++s) {
-          const int vs = d.cmaj[base + s];
-          const uint32_t ks = (uint32_t)d.okey_out[vs];
-          rk += (ks < kr || (ks == kr && pos_before(X, ids, cap, vs, v))) ? 1 : 0;
-        }
-        d.perm[d.kslot[base + rk]] = v;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+      // this position's cell: the last c with coff[c] <= t (empty cells share offsets)
+      int lo = 0, hi = BLK3;   // coff[lo] <= t < coff[hi]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (coff[mid] <= t) lo = mid; else hi = mid;
       }
+      const int c0 = coff[lo], c1 = coff[lo + 1];
+      const int v = staged ? sslot[t] : d.cmaj[start + t];
+      const uint32_t kr = staged ? skey[t] : (uint32_t)d.okey_out[v];
+      int rk = 0;
+      bool tie = false;
+      for (int s2 = c0; s2 < c1; ++s2) {
+        const uint32_t ks = staged ? skey[s2] : (uint32_t)d.okey_out[d.cmaj[start + s2]];
+        rk += ks < kr ? 1 : 0;
+        tie |= s2 != t && ks == kr;
+      }
+      if (tie) {   // equal order keys (particles within 1/1024 of a cell of each other): full positions decide
+        for (int s2 = c0; s2 < c1; ++s2) {
+          const int vs = staged ? sslot[s2] : d.cmaj[start + s2];
+          const uint32_t ks = staged ? skey[s2] : (uint32_t)d.okey_out[vs];
+          if (s2 != t && ks == kr) rk += pos_before(X, ids, cap, vs, v) ? 1 : 0;
+        }
+      }
+      d.perm[d.kslot[start + c0 + rk]] = v;
     }
   }
 }
@@ -1603,7 +1611,7 @@ __global__ void __launch_bounds__(256) k_grid(DevPtrs d) {
       const int pi = bi - ox, pj = bj - oy, pk = bk - oz;
       const int key = sc.node_block_base + (pi * sc.bdims[1] + pj) * sc.bdims[2] + pk;
       const bool in = pi >= 0 && pj >= 0 && pk >= 0;
-      te[q] = in && d.count_cur[in ? key : 0] > 0 ? d.list_pos[key] : d.tile_cap;
+      te[q] = in ? d.list_pos[key] : d.tile_cap;
     }
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -1693,7 +1701,6 @@ __global__ void __launch_bounds__(256) k_grid_musl(DevPtrs d) {
       const int pi = bi - ox, pj = bj - oy, pk = bk - oz;
       if (pi < 0 || pj < 0 || pk < 0) continue;
       const int key = sc.node_block_base + (pi * sc.bdims[1] + pj) * sc.bdims[2] + pk;
-      if (d.count_cur[key] <= 0) continue;
       const int te = d.list_pos[key];
       if (te >= d.tile_cap) continue;
       const int tx = li + 4 * ox, ty = lj + 4 * oy, tz = lk + 4 * oz;
@@ -1817,7 +1824,7 @@ void launch_cellscan(const DevPtrs& d, int grid, hipStream_t st) {
 void launch_perm(const DevPtrs& d, int64_t n, int grid, int src_is_b, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_perm, dim3(blocks_for(n, 256)), dim3(256), 0, st, d, n);
-  hipLaunchKernelGGL(k_cellsort, dim3(grid), dim3(256), 0, st, d, (const float*)(src_is_b ? d.pb : d.pa),
+  hipLaunchKernelGGL(k_cellsort, dim3(grid * 4), dim3(256), 0, st, d, (const float*)(src_is_b ? d.pb : d.pa),
                      (const int*)(src_is_b ? d.pb_id : d.pa_id));
 }
 void launch_g2p(const DevPtrs& d, int grid, int learned, int phase, hipStream_t st) {

@@ -46,8 +46,8 @@ def small(yield_pa=50.0, knife=True, cells=(6, 4, 6), dims=(24, 16, 24)):
 
 
 def mass_quantum(s):
-    # 27 contributions per node at most per particle cell neighbourhood, each
-    # rounded to 2^-52 of the particle mass
+    # absolute floor of node-mass comparisons: far below any weight a node can
+    # receive above the reference's cut w > 1e-25 in FP32 (216 x 2^-52 m_p)
     mp = s["cfg"].rho * s["cfg"].dx ** 3 / s["cfg"].ppc
     return 216 * mp * 2.0 ** -52
 
@@ -138,6 +138,30 @@ def test_parity_small_elastic_knife_100_steps():
     s = small()
     h, o = run_both(s, 100)
     check_particles(h.download(), o.download(), scenes.DX, "small/knife/100")
+
+
+@pytest.mark.parametrize("musl", [0, 1])
+def test_parity_atomic_scatter_path(monkeypatch, musl):
+    """Particle blocks past the merged-tile capacity add their P2G sums to the
+    grid with atomics and the grid kernel reads and clears them (one
+    wavefront per node block): with the capacity lowered to 2 tiles, the
+    51,200-particle knife scene (USL and MUSL) still follows the oracle at the
+    parity bars after 50 steps, and its node masses match before stepping."""
+    monkeypatch.setenv("MPMX_TILE_CAP", "2")
+    s = scenes.paint_2blob()
+    s["cfg"].use_musl = musl
+    h = scenes.make_handle(s)
+    o = orc.from_scene(s)
+    o.step(1)
+    gg, go = h.grid(), o.grid()
+    np.testing.assert_array_equal(gg["mass"] > 0, go["mass"] > 0)
+    m = go["mass"] > 0
+    np.testing.assert_allclose(gg["mass"][m], go["mass"][m], rtol=4e-6, atol=mass_quantum(s))
+    h.step(50)
+    o.step(49)
+    st = h.stats()
+    assert st.device_error == 0 and st.n_active == s["x"].shape[0]
+    check_particles(h.download(), o.download(), scenes.DX, f"atomic path/musl={musl}/50")
 
 
 def test_parity_small_plastic_200_steps():
