@@ -230,12 +230,12 @@ def principal_tau(hp, E, nu):
 
 def checkpoint_with_closure(src, dst, mode, tol):
     """Copy of a checkpoint file whose stored configuration selects the
-    learned closure: the file holds "MPMXCKP2", u32 sizeof(mpmx_config), then
+    learned closure: the file holds "MPMXCKP3", u32 sizeof(mpmx_config), then
     the mpmx_config (include/mpmx.h), whose closure fields are rewritten."""
     import struct
     from pigment_mixing_amd import abi
     data = bytearray(open(src, "rb").read())
-    assert data[:8] == b"MPMXCKP2" and struct.unpack_from("<I", data, 8)[0] == __import__("ctypes").sizeof(abi.Config)
+    assert data[:8] == b"MPMXCKP3" and struct.unpack_from("<I", data, 8)[0] == __import__("ctypes").sizeof(abi.Config)
     struct.pack_into("<i", data, 12 + abi.Config.closure.offset, mode)
     struct.pack_into("<f", data, 12 + abi.Config.closure_tol.offset, tol)
     with open(dst, "wb") as f:

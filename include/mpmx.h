@@ -254,6 +254,19 @@ int mpmx_set_knife_state(mpmx_handle* h, int k, const mpmx_knife_state* s);
 /* Knife state that step `step` uses (host kinematics; for tests/renderer). */
 int mpmx_knife_state_at(mpmx_handle* h, int k, int64_t step, mpmx_knife_state* out);
 
+/* Simulation::updateDt (update_dt.cpp:5-71) as an option: the caller computes
+ * dt = min(cfl dx / max speed, dt_max), clamped to the frame end and min_dt,
+ * from mpmx_max_speed and sets it with mpmx_set_dt before the next step.
+ *   mpmx_max_speed: largest |v| of the live particles (all scenes of a batch),
+ *     a device reduction read back to the host (synchronises the stream).
+ *   mpmx_set_dt: the step of every scene from the next step on (a batch
+ *     shares one dt); the derived constants follow, the knife's clock and the
+ *     plates' positions continue from the current step (re-based there with
+ *     the old dt), and the captured step graph stays valid (it reads the
+ *     scene constants from device memory).  Valid before and after prepare. */
+int mpmx_max_speed(mpmx_handle* h, float* vmax);
+int mpmx_set_dt(mpmx_handle* h, float dt);
+
 /* Same kinematics without a handle (no device needed): knife `k`, optional
  * keyframes (n_frames x {pos[3], quat_xyzw[4]} at fps), time step dt. */
 int mpmx_knife_state_eval(const mpmx_knife* k, const float* frames7, int n_frames, float fps, double dt,

@@ -1152,6 +1152,17 @@ int orc_prepare(void* h) {
   return 0;
 }
 
+// Step of the following steps (the caller's updateDt, update_dt.cpp:5-71).
+// Only for scenes without a moving knife or moving plates: this restatement
+// evaluates their kinematics at step * dt.
+int orc_set_dt(void* h, float dt) {
+  Oracle* o = (Oracle*)h;
+  if (!(dt > 0.0f)) return MPMX_ERR_ARG;
+  o->cfg.dt = dt;
+  if (o->prepared) o->derived();
+  return 0;
+}
+
 int orc_step(void* h, int n) {
   Oracle* o = (Oracle*)h;
   if (!o->prepared) return MPMX_ERR_STATE;

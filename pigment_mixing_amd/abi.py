@@ -159,6 +159,8 @@ def _sig(lib):
         ("mpmx_set_knife_keyframes", ctypes.c_int, [H, ctypes.c_int, c_float_p, ctypes.c_int, ctypes.c_float]),
         ("mpmx_set_knife_state", ctypes.c_int, [H, ctypes.c_int, ctypes.POINTER(KnifeState)]),
         ("mpmx_knife_state_at", ctypes.c_int, [H, ctypes.c_int, ctypes.c_int64, ctypes.POINTER(KnifeState)]),
+        ("mpmx_max_speed", ctypes.c_int, [H, c_float_p]),
+        ("mpmx_set_dt", ctypes.c_int, [H, ctypes.c_float]),
         ("mpmx_prepare", ctypes.c_int, [H]),
         ("mpmx_get_lattice", ctypes.c_int, [H, c_int_p, c_float_p, c_float_p, c_float_p, c_float_p]),
         ("mpmx_step", ctypes.c_int, [H, ctypes.c_int]),
@@ -605,6 +607,16 @@ class Handle:
         kt = KernelTimes()
         self._chk(self.lib.mpmx_profile_steps(self.h, nsteps, ctypes.byref(kt)), "profile")
         return kt
+
+    def max_speed(self):
+        """Largest |v| of the live particles (mpmx_max_speed)."""
+        v = ctypes.c_float(0.0)
+        self._chk(self.lib.mpmx_max_speed(self.h, ctypes.byref(v)), "max_speed")
+        return v.value
+
+    def set_dt(self, dt):
+        """Step of the following steps (mpmx_set_dt; objects continue from the current step)."""
+        self._chk(self.lib.mpmx_set_dt(self.h, float(dt)), "set_dt")
 
     def knife_state(self, step):
         ks = KnifeState()

@@ -120,8 +120,15 @@ Pose knife_pose(const mpmx_knife& k, const std::vector<float>& frames7, float fp
 
 mpmx_knife_state knife_state_at(const mpmx_knife& k, const std::vector<float>& frames7, float fps, double dt,
                                 int64_t step) {
-  const Pose P1 = knife_pose(k, frames7, fps, (double)step * dt);
-  const Pose P0 = knife_pose(k, frames7, fps, (double)(step - 1) * dt);
+  return knife_state_at_time(k, frames7, fps, (double)step * dt, (double)(step - 1) * dt, dt);
+}
+
+// The state during the step that ends at time t1 (pose at t1, velocities from
+// the pose at t0 = t1 - dt).
+mpmx_knife_state knife_state_at_time(const mpmx_knife& k, const std::vector<float>& frames7, float fps, double t1,
+                                     double t0, double dt) {
+  const Pose P1 = knife_pose(k, frames7, fps, t1);
+  const Pose P0 = knife_pose(k, frames7, fps, t0);
   mpmx_knife_state s;
   for (int i = 0; i < 3; ++i) {
     for (int j = 0; j < 3; ++j) {

@@ -19,5 +19,7 @@ struct Pose {
 Pose knife_pose(const mpmx_knife& k, const std::vector<float>& frames7, float fps, double t);
 mpmx_knife_state knife_state_at(const mpmx_knife& k, const std::vector<float>& frames7, float fps, double dt,
                                 int64_t step);
+mpmx_knife_state knife_state_at_time(const mpmx_knife& k, const std::vector<float>& frames7, float fps, double t1,
+                                     double t0, double dt);
 
 }  // namespace mpmx

@@ -35,9 +35,14 @@ struct PlateRun {
 struct Scene {
   mpmx_config cfg;
   std::vector<mpmx_plate> plates;    // as set (construction state)
-  int64_t plates_step0 = 0;          // step at which they were set
+  std::vector<mpmx_plate> plates_base;  // their state at plates_step0 (moved up to a dt change)
+  int64_t plates_step0 = 0;          // step from which they are replayed
   float plates_time0 = 0.0f;
   PlateRun prun;
+  // the knife's clock: time(step) = knife_t0 + (step - knife_s0) dt (a dt
+  // change re-bases it at the current step)
+  double knife_t0 = 0.0;
+  int64_t knife_s0 = 0;
   bool has_knife = false;
   mpmx_knife knife;
   std::vector<float> frames;
@@ -309,7 +314,7 @@ const std::vector<mpmx_plate>& plates_at(Scene& s, int64_t step) {
   if (r.step < 0 || r.step > step || r.cur.size() != s.plates.size()) {
     r.step = s.plates_step0;
     r.time = s.plates_time0;
-    r.cur = s.plates;
+    r.cur = s.plates_base;
   }
   while (r.step < step) plate_move(s, r);
   return r.cur;
@@ -342,7 +347,9 @@ mpmx_knife_state scene_knife_state(const Scene& s, int64_t step) {
     std::memset(&z, 0, sizeof z);
     return z;
   }
-  return knife_state_at(s.knife, s.frames, s.frame_fps, (double)s.cfg.dt, step);
+  const double dt = (double)s.cfg.dt;
+  return knife_state_at_time(s.knife, s.frames, s.frame_fps, s.knife_t0 + (double)(step - s.knife_s0) * dt,
+                             s.knife_t0 + (double)(step - 1 - s.knife_s0) * dt, dt);
 }
 
 // Upload knife states + step times for global steps [g0, g0 + n] and reset
@@ -808,6 +815,7 @@ int mpmx_set_objects_scene(mpmx_handle* h, int scene, const mpmx_plate* plates, 
     return fail(h, MPMX_ERR_ARG, "set_objects: invalid knife bc/shape");
   Scene& s = h->scenes[scene];
   s.plates.assign(plates, plates + n_plates);
+  s.plates_base = s.plates;
   // plates set on a running handle start from the given state at the current step
   s.plates_step0 = h->prepared ? h->step : 0;
   s.plates_time0 = h->prepared ? h->time_f : 0.0f;
@@ -841,6 +849,71 @@ int mpmx_set_knife_state(mpmx_handle* h, int k, const mpmx_knife_state* st) {
 int mpmx_knife_state_at(mpmx_handle* h, int k, int64_t step, mpmx_knife_state* out) {
   if (!h || k < 0 || k >= (int)h->scenes.size() || !out) return MPMX_ERR_ARG;
   *out = scene_knife_state(h->scenes[k], step);
+  return MPMX_OK;
+}
+
+int mpmx_set_dt(mpmx_handle* h, float dt) {
+  if (!h || !(dt > 0.0f) || !std::isfinite(dt)) return MPMX_ERR_ARG;
+  if (h->scenes.empty() || h->scenes[0].cfg.dt == dt) return MPMX_OK;
+  const int64_t now = h->prepared ? h->step : 0;
+  if (h->prepared) {
+    // the grid velocities the next step gathers were updated at the end of
+    // the last step (its P2G + grid update run ahead of the next G2P); they
+    // are recomputed below with the new dt, from the merged tiles
+    int rc = read_stats(h);
+    if (rc) return rc;
+    if (h->h_counters[CNT_PBLOCKS] > h->d.tile_cap)
+      return fail(h, MPMX_ERR_STATE, "set_dt: the last scatter used the atomic path; its grid cannot be redone");
+  }
+  for (size_t i = 0; i < h->scenes.size(); ++i) {
+    Scene& s = h->scenes[i];
+    // the objects continue from their state at the current step: the knife's
+    // clock and the plates' replay are re-based there with the old dt
+    s.knife_t0 += (double)(now - s.knife_s0) * (double)s.cfg.dt;
+    s.knife_s0 = now;
+    if (h->prepared) {
+      s.plates_base = plates_at(s, now);
+      s.plates_time0 = s.prun.time;
+      s.plates_step0 = now;
+      s.prun = PlateRun{};
+    }
+    s.cfg.dt = dt;
+    if (h->prepared) {
+      fill_scene_dev(s, s.dev.node_block_base, s.dev.axis_off);
+      HIPCHK(h, hipMemcpyAsync(h->d_scenes + i, &s.dev, sizeof(SceneDev), hipMemcpyHostToDevice, h->stream));
+    }
+  }
+  if (h->prepared) {
+    // the knife and plates of the next step as its grid update sees them
+    // (entry 0, step counter 0), then the grid update again
+    int rc = upload_kstates(h, h->step, 0, h->time_f);
+    if (rc) return rc;
+    launch_grid_redo(current(h), grid_grid(h), h->stream);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
+  return MPMX_OK;
+}
+
+int mpmx_max_speed(mpmx_handle* h, float* vmax) {
+  if (!h || !vmax) return MPMX_ERR_ARG;
+  if (!h->prepared) return fail(h, MPMX_ERR_STATE, "max_speed: prepare first");
+  unsigned* d_out = nullptr;
+  HIPCHK(h, hipMalloc(&d_out, sizeof(unsigned)));
+  unsigned bits = 0u;
+  int rc = MPMX_OK;
+  if (hipMemsetAsync(d_out, 0, sizeof(unsigned), h->stream) != hipSuccess) rc = fail(h, MPMX_ERR_HIP, "max_speed");
+  if (rc == MPMX_OK) {
+    launch_max_speed(current(h), h->np, d_out, h->stream);
+    if (hipMemcpyAsync(&bits, d_out, sizeof(unsigned), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+      rc = fail(h, MPMX_ERR_HIP, "max_speed copy failed");
+  }
+  (void)hipFree(d_out);
+  if (rc != MPMX_OK) return rc;
+  float v2;
+  std::memcpy(&v2, &bits, sizeof v2);
+  *vmax = std::sqrt(v2);
   return MPMX_OK;
 }
 
@@ -1142,7 +1215,7 @@ int download_rows(mpmx_handle* h, int first, int ncomp, float* out) {
 //   i32 n_frames; f32 fps; f32[7 n_frames];
 //   f32 x[3n] v[3n] F[9n] C[9n] pig[7n] diff[n] eps_pl_dev[n] delta_gamma[n]
 //   eps_pl_vol[n] eps_pl_vol_pradhana[n] tau[6n]; u8 active[n]   (upload order)
-constexpr char CKPT_MAGIC[8] = {'M', 'P', 'M', 'X', 'C', 'K', 'P', '2'};
+constexpr char CKPT_MAGIC[8] = {'M', 'P', 'M', 'X', 'C', 'K', 'P', '3'};
 
 struct FileCloser {
   FILE* f;
@@ -1179,6 +1252,13 @@ int mpmx_checkpoint_save(mpmx_handle* h, const char* path) {
   put(&np, 4); put(s.plates.data(), sizeof(mpmx_plate) * np);
   put(&hk, 4); if (hk) put(&s.knife, sizeof(mpmx_knife));
   put(&nf, 4); put(&s.frame_fps, 4); put(s.frames.data(), sizeof(float) * s.frames.size());
+  // the objects' clocks (re-based by dt changes)
+  {
+    Scene& sm = h->scenes[0];
+    const std::vector<mpmx_plate>& base = sm.plates_base;
+    put(&sm.knife_t0, 8); put(&sm.knife_s0, 8); put(&sm.plates_step0, 8); put(&sm.plates_time0, 4);
+    put(base.data(), sizeof(mpmx_plate) * np);
+  }
   for (const std::vector<float>* a : {&x, &v, &F, &C, &pig, &diff, &eps, &dg, &epsv, &prad, &tau})
     put(a->data(), sizeof(float) * a->size());
   put(act.data(), act.size());
@@ -1213,6 +1293,12 @@ int mpmx_checkpoint_load(const char* path, mpmx_handle** out) {
   if (!ok || nf < 0 || nf > (1 << 24)) return MPMX_ERR_IO;
   std::vector<float> frames(7 * (size_t)nf);
   get(frames.data(), sizeof(float) * frames.size());
+  double knife_t0 = 0.0;
+  int64_t knife_s0 = 0, plates_step0 = 0;
+  float plates_time0 = 0.0f;
+  std::vector<mpmx_plate> plates_base(np);
+  get(&knife_t0, 8); get(&knife_s0, 8); get(&plates_step0, 8); get(&plates_time0, 4);
+  get(plates_base.data(), sizeof(mpmx_plate) * np);
   std::vector<float> x(3 * n), v(3 * n), F(9 * n), C(9 * n), pig(7 * n), diff(n), eps(n), dg(n), epsv(n), prad(n),
       tau(6 * n);
   std::vector<uint8_t> act(n);
@@ -1230,6 +1316,14 @@ int mpmx_checkpoint_load(const char* path, mpmx_handle** out) {
       (nf > 0 && (rc = mpmx_set_knife_keyframes(h, 0, frames.data(), nf, fps)) != MPMX_OK)) {
     mpmx_destroy(h);
     return rc;
+  }
+  {
+    Scene& sm = h->scenes[0];
+    sm.knife_t0 = knife_t0;
+    sm.knife_s0 = knife_s0;
+    sm.plates_base = plates_base;
+    sm.plates_step0 = plates_step0;
+    sm.plates_time0 = plates_time0;
   }
   for (int64_t p = 0; p < n; ++p) {
     h->h_comp[(size_t)P_DG * n + p] = dg[p];

@@ -233,7 +233,12 @@ class Simulation {
   bool is_initialized = false;
   unsigned int Np = 0;
   T dx = 0.01f;
-  T dt = 0;                       // fixed step; 0: chosen by prepareSimulation
+  T dt = 0;                       // step; 0: chosen by prepareSimulation
+  // Simulation::updateDt (update_dt.cpp:5-71) before every step when set:
+  // dt = min(cfl dx / max |v|, dt_max), clamped to the frame's end and min_dt
+  bool adaptive_dt = false;
+  T cfl = 0.5f, cfl_elastic = 1.0f, min_dt = 1e-14f;
+  T dt_max = 0, frame_dt = 0;     // set by prepareSimulation (simulation_units.cpp:170-174)
   T fps = 60;
   T particle_mass = 0, particle_volume = 0;
   T pigment_D_max = 0, pigment_D_edge0 = 0, pigment_D_edge1 = 0;      // pigment_config.json
@@ -418,6 +423,9 @@ class Simulation {
     }
     frame = 0;
     time_ = 0;
+    time_f_ = 0.0f;
+    dt_max = cfl_elastic * dx / std::sqrt(cfg.E / cfg.rho);
+    frame_dt = 1.0f / fps;
     if (knife) {
       mpmx_knife_state s;
       check(mpmx_knife_state_at(h_, 0, 0, &s), "knife state");
@@ -427,10 +435,12 @@ class Simulation {
 
   // Simulation::step (simulation_units.cpp:217-233): one advanceStep on the device.
   void step() {
+    if (adaptive_dt) updateDt();
     const int rc = mpmx_step(h_, 1);
     if (rc == MPMX_ERR_DEVICE) exit = 1;  // reference exit flag
     else check(rc, "step");
     time_ += cfg.dt;
+    time_f_ += cfg.dt;   // Simulation::time in T = float, as the device counts it
     if (knife) {
       mpmx_knife_state s;
       check(mpmx_knife_state_at(h_, 0, stepsDone(), &s), "knife state");
@@ -509,6 +519,7 @@ class Simulation {
   mpmx_handle* h_ = nullptr;
   T low_[3] = {0, 0, 0}, high_[3] = {0, 0, 0};
   double time_ = 0;
+  float time_f_ = 0.0f;
 
   void pull_pigment_members() {
     pigment_D_max = cfg.D_max;
@@ -525,6 +536,18 @@ class Simulation {
     cfg.start_boost_time = start_boost_time;
     cfg.end_boost_time = end_boost_time;
     cfg.boost_factor = boost_factor;
+  }
+  void updateDt() {
+    float vmax = 0.0f;
+    check(mpmx_max_speed(h_, &vmax), "max_speed");
+    T d = std::fabs(vmax) > 1e-10f ? std::min(cfl * dx / vmax, dt_max) : dt_max;
+    d = std::min(d, frame_dt * (frame + 1) - time_f_);
+    d = std::max(d, min_dt);
+    if (d != cfg.dt) {
+      check(mpmx_set_dt(h_, d), "set_dt");
+      cfg.dt = d;
+      dt = d;
+    }
   }
   T default_dt() const {
     T d = dx / std::sqrt(cfg.E / cfg.rho);

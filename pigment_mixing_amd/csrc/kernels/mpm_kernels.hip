@@ -1572,13 +1572,19 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
 
 // ------------------------------------------------------------------ k_grid
 // One wavefront per active node block; lane = node.
+// REDO: the grid update of the last step again from its merged tiles (which
+// persist until the next scatter), with the scene constants as they are now
+// (a changed dt); the list and statistics bookkeeping of the first pass is
+// left alone.  Only for steps whose scatter used no atomics (the atomic
+// accumulators are cleared by the first pass).
+template <bool REDO>
 __global__ void __launch_bounds__(256) k_grid(DevPtrs d) {
   __shared__ unsigned int active_wg;
   if (threadIdx.x == 0) active_wg = 0u;
   __syncthreads();
   const int step = d.counters[CNT_STEP];
   const int parity = d.counters[CNT_GSTEP] & 1;
-  if (blockIdx.x == 0 && threadIdx.x == 0) d.counters[CNT_NBLOCKS0 + (parity ^ 1)] = 0;
+  if (!REDO && blockIdx.x == 0 && threadIdx.x == 0) d.counters[CNT_NBLOCKS0 + (parity ^ 1)] = 0;
   const int nblk = d.counters[CNT_NBLOCKS0 + parity];
   const bool scattered = d.counters[CNT_PBLOCKS] > d.tile_cap;   // P2G used its atomic path
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -1668,10 +1674,10 @@ __global__ void __launch_bounds__(256) k_grid(DevPtrs d) {
 #pragma unroll
       for (int c = 0; c < ACC_CH; ++c) acc[c * BLK3 + lane] = 0.0f;
     }
-    if (lane == 0) d.nb_flag[nb] = 0;
+    if (!REDO && lane == 0) d.nb_flag[nb] = 0;
   }
   __syncthreads();
-  if (threadIdx.x == 0 && active_wg) atomicAdd(&d.stats[ST_ACTIVE_NODES], (unsigned long long)active_wg);
+  if (!REDO && threadIdx.x == 0 && active_wg) atomicAdd(&d.stats[ST_ACTIVE_NODES], (unsigned long long)active_wg);
 }
 
 // ------------------------------------------------------------------ k_grid_musl
@@ -1732,6 +1738,22 @@ __global__ void k_unsort(DevPtrs d, int64_t n, float* out, int ncomp_first, int 
   if (j >= n) return;
   const int64_t id = d.pa_id[j];
   for (int c = 0; c < ncomp; ++c) out[id * ncomp + c] = d.pa[(ncomp_first + c) * d.cap + j];
+}
+
+// Largest squared speed of the live particles (update_dt.cpp:7-15) as the
+// bits of a non-negative float, whose order is that of unsigned integers.
+__global__ void k_max_speed(DevPtrs d, int64_t n, unsigned* out) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float v2 = 0.0f;
+  if (j < n && d.cell_out[j] >= 0) {
+    const float a = d.pa[(P_V + 0) * d.cap + j], b = d.pa[(P_V + 1) * d.cap + j], c = d.pa[(P_V + 2) * d.cap + j];
+    v2 = (a * a + b * b) + c * c;
+    if (!(v2 >= 0.0f)) v2 = 0.0f;   // NaN: not a speed
+  }
+  unsigned m = __float_as_uint(v2);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off));
+  if ((threadIdx.x & 63) == 0 && m) atomicMax(out, m);
 }
 
 // Active flag of the particle in row slot j: the sort's record of its new
@@ -1872,7 +1894,13 @@ void launch_musl(const DevPtrs& d, int grid_p2g, int grid_grid, hipStream_t st) 
   hipLaunchKernelGGL(k_grid_musl, dim3(grid_grid), dim3(256), 0, st, d);
 }
 void launch_grid(const DevPtrs& d, int grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_grid, dim3(grid), dim3(256), 0, st, d);
+  hipLaunchKernelGGL(k_grid<false>, dim3(grid), dim3(256), 0, st, d);
+}
+void launch_grid_redo(const DevPtrs& d, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_grid<true>, dim3(grid), dim3(256), 0, st, d);
+}
+void launch_max_speed(const DevPtrs& d, int64_t n, unsigned* out, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_max_speed, dim3(blocks_for(n, 256)), dim3(256), 0, st, d, n, out);
 }
 void launch_unsort(const DevPtrs& d, int64_t n, float* out, int first, int ncomp, hipStream_t st) {
   if (n > 0) hipLaunchKernelGGL(k_unsort, dim3(blocks_for(n, 256)), dim3(256), 0, st, d, n, out, first, ncomp);

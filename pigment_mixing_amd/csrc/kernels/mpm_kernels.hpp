@@ -149,6 +149,11 @@ void launch_closure_eval(const SceneDev* scenes, const void* mlp_blob, const flo
                          float* learned4, float* exact4, uint8_t* accepted, int max_grid, hipStream_t st);
 void launch_p2g(const DevPtrs& d, int grid, int src_is_b, hipStream_t st);
 void launch_grid(const DevPtrs& d, int grid, hipStream_t st);
+// the last step's grid update again from its tiles with the current scene
+// constants (after a dt change; not after an atomic scatter)
+void launch_grid_redo(const DevPtrs& d, int grid, hipStream_t st);
+// largest squared speed of the live particles of the current side -> *out (float bits; clear it first)
+void launch_max_speed(const DevPtrs& d, int64_t n, unsigned* out, hipStream_t st);
 void launch_unsort(const DevPtrs& d, int64_t n, float* out, int first, int ncomp, hipStream_t st);
 void launch_active_out(const DevPtrs& d, int64_t n, uint8_t* out, hipStream_t st);
 void launch_pack_render(const DevPtrs& d, int64_t n, float* pos4, float* lat7, float* diff, hipStream_t st);

@@ -61,10 +61,28 @@ int main(int argc, char** argv) {
     if (stat(f.c_str(), &st) == 0) ply_bytes = (long)st.st_size;
   }
   const auto gb = sim.getGridBoundaries();
+  const unsigned frame1 = sim.frame;
+  const double time1 = sim.time();
+  // a second frame with the reference's adaptive step (updateDt before every
+  // step, dt_max = cfl_elastic dx / sqrt(E / rho) with cfl_elastic 0.1)
+  sim.adaptive_dt = true;
+  sim.cfl_elastic = 0.1f;
+  sim.dt_max = sim.cfl_elastic * sim.dx / std::sqrt(sim.cfg.E / sim.cfg.rho);
+  int steps2 = 0;
+  float dt_min2 = 1e30f, dt_max2 = 0.0f;
+  while (!sim.frameFinished()) {
+    sim.step();
+    ++steps2;
+    dt_min2 = std::fmin(dt_min2, sim.dt);
+    dt_max2 = std::fmax(dt_max2, sim.dt);
+    if (steps2 > 1000) break;
+  }
   std::printf("{\"config_rc\": %d, \"Np\": %u, \"steps\": %d, \"frame\": %u, \"active\": %d, \"ymean\": %.9g, "
               "\"vymin\": %.9g, \"exit\": %d, \"pos_w\": %.3g, \"lo\": [%.6g, %.6g, %.6g], \"hi\": [%.6g, %.6g, %.6g], "
-              "\"time\": %.9g, \"ply_bytes\": %ld}\n",
-              rc, sim.Np, steps, sim.frame, act, ymean / sim.Np, vmin, sim.exit, pos4[0][3], gb.first[0], gb.first[1],
-              gb.first[2], gb.second[0], gb.second[1], gb.second[2], sim.time(), ply_bytes);
+              "\"time\": %.9g, \"ply_bytes\": %ld, \"adaptive\": {\"steps\": %d, \"frame\": %u, \"time\": %.9g, "
+              "\"dt_min\": %.9g, \"dt_max\": %.9g, \"dt_bound\": %.9g, \"exit\": %d}}\n",
+              rc, sim.Np, steps, frame1, act, ymean / sim.Np, vmin, sim.exit, pos4[0][3], gb.first[0], gb.first[1],
+              gb.first[2], gb.second[0], gb.second[1], gb.second[2], time1, ply_bytes, steps2, sim.frame, sim.time(),
+              dt_min2, dt_max2, sim.dt_max, sim.exit);
   return 0;
 }

@@ -39,6 +39,7 @@ def _load(path):
         ("orc_set_keyframes", ctypes.c_int, [H, F, ctypes.c_int, ctypes.c_float]),
         ("orc_prepare", ctypes.c_int, [H]),
         ("orc_step", ctypes.c_int, [H, ctypes.c_int]),
+        ("orc_set_dt", ctypes.c_int, [H, ctypes.c_float]),
         ("orc_download", ctypes.c_int, [H, F, F, F, F, F, F, F, F, abi.c_u8_p]),
         ("orc_get_lattice", ctypes.c_int, [H, abi.c_int_p, F, F, F, F]),
         ("orc_get_grid", ctypes.c_int, [H, F, F, F, F, F]),
@@ -121,6 +122,10 @@ class Oracle:
 
     def step(self, n=1):
         return self.lib.orc_step(self.h, n)
+
+    def set_dt(self, dt):
+        """Step of the following steps (scenes without moving objects)."""
+        assert self.lib.orc_set_dt(self.h, float(dt)) == 0
 
     def download(self):
         n = self.n

@@ -534,7 +534,8 @@ def test_large_scene_mass_budget():
 
 def test_cpp_simulation_facade_lifecycle(tmp_path):
     """The C++ Simulation facade (reference API) runs a frame on the device and
-    writes it with saveParticleData."""
+    writes it with saveParticleData, then a second frame with the adaptive
+    step (Simulation::updateDt before every step)."""
     import json
     import subprocess
     from pigment_mixing_amd import build as b
@@ -549,6 +550,13 @@ def test_cpp_simulation_facade_lifecycle(tmp_path):
     assert out["steps"] == 100 and out["frame"] == 1      # fps 1000, dt 1e-5
     assert out["exit"] == 0 and out["pos_w"] == 1.0
     assert out["vymin"] < 0                              # falling under gravity
+    # the second frame with updateDt: steps of at most dt_max, the last one
+    # shortened to land on the frame's end
+    ad = out["adaptive"]
+    assert ad["exit"] == 0 and ad["frame"] == 2
+    assert ad["dt_max"] <= ad["dt_bound"] * (1 + 1e-6) and ad["dt_min"] < ad["dt_max"]
+    assert ad["steps"] == int(np.ceil(1e-3 / ad["dt_bound"] - 1e-6))
+    assert abs(ad["time"] - 2e-3) < 1e-6
     # x y z vx vy vz p q Je eps_pl_vol eps_pl_dev delta_gamma per particle after the header
     with open(tmp_path / "api_demo" / f"particles_f{out['frame']}.ply", "rb") as f:
         head = f.read(512)
