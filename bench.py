@@ -125,7 +125,7 @@ def timed_run(h, steps, warmup, tune=True):
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                             "hbm_traffic_paint_4blob_256x64x256.json")
 TRAFFIC_NAMES = {"k_g2p": ("k_g2p<false, 0, false>", "k_g2p<true, 0, false>"), "k_p2g": ("k_p2g<false>",) * 2,
-                 "k_grid": ("k_grid",) * 2}
+                 "k_grid": ("k_grid<false>",) * 2}
 
 
 def measured_traffic(kernel, learned):

@@ -762,6 +762,7 @@ int mpmx_boundary_eval(const mpmx_config* cfg, const mpmx_plate* plates, int n_p
   Scene s;
   s.cfg = *cfg;
   if (n_plates > 0) s.plates.assign(plates, plates + n_plates);
+  s.plates_base = s.plates;
   s.has_knife = knife != nullptr;
   if (knife) s.knife = *knife;
   fill_scene_dev(s, 0, 0);
