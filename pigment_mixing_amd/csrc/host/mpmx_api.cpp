@@ -1691,7 +1691,7 @@ int mpmx_autotune(mpmx_handle* h, int steps_per_trial, mpmx_tune_report* rep) {
   if (!h || steps_per_trial <= 0) return MPMX_ERR_ARG;
   if (!h->prepared) return fail(h, MPMX_ERR_STATE, "autotune: prepare first");
   const std::vector<int> cand_p2g = {8, 12, 16, 20, 24, 32};
-  const std::vector<int> cand_g2p = {2, 3, 4, 6, 8};
+  const std::vector<int> cand_g2p = {2, 3, 4, 6, 8, 12, 16};
   const std::vector<int> cand_grid = {2, 4, 8};
   int trials = 0;
   mpmx_kernel_times kt;

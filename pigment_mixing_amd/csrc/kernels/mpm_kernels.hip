@@ -592,9 +592,12 @@ __device__ __forceinline__ void harvest_pair(const DevPtrs& d, const SceneDev& s
 // writes their F, eps, dgamma and stress.
 // Four wavefronts per SIMD (128 VGPRs; the fused phase spills a few dwords):
 // the gather's latency hiding is worth more than the spills (measured ~20%
-// faster than three wavefronts at 137 VGPRs).
+// faster than three wavefronts at 137 VGPRs).  Two-wavefront workgroups: a
+// block's particles run in passes of 128, so that fewer wavefronts wait at
+// the next block's barrier while the last pass finishes (blocks hold a few
+// hundred particles; with four-wavefront workgroups G2P took 5 % longer).
 template <bool LEARNED, int PHASE, bool VOL>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_g2p(DevPtrs d, const SceneDev* __restrict__ scenes,
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k_g2p(DevPtrs d, const SceneDev* __restrict__ scenes,
                                              const float* __restrict__ axis, const float* __restrict__ A,
                                              const int* __restrict__ perm, const float* __restrict__ gv,
                                              const int4* __restrict__ pdesc) {
@@ -1858,19 +1861,19 @@ void launch_g2p(const DevPtrs& d, int grid, int learned, int phase, hipStream_t 
   const int4* pd = d.pdesc;
   // the learned closure never runs with volume tracking (refused at create)
   if (phase == 1)
-    hipLaunchKernelGGL((k_g2p<false, 1, false>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, pd);
+    hipLaunchKernelGGL((k_g2p<false, 1, false>), dim3(grid * 2), dim3(128), 0, st, d, sc, ax, pa, pm, gv, pd);
   else if (phase == 2 && learned)
-    hipLaunchKernelGGL((k_g2p<true, 2, false>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, pd);
+    hipLaunchKernelGGL((k_g2p<true, 2, false>), dim3(grid * 2), dim3(128), 0, st, d, sc, ax, pa, pm, gv, pd);
   else if (phase == 2 && d.track_vol)
-    hipLaunchKernelGGL((k_g2p<false, 2, true>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, pd);
+    hipLaunchKernelGGL((k_g2p<false, 2, true>), dim3(grid * 2), dim3(128), 0, st, d, sc, ax, pa, pm, gv, pd);
   else if (phase == 2)
-    hipLaunchKernelGGL((k_g2p<false, 2, false>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, pd);
+    hipLaunchKernelGGL((k_g2p<false, 2, false>), dim3(grid * 2), dim3(128), 0, st, d, sc, ax, pa, pm, gv, pd);
   else if (learned)
-    hipLaunchKernelGGL((k_g2p<true, 0, false>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, pd);
+    hipLaunchKernelGGL((k_g2p<true, 0, false>), dim3(grid * 2), dim3(128), 0, st, d, sc, ax, pa, pm, gv, pd);
   else if (d.track_vol)
-    hipLaunchKernelGGL((k_g2p<false, 0, true>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, pd);
+    hipLaunchKernelGGL((k_g2p<false, 0, true>), dim3(grid * 2), dim3(128), 0, st, d, sc, ax, pa, pm, gv, pd);
   else
-    hipLaunchKernelGGL((k_g2p<false, 0, false>), dim3(grid), dim3(256), 0, st, d, sc, ax, pa, pm, gv, pd);
+    hipLaunchKernelGGL((k_g2p<false, 0, false>), dim3(grid * 2), dim3(128), 0, st, d, sc, ax, pa, pm, gv, pd);
 }
 void launch_closure(const DevPtrs& d, const void* mlp_blob, int grid, hipStream_t st) {
   hipLaunchKernelGGL(k_closure, dim3(grid), dim3(256), 0, st, d, d.scenes, (const uint8_t*)mlp_blob);
