@@ -419,7 +419,6 @@ void enqueue_g2p(mpmx_handle* h, const DevPtrs& d, hipEvent_t before_closure = n
 void enqueue_step(mpmx_handle* h, const DevPtrs& d) {
   enqueue_g2p(h, d);
   launch_scan(d, h->nkeys, 1, h->stream);
-  launch_cellscan(d, grid_grid(h), h->stream);
   launch_perm(d, h->np, grid_grid(h), 1, h->stream);
   launch_p2g(d, grid_p2g(h), 1, h->stream);
   launch_grid(d, grid_grid(h), h->stream);
@@ -984,8 +983,6 @@ int mpmx_prepare(mpmx_handle* h) {
   ALLOC(d.perm, int, n);
   ALLOC(d.kslot, int, n);
   ALLOC(d.cmaj, int, n);
-  ALLOC(d.cell_cnt, int, (size_t)h->nkeys * BLK3);
-  ALLOC(d.cell_off, int, (size_t)h->nkeys * BLK3);
   ALLOC(d.cell_num, int, (size_t)h->nkeys * BLK3);
   ALLOC(d.pdesc, int4, h->nkeys);
   // merged P2G tiles: room for the listed blocks of dense paint (~355 particles
@@ -1051,7 +1048,6 @@ int mpmx_prepare(mpmx_handle* h) {
   h->resume = h->tau_given = false;
   launch_keys(d, n, h->stream);
   launch_scan(d, h->nkeys, 0, h->stream);
-  launch_cellscan(d, grid_grid(h), h->stream);
   launch_perm(d, n, grid_grid(h), 0, h->stream);
   launch_p2g(d, grid_p2g(h), 0, h->stream);
   launch_grid(d, grid_grid(h), h->stream);
@@ -1627,7 +1623,7 @@ int mpmx_profile_steps(mpmx_handle* h, int nsteps, mpmx_kernel_times* out) {
   for (auto& e : ev) HIPCHK(h, hipEventCreate(&e));
   double t_f = 0, t_c = 0, t_s = 0, t_p = 0, t_g = 0, t_all = 0, live = 0;
   // t_f: k_g2p (+ dead copy, MUSL halves), t_c: k_closure, t_s: sort (scans,
-  // cell scan, permutation), t_p: k_p2g, t_g: k_grid
+  // permutation, in-cell ranking), t_p: k_p2g, t_g: k_grid
   int done = 0;
   while (done < nsteps) {
     const int chunk = std::min(nsteps - done, MAX_CHUNK);
@@ -1638,7 +1634,6 @@ int mpmx_profile_steps(mpmx_handle* h, int nsteps, mpmx_kernel_times* out) {
       enqueue_g2p(h, d, ev[5]);
       HIPCHK(h, hipEventRecord(ev[1], h->stream));
       launch_scan(d, h->nkeys, 1, h->stream);
-      launch_cellscan(d, grid_grid(h), h->stream);
       launch_perm(d, h->np, grid_grid(h), 1, h->stream);
       HIPCHK(h, hipEventRecord(ev[2], h->stream));
       launch_p2g(d, grid_p2g(h), 1, h->stream);

@@ -9,11 +9,10 @@
 //             stress of the new F for the next scatter) in sorted order, and counts
 //             the particles' new blocks and cells for the next sort.
 //   k_scan_*  prefix sums over block counts -> block offsets + block list.
-//   k_cellscan per block: offsets of the 64 cells inside the block, and the
-//             rank-major slot of every (cell, rank).
-//   k_perm    particles to cell-major positions by their counting ranks;
-//   k_cellsort orders each cell by particle position (canonical) and writes the
-//             permutation slot <- particle: block-major, rank-major in a block.
+//   k_perm    particles to their block's range by their rank in the block;
+//   k_cellsort per block: cell counts and offsets, each cell's particles ordered
+//             by position (canonical), and the permutation slot <- particle:
+//             block-major, rank-major in a block.
 //   k_p2g     lane = cell, wave = (block, channel group): each lane walks its
 //             cell's particles and accumulates their 27-node contributions (mass,
 //             APIC momentum, Kirchhoff force, pigment latent, shear intensity;
@@ -129,20 +128,19 @@ __device__ __forceinline__ XcdRange xcd_range(int nblk) {
   return r;
 }
 
-// Count a particle for the next sort: block count (+ rank for dead particles)
-// and per-cell rank for live ones.  All lanes call.  Block keys repeat across
-// a wavefront (one atomic per distinct key); cell keys mostly do not (the
-// lanes walk a rank-major block layout, one particle per cell and rank), so
-// each live lane takes its cell rank with its own atomic.
+// Count a particle for the next sort: its block (count and rank inside the
+// block, one atomic per distinct key in the wavefront), cell and order key.
+// All lanes call.  The order inside a block is fixed later by k_cellsort,
+// from the cells and order keys alone, so the ranks' atomic timing does not
+// reach any result.
 __device__ __forceinline__ void count_for_sort(const DevPtrs& d, bool valid, int key, int cell, int64_t j,
                                                uint32_t okey = 0u) {
   const bool live = valid && cell >= 0;
   const int rb = wave_rank(valid, key, d.count_next);
-  const int rc = live ? atomicAdd(&d.cell_cnt[key * BLK3 + cell], 1) : 0;
   if (valid) {
     d.key_out[j] = key;
     d.cell_out[j] = live ? cell : -1;
-    d.rank_out[j] = live ? rc : rb;
+    d.rank_out[j] = rb;
     if (live) d.okey_out[j] = okey;
   }
 }
@@ -291,82 +289,48 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_down(DevPtrs d, int nkeys
   }
 }
 
-// Per block (one wavefront, lane = cell): exclusive scan of the 64 cell
-// counts, and the rank-major slot of every (cell, rank) of the block.
-//
-// Inside a block the particles are laid out rank-major: first rank 0 of every
-// non-empty cell (cells in lane order), then rank 1 of every cell holding two
-// or more, and so on.  The scatter walks a block with lane = cell and step k =
-// rank, so at every step its active lanes read consecutive slots (coalesced),
-// where a cell-major layout would spread one load over ~8 particles' stride.
-// Slot of (cell c, rank r) = block start + sum_{r' < r} #{cells with count > r'}
-// + #{cells below c with count > r}.  kslot maps the cell-major position
-// (block start + cell offset + rank, what the ranks of the next sort give)
-// to it.
-__global__ void __launch_bounds__(256) k_cellscan(DevPtrs d) {
-  const int nblk = d.counters[CNT_PBLOCKS];
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int nwaves = (gridDim.x * blockDim.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  const uint64_t below = lanemask_lt();
-  for (int e = wave; e < nblk; e += nwaves) {
-    const int key = d.pb_list[e];
-    const int idx = key * BLK3 + lane;
-    const int c = d.cell_cnt[idx];
-    int incl = c;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const int t = __shfl_up(incl, off);
-      if (lane >= off) incl += t;
-    }
-    const int coff = incl - c;
-    d.cell_off[e * BLK3 + lane] = coff;
-    d.cell_num[e * BLK3 + lane] = c;
-    d.cell_cnt[idx] = 0;
-    const int start = d.offsets[key];
-    int koff = 0;                          // slots of the ranks below r (wave-uniform)
-    for (int r = 0;; ++r) {
-      const uint64_t m = __ballot(c > r);
-      if (m == 0ull) break;
-      if (c > r) d.kslot[start + coff + r] = start + koff + (int)__popcll(m & below);
-      koff += (int)__popcll(m);
-    }
-  }
-}
-
 // ------------------------------------------------------------------ k_perm
-// Live particles go to the cell-major position (block start + cell offset +
-// counting rank) of `cmaj`, which k_cellsort orders; dead particles go straight
-// to their slot (block offset of the DEAD key + rank).
+// Every particle to its block's range by its rank inside the block: live ones
+// into `cmaj` (arrival order, which k_cellsort replaces by the block's
+// rank-major layout), dead ones straight to their slot of the DEAD key's range.
 __global__ void k_perm(DevPtrs d, int64_t n) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  const int key = d.key_out[j];
-  const int cell = d.cell_out[j];
-  if (cell >= 0)
-    d.cmaj[d.offsets[key] + d.cell_off[d.list_pos[key] * BLK3 + cell] + d.rank_out[j]] = (int)j;
+  const int pos = d.offsets[d.key_out[j]] + d.rank_out[j];
+  if (d.cell_out[j] >= 0)
+    d.cmaj[pos] = (int)j;
   else
-    d.perm[d.offsets[key] + d.rank_out[j]] = (int)j;
+    d.perm[pos] = (int)j;
 }
 
 // ------------------------------------------------------------------ k_cellsort
-// Per listed block (one wavefront, lane = cell): the particles of a cell in
-// ascending order of their position: the order key (cell_order_key, written
-// by the key pass), then on equal keys x, y, z and the particle id.  The
-// counting ranks depend on the timing of the atomics; ordering by position
-// makes the order inside a cell, and with it every FP32 sum of the scatter
-// (mass, momentum, force, latents, shear), a function of the particles' state
-// alone: independent of the upload order, of the layout the previous step
-// left, of the other scenes of a batch and of the launch shapes.  Positions
-// and ids come from the side the sort orders.  Ranks are a permutation of
-// 0..c-1; rank r of the cell goes to its rank-major slot kslot[cell start +
-// r].  One 256-thread workgroup per listed block (grid-stride over the list;
-// the launch gives most blocks a workgroup of their own), one thread per particle:
-// the block's order keys and slots are staged in LDS in cell-major order
-// (blocks of up to CS_CAP particles; larger ones read them from global
-// memory, same order), and each thread ranks its particle against the others
-// of its cell.
+// Per listed block (one 256-thread workgroup; grid-stride over the list): the
+// block's particles in their final, rank-major order.  Inside a block the
+// particles are laid out rank-major: first rank 0 of every non-empty cell
+// (cells in lane order), then rank 1 of every cell holding two or more, and so
+// on.  The scatter walks a block with lane = cell and step k = rank, so at
+// every step its active lanes read consecutive slots (coalesced), where a
+// cell-major layout would spread one load over ~8 particles' stride.
+//   1. the cells and order keys of the block's particles (cmaj, arrival
+//      order) into LDS; cell counts by LDS atomics;
+//   2. cell offsets (wave scan), the counts for the scatter (cell_num), and
+//      per rank r the mask of cells holding more than r particles with the
+//      slots of the ranks below: slot(c, r) = sum_{r' < r} #{cells with count
+//      > r'} + #{cells below c with count > r};
+//   3. the particles bucketed by cell;
+//   4. each particle's rank inside its cell: ascending position, i.e. the
+//      order key (cell_order_key, written by the key pass), on equal keys x,
+//      y, z and then the particle id (pos_before); perm[slot] = particle.
+// Ranking by position makes the order inside a cell, and with it every FP32
+// sum of the scatter (mass, momentum, force, latents, shear), a function of
+// the particles' state alone: independent of the upload order, of the layout
+// the previous step left, of the atomics' timing, of the other scenes of a
+// batch and of the launch shapes.  Blocks of more than CS_CAP particles keep
+// the per-particle arrays in global memory (cmaj, the key pass's arrays, and
+// `kslot` as the bucket array); ranks of CS_MAXR and above take their slot
+// from a loop over the 64 counts.
 constexpr int CS_CAP = 2048;
+constexpr int CS_MAXR = 64;
 // float -> unsigned with the same order (negative values reversed below the
 // positive ones); -0 sorts just below +0
 __device__ __forceinline__ uint32_t ord_key(float f) {
@@ -384,51 +348,93 @@ __device__ __forceinline__ bool pos_before(const float* __restrict__ X, const in
   return a != b && ids[a] < ids[b];
 }
 __global__ void __launch_bounds__(256) k_cellsort(DevPtrs d, const float* __restrict__ X, const int* __restrict__ ids) {
-  __shared__ uint32_t skey[CS_CAP];
-  __shared__ int sslot[CS_CAP];
-  __shared__ int coff[BLK3 + 1];
+  __shared__ uint32_t s_key[CS_CAP];
+  __shared__ unsigned short s_bucket[CS_CAP];
+  __shared__ unsigned char s_cell[CS_CAP];
+  __shared__ int s_cnt[BLK3], s_fill[BLK3], s_off[BLK3];
+  __shared__ unsigned long long s_mask[CS_MAXR];
+  __shared__ int s_base[CS_MAXR];
   const int nblk = d.counters[CNT_PBLOCKS];
   const int64_t cap = d.cap;
+  const int lane = threadIdx.x & 63;
   for (int e = blockIdx.x; e < nblk; e += gridDim.x) {
     const int4 ds = d.pdesc[e];
     const int start = ds.y, cnt = ds.z;
-    __syncthreads();   // the previous block is done with the LDS
-    if (threadIdx.x < BLK3) coff[threadIdx.x] = d.cell_off[e * BLK3 + threadIdx.x];
-    if (threadIdx.x == 0) coff[BLK3] = cnt;
     const bool staged = cnt <= CS_CAP;
-    if (staged) {
-      for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
-        const int v = d.cmaj[start + t];
-        sslot[t] = v;
-        skey[t] = (uint32_t)d.okey_out[v];
+    int* const gbucket = d.kslot + start;   // bucket array of an unstaged block
+    __syncthreads();   // the previous block is done with the LDS
+    if (threadIdx.x < BLK3) { s_cnt[threadIdx.x] = 0; s_fill[threadIdx.x] = 0; }
+    __syncthreads();
+    // 1. particles, cells, keys; cell counts
+    for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+      const int j = d.cmaj[start + t];
+      const int c = d.cell_out[j];
+      if (staged) {
+        s_key[t] = (uint32_t)d.okey_out[j];
+        s_cell[t] = (unsigned char)c;
+      }
+      atomicAdd(&s_cnt[c], 1);
+    }
+    __syncthreads();
+    // 2. offsets, counts for the scatter, rank masks (first wavefront)
+    if (threadIdx.x < 64) {
+      const int n = s_cnt[lane];
+      int incl = n;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int t = __shfl_up(incl, off);
+        if (lane >= off) incl += t;
+      }
+      s_off[lane] = incl - n;
+      d.cell_num[e * BLK3 + lane] = n;
+      int maxc = n;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) maxc = max(maxc, __shfl_xor(maxc, off));
+      int run = 0;
+      for (int r = 0; r < min(maxc, CS_MAXR); ++r) {
+        const uint64_t m = __ballot(n > r);
+        if (lane == 0) { s_mask[r] = m; s_base[r] = run; }
+        run += (int)__popcll(m);
       }
     }
     __syncthreads();
+    // 3. bucket by cell (order inside a bucket arbitrary)
     for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
-      // this position's cell: the last c with coff[c] <= t (empty cells share offsets)
-      int lo = 0, hi = BLK3;   // coff[lo] <= t < coff[hi]
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (coff[mid] <= t) lo = mid; else hi = mid;
-      }
-      const int c0 = coff[lo], c1 = coff[lo + 1];
-      const int v = staged ? sslot[t] : d.cmaj[start + t];
-      const uint32_t kr = staged ? skey[t] : (uint32_t)d.okey_out[v];
+      const int c = staged ? (int)s_cell[t] : d.cell_out[d.cmaj[start + t]];
+      const int q = s_off[c] + atomicAdd(&s_fill[c], 1);
+      if (staged) s_bucket[q] = (unsigned short)t; else gbucket[q] = t;
+    }
+    __syncthreads();
+    // 4. rank inside the cell, slot, permutation
+    for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+      const int j = d.cmaj[start + t];
+      const int c = staged ? (int)s_cell[t] : d.cell_out[j];
+      const uint32_t kr = staged ? s_key[t] : (uint32_t)d.okey_out[j];
+      const int q0 = s_off[c], q1 = q0 + s_cnt[c];
       int rk = 0;
       bool tie = false;
-      for (int s2 = c0; s2 < c1; ++s2) {
-        const uint32_t ks = staged ? skey[s2] : (uint32_t)d.okey_out[d.cmaj[start + s2]];
-        rk += ks < kr ? 1 : 0;
-        tie |= s2 != t && ks == kr;
+      for (int q = q0; q < q1; ++q) {
+        const int u = staged ? s_bucket[q] : gbucket[q];
+        const uint32_t ku = staged ? s_key[u] : (uint32_t)d.okey_out[d.cmaj[start + u]];
+        rk += ku < kr ? 1 : 0;
+        tie |= u != t && ku == kr;
       }
       if (tie) {   // equal order keys (particles within 1/1024 of a cell of each other): full positions decide
-        for (int s2 = c0; s2 < c1; ++s2) {
-          const int vs = staged ? sslot[s2] : d.cmaj[start + s2];
-          const uint32_t ks = staged ? skey[s2] : (uint32_t)d.okey_out[vs];
-          if (s2 != t && ks == kr) rk += pos_before(X, ids, cap, vs, v) ? 1 : 0;
+        for (int q = q0; q < q1; ++q) {
+          const int u = staged ? s_bucket[q] : gbucket[q];
+          const int ju = d.cmaj[start + u];
+          const uint32_t ku = staged ? s_key[u] : (uint32_t)d.okey_out[ju];
+          if (u != t && ku == kr) rk += pos_before(X, ids, cap, ju, j) ? 1 : 0;
         }
       }
-      d.perm[d.kslot[start + c0 + rk]] = v;
+      int slot;
+      if (rk < CS_MAXR) {
+        slot = s_base[rk] + (int)__popcll(s_mask[rk] & ((1ull << c) - 1ull));
+      } else {
+        slot = 0;
+        for (int c2 = 0; c2 < BLK3; ++c2) slot += min(s_cnt[c2], rk) + (c2 < c && s_cnt[c2] > rk ? 1 : 0);
+      }
+      d.perm[start + slot] = j;
     }
   }
 }
@@ -1843,9 +1849,6 @@ void launch_scan(const DevPtrs& d, int nkeys, int advance, hipStream_t st) {
   hipLaunchKernelGGL(k_scan_down, dim3(nchunks), dim3(SCAN_THREADS), 0, st, d, nkeys, nchunks, advance);
 }
 int scan_chunks(int nkeys) { return (nkeys + SCAN_CHUNK - 1) / SCAN_CHUNK; }
-void launch_cellscan(const DevPtrs& d, int grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_cellscan, dim3(grid), dim3(256), 0, st, d);
-}
 void launch_perm(const DevPtrs& d, int64_t n, int grid, int src_is_b, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_perm, dim3(blocks_for(n, 256)), dim3(256), 0, st, d, n);

@@ -52,10 +52,8 @@ struct DevPtrs {
   int* cell_out;           // cell (0..63) of the particle in its block, -1 if dead
   int* okey_out;           // order key inside the cell (cell_order_key), live particles
   int* perm;
-  int* kslot;              // cell-major position (block start + cell offset + rank) -> rank-major slot
-  int* cmaj;               // live particles at cell-major positions by counting rank (k_perm -> k_cellsort)
-  int* cell_cnt;           // per (block, cell) particle counts of the next sort
-  int* cell_off;           // per (listed block, cell) offsets inside the block, list order
+  int* kslot;              // k_cellsort's bucket array for blocks past its LDS capacity (by slot)
+  int* cmaj;               // live particles in their block's range by block rank (k_perm -> k_cellsort)
   int* cell_num;           // per (listed block, cell) counts of the current sort, list order
   int4* pdesc;             // per listed block: key, first slot, particle count, scene
   float* tile_acc;         // per listed block (< tile_cap): merged 6x6x6 node tile, [ACC_CH][216]
@@ -133,7 +131,6 @@ void launch_init_tau(const DevPtrs& d, int64_t n, hipStream_t st);
 void launch_keys(const DevPtrs& d, int64_t n, hipStream_t st);
 void launch_scan(const DevPtrs& d, int nkeys, int advance, hipStream_t st);
 int scan_chunks(int nkeys);
-void launch_cellscan(const DevPtrs& d, int grid, hipStream_t st);
 // k_perm + k_cellsort (grid: workgroups of the per-block pass; src_is_b: the
 // sorted slots index the B side's rows and ids, else the A side's)
 void launch_perm(const DevPtrs& d, int64_t n, int grid, int src_is_b, hipStream_t st);
