@@ -419,6 +419,10 @@ typedef struct {
   int64_t steps;                 /* simulation steps spent tuning             */
 } mpmx_tune_report;
 int mpmx_autotune(mpmx_handle* h, int steps_per_trial, mpmx_tune_report* report);
+/* Set the launch shapes directly (workgroups per CU, each in 1..64), e.g. to
+ * reuse an earlier autotune of the same workload.  Shapes change no result
+ * (every sum has a fixed order).  No counterpart in the reference. */
+int mpmx_set_launch_shape(mpmx_handle* h, int wg_p2g, int wg_g2p, int wg_grid);
 
 /* ---------------- learned return-map closure ---------------- */
 

@@ -192,6 +192,7 @@ def _sig(lib):
          [H, ctypes.c_int, ctypes.POINTER(Plate), ctypes.c_int, ctypes.POINTER(Knife), ctypes.c_int]),
         ("mpmx_scene_range", ctypes.c_int, [H, ctypes.c_int, c_i64_p, c_i64_p]),
         ("mpmx_autotune", ctypes.c_int, [H, ctypes.c_int, ctypes.POINTER(TuneReport)]),
+        ("mpmx_set_launch_shape", ctypes.c_int, [H, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         ("mpmx_closure_n_params", ctypes.c_int64, []),
         ("mpmx_closure_set_params", ctypes.c_int, [H, c_float_p, ctypes.c_int64]),
         ("mpmx_closure_get_params", ctypes.c_int, [H, c_float_p, ctypes.c_int64]),
@@ -535,6 +536,10 @@ class Handle:
         r = TuneReport()
         self._chk(self.lib.mpmx_autotune(self.h, steps_per_trial, ctypes.byref(r)), "autotune")
         return r
+
+    def set_launch_shape(self, wg_p2g, wg_g2p, wg_grid):
+        """Workgroups per CU for P2G, G2P and the grid update (mpmx_set_launch_shape)."""
+        self._chk(self.lib.mpmx_set_launch_shape(self.h, wg_p2g, wg_g2p, wg_grid), "set_launch_shape")
 
     def set_objects(self, plates=(), knife=None):
         P = (Plate * max(1, len(plates)))(*plates) if plates else None

@@ -118,7 +118,8 @@ struct mpmx_handle {
   // batched handle (mpmx_create_batch): particles staged per scene
   bool batch = false;
   // launch shapes (workgroups per CU), set by mpmx_autotune
-  int wg_g2p = 4, wg_p2g = 16, wg_grid = 4;
+  int wg_g2p = 4, wg_p2g = 12, wg_grid = 4;
+  int p2g_resident = 12;   // k_p2g workgroups resident per CU (its occupancy)
 };
 
 namespace {
@@ -586,6 +587,9 @@ int mpmx_create_batch(const mpmx_config* cfgs, int n_scenes, mpmx_handle** out) 
   h->device = dev;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) == hipSuccess) h->n_cu = prop.multiProcessorCount;
+  // P2G's first items per wavefront are static and assume every wavefront is
+  // resident from the start: launch it at its occupancy
+  h->wg_p2g = h->p2g_resident = p2g_resident_per_cu();
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
     return MPMX_ERR_HIP;
@@ -1011,6 +1015,7 @@ int mpmx_prepare(mpmx_handle* h) {
   ALLOC(d.nb_list, int, (size_t)2 * nb_total);
   ALLOC(d.nb_live, int, nb_total);
   ALLOC(d.counters, int, CNT_NCOUNTERS);
+  ALLOC(d.wq, int, 2 * CQ_SEGS * CQ_CNT_STRIDE);
   ALLOC(d.stats, unsigned long long, ST_N);
   ALLOC(d.err, uint32_t, 4);
   ALLOC(h->d_step_zero, int, 1);
@@ -1676,6 +1681,20 @@ int mpmx_profile_steps(mpmx_handle* h, int nsteps, mpmx_kernel_times* out) {
 
 }  // extern "C"
 
+// Launch shapes set by the caller (e.g. the result of an earlier autotune of
+// the same workload); the step graph is rebuilt at the next mpmx_step.
+int mpmx_set_launch_shape(mpmx_handle* h, int wg_p2g, int wg_g2p, int wg_grid) {
+  if (!h) return MPMX_ERR_ARG;
+  if (wg_p2g < 1 || wg_p2g > 64 || wg_g2p < 1 || wg_g2p > 64 || wg_grid < 1 || wg_grid > 64)
+    return fail(h, MPMX_ERR_ARG, "set_launch_shape: workgroups per CU must be in 1..64");
+  h->wg_p2g = wg_p2g;
+  h->wg_g2p = wg_g2p;
+  h->wg_grid = wg_grid;
+  if (h->graph_exec) { (void)hipGraphExecDestroy(h->graph_exec); h->graph_exec = nullptr; }
+  if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+  return MPMX_OK;
+}
+
 // ---------------------------------------------------------- launch-shape autotuner
 // Coordinate descent over workgroups per CU: P2G (one-wavefront workgroups),
 // then G2P (256-thread workgroups), then the grid update.  Each candidate is timed with HIP events over
@@ -1685,7 +1704,10 @@ int mpmx_profile_steps(mpmx_handle* h, int nsteps, mpmx_kernel_times* out) {
 int mpmx_autotune(mpmx_handle* h, int steps_per_trial, mpmx_tune_report* rep) {
   if (!h || steps_per_trial <= 0) return MPMX_ERR_ARG;
   if (!h->prepared) return fail(h, MPMX_ERR_STATE, "autotune: prepare first");
-  const std::vector<int> cand_p2g = {8, 12, 16, 20, 24, 32};
+  // P2G: up to its occupancy (its static first items assume resident wavefronts)
+  std::vector<int> cand_p2g;
+  for (int c : {h->p2g_resident / 2, (2 * h->p2g_resident) / 3, h->p2g_resident})
+    if (c >= 1 && (cand_p2g.empty() || c != cand_p2g.back())) cand_p2g.push_back(c);
   const std::vector<int> cand_g2p = {2, 3, 4, 6, 8, 12, 16};
   const std::vector<int> cand_grid = {2, 4, 8};
   int trials = 0;

@@ -1131,6 +1131,7 @@ __global__ void __launch_bounds__(256) k_closure_eval(const SceneDev* __restrict
 // wavefronts per SIMD) measured 131 us against 110 us on the 1M-particle
 // scene.
 constexpr int P2G_GROUPS = 5;
+constexpr int P2G_STATIC = 4;   // items per wavefront assigned statically before the queue
 // Accumulator channels of group G (three each): latent 6 + shear + mass |
 // momentum | force | latent 0..2 | latent 3..5 | (MUSL) momentum.  Every
 // channel is an FP32 sum in a fixed order (the particles of a cell in the
@@ -1173,31 +1174,61 @@ __device__ __forceinline__ int ld_slot(const int* __restrict__ A, int i) {
   return *(const int*)((const char*)A + (uint64_t)((uint32_t)i * 4u));
 }
 
+// The particle stream of an item is prefetched one rank ahead into LDS with
+// direct-to-LDS loads (global_load_lds: lane l's dword of a row lands at word l
+// of the row's 64-word LDS line), so that the prefetch holds no registers.  A
+// ring of two buffers, each of P2G_RING_ROWS lines (x, the group's payload, and
+// the permutation entry of the rank after), lives in the first words of the
+// wavefront's staging area, which is zeroed again once the item's loop is done.
+constexpr int P2G_RING_ROWS = 16;
+typedef __attribute__((address_space(1))) const void* GlobalPtr;
+typedef __attribute__((address_space(3))) void* LdsPtr;
+
+__device__ __forceinline__ void dma_dword(const void* row_base, int i, float* lds_line) {
+  const char* p = (const char*)row_base + (uint64_t)((uint32_t)i * 4u);
+  __builtin_amdgcn_global_load_lds((GlobalPtr)p, (LdsPtr)lds_line, 4, 0, 0);
+}
+
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maximum): every
+// direct-to-LDS load of this wavefront has landed.
+__device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// Lines of a buffer: 0..2 x, 3..3+NQ-1 payload, 15 the permutation entry.
 template <int G>
-__device__ __forceinline__ void p2g_load(const float* __restrict__ P, const float* __restrict__ PB, int64_t cap,
-                                         int src, int j, P2GLoad<G>& o) {
+__device__ __forceinline__ void p2g_fetch(const float* __restrict__ P, const float* __restrict__ PB, int64_t cap,
+                                          const int* __restrict__ perm, int src, int j, int j_after, float* buf) {
 #pragma unroll
-  for (int c = 0; c < 3; ++c) o.x[c] = ld_slot(P, cap, P_X + c, src);
+  for (int c = 0; c < 3; ++c) dma_dword(P + (P_X + c) * cap, src, buf + 64 * c);
   if constexpr (G == 6) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) o.q[c] = ld_slot(PB, cap, P_V + c, j);
+    for (int c = 0; c < 3; ++c) dma_dword(PB + (P_V + c) * cap, j, buf + 64 * (3 + c));
 #pragma unroll
-    for (int c = 0; c < 9; ++c) o.q[3 + c] = ld_slot(PB, cap, P_C + c, j);
+    for (int c = 0; c < 9; ++c) dma_dword(PB + (P_C + c) * cap, j, buf + 64 * (6 + c));
   } else if constexpr (G == 1) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) o.q[c] = ld_slot(P, cap, P_V + c, src);
+    for (int c = 0; c < 3; ++c) dma_dword(P + (P_V + c) * cap, src, buf + 64 * (3 + c));
 #pragma unroll
-    for (int c = 0; c < 9; ++c) o.q[3 + c] = ld_slot(P, cap, P_C + c, src);
+    for (int c = 0; c < 9; ++c) dma_dword(P + (P_C + c) * cap, src, buf + 64 * (6 + c));
   } else if constexpr (G == 2) {
 #pragma unroll
-    for (int c = 0; c < 6; ++c) o.q[c] = ld_slot(P, cap, P_TAU + c, src);
+    for (int c = 0; c < 6; ++c) dma_dword(P + (P_TAU + c) * cap, src, buf + 64 * (3 + c));
   } else if constexpr (G == 3 || G == 4) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) o.q[c] = ld_slot(P, cap, P_PIG + 3 * (G - 3) + c, src);
+    for (int c = 0; c < 3; ++c) dma_dword(P + (P_PIG + 3 * (G - 3) + c) * cap, src, buf + 64 * (3 + c));
   } else if constexpr (G == 0) {
-    o.q[0] = ld_slot(P, cap, P_PIG + 6, src);
-    o.q[1] = ld_slot(P, cap, P_DIFF, src);
+    dma_dword(P + (P_PIG + 6) * cap, src, buf + 64 * 3);
+    dma_dword(P + P_DIFF * cap, src, buf + 64 * 4);
   }
+  dma_dword(perm, j_after, buf + 64 * (P2G_RING_ROWS - 1));
+}
+
+template <int G>
+__device__ __forceinline__ int p2g_read(const float* buf, int lane, P2GLoad<G>& o) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) o.x[c] = buf[64 * c + lane];
+#pragma unroll
+  for (int c = 0; c < P2GLoad<G>::NQ; ++c) o.q[c] = buf[64 * (3 + c) + lane];
+  return __float_as_int(buf[64 * (P2G_RING_ROWS - 1) + lane]);
 }
 
 // Accumulate this lane's cell into acc[27][3] for channel group G.  The loop
@@ -1216,10 +1247,11 @@ __device__ __forceinline__ void p2g_load(const float* __restrict__ P, const floa
 // by O(ulp); the mass channel, which decides node activation, keeps the
 // reference's product order and its tiny-weight cut.)
 template <int G>
-__device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[27][p2g_nch(G)]) {
+__device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[27][p2g_nch(G)], float* ring) {
   const float* __restrict__ P = it.P;
   const int64_t cap = it.cap;
   const float inv_dx = it.inv_dx;
+  const int lane = threadIdx.x;
   // Slot of this lane's rank k in the block's rank-major layout: the block
   // start + the slots of the ranks below k (kb, wave-uniform) + the cells
   // below this one that hold a rank k.  A lane without rank k reads the
@@ -1229,19 +1261,23 @@ __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[2
   const int j0 = it.cnt > 0 ? it.start + (int)__popcll(m & below) : it.start;
   int kb = (int)__popcll(m);
   m = __ballot(it.cnt > 1);
-  int j_next = it.cnt > 1 ? it.start + kb + (int)__popcll(m & below) : it.start;
+  int j_next = it.cnt > 1 ? it.start + kb + (int)__popcll(m & below) : it.start;   // rank k + 1
   kb += (int)__popcll(m);
-  P2GLoad<G> cur;
-  p2g_load<G>(P, it.PB, cap, ld_slot(it.perm, j0), j0, cur);
-  int src_next = ld_slot(it.perm, j_next);
+  p2g_fetch<G>(P, it.PB, cap, it.perm, ld_slot(it.perm, j0), j0, j_next, ring);
   for (int k = 0; k < it.kmax; ++k) {
     const bool act = k < it.cnt;
-    P2GLoad<G> nxt;
-    p2g_load<G>(P, it.PB, cap, src_next, j_next, nxt);   // rank k + 1
-    m = __ballot(it.cnt > k + 2);
-    j_next = it.cnt > k + 2 ? it.start + kb + (int)__popcll(m & below) : it.start;
-    kb += (int)__popcll(m);
-    src_next = ld_slot(it.perm, j_next);                 // rank k + 2
+    // rank k (fetched during the previous step) and the permutation entry of rank k + 1
+    float* buf = ring + (k & 1) * (P2G_RING_ROWS * 64);
+    wait_vm_all();
+    P2GLoad<G> cur;
+    const int src_next = p2g_read<G>(buf, lane, cur);
+    if (k + 1 < it.kmax) {
+      m = __ballot(it.cnt > k + 2);
+      const int j_after = it.cnt > k + 2 ? it.start + kb + (int)__popcll(m & below) : it.start;   // rank k + 2
+      kb += (int)__popcll(m);
+      p2g_fetch<G>(P, it.PB, cap, it.perm, src_next, j_next, j_after, ring + ((k + 1) & 1) * (P2G_RING_ROWS * 64));
+      j_next = j_after;
+    }
     const float x0 = act ? cur.x[0] : it.X[0][1];
     const float x1 = act ? cur.x[1] : it.X[1][1];
     const float x2 = act ? cur.x[2] : it.X[2][1];
@@ -1258,6 +1294,11 @@ __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[2
       N[3 * a + 1] = spline_N_at<1>(u[3 * a + 1]);
       N[3 * a + 2] = spline_N_at<2>(u[3 * a + 2]);
     }
+    // group 0: whether the mass cut below can act in this step (see there),
+    // from the factors before the x ones of lanes past their count are zeroed
+    bool near_face = false;
+    if constexpr (G == 0)
+      near_face = fminf(fminf(fminf(N[0], N[2]), fminf(N[3], N[5])), fminf(N[6], N[8])) < 1e-8f;
 #pragma unroll
     for (int o = 0; o < 3; ++o) N[o] = act ? N[o] : 0.0f;
     if constexpr (G == 0) {
@@ -1277,8 +1318,22 @@ __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[2
         const float nyz = N[3 + (n / 3) % 3] * N[6 + n % 3];
         acc[n][0] += fq[0][n / 9] * nyz;
         acc[n][1] += fq[1][n / 9] * nyz;
-        const float w = (N[n / 9] * N[3 + (n / 3) % 3]) * N[6 + n % 3];
-        acc[n][2] += w > 1e-25f ? w : 0.0f;
+      }
+      // The cut acts only on a weight in (0, 1e-25], which needs a 1-D factor
+      // below 1e-8: a particle within 1.5e-4 cells of a face of its cell (the
+      // outer offsets' factors 0.5 t^2; the middle one is >= 0.5).  Lanes past
+      // their count carry node 1's coordinates, whose factors are 0.125 and
+      // 0.75.  When no lane of the wave is near a face, every nonzero weight
+      // is at least 1e-24 and the compare is left out; the sums are the same.
+      if (__ballot(near_face) == 0) {
+#pragma unroll
+        for (int n = 0; n < 27; ++n) acc[n][2] += (N[n / 9] * N[3 + (n / 3) % 3]) * N[6 + n % 3];
+      } else {
+#pragma unroll
+        for (int n = 0; n < 27; ++n) {
+          const float w = (N[n / 9] * N[3 + (n / 3) % 3]) * N[6 + n % 3];
+          acc[n][2] += w > 1e-25f ? w : 0.0f;
+        }
       }
     } else {
       float NyNz[9];
@@ -1370,75 +1425,100 @@ __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[2
         }
       }
     }
-    cur = nxt;
   }
+  wait_vm_all();
 }
 
 // ---- cross-lane merge of the 27-node stencils (lane = ci*16 + cj*4 + ck)
-// Lane shifts: the value of the lane N below within the 16-lane row (DPP
-// row_shr, cells N apart in z for N = 1, 2, in y for N = 4, 8) or N lanes below
-// across rows (cells 1 or 2 apart in x for N = 16, 32).  All are wave-wide
-// convergent operations executed by every lane; lanes without a source are
-// masked by the caller.
+// The z and y axes merge in registers with DPP row shifts: the value of the
+// lane 1 or 2 below within the 16-lane row (cells 1, 2 apart in z) or 4, 8
+// below (cells 1, 2 apart in y).  A shift whose source lies outside the row
+// reads zero (bound_ctrl), which for y is exactly the lanes without a source
+// (cj < 1 or 2).  A z shift into a lane with ck < 1 or 2 reads the last cells
+// of the previous z row instead, so it is scaled by a 0/1 lane mask: an FMA by
+// 1 is the plain sum, an FMA by 0 leaves the accumulator as it is.  The x axis
+// (cells 16 lanes apart) merges in the tile read-out of p2g_group, which sums
+// the x sources of each tile node from the LDS staging area.  Every sum keeps
+// the order own offset 0, then the cell below's offset 1, then the cell two
+// below's offset 2.
 template <int N>
-__device__ __forceinline__ uint32_t lane_shr_u32(uint32_t v) {
-  if constexpr (N >= 16) return (uint32_t)__shfl_up((int)v, N);
-  else return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x110 + N, 0xF, 0xF, true);
-}
-template <int N>
-__device__ __forceinline__ float lane_shr(float v) { return __uint_as_float(lane_shr_u32<N>(__float_as_uint(v))); }
-template <int N>
-__device__ __forceinline__ unsigned long long lane_shr(unsigned long long v) {
-  const uint32_t lo = lane_shr_u32<N>((uint32_t)v), hi = lane_shr_u32<N>((uint32_t)(v >> 32));
-  return ((unsigned long long)hi << 32) | lo;
+__device__ __forceinline__ float row_shr(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x110 + N, 0xF, 0xF, true));
 }
 
-// One axis of the merge.  v[27] is indexed (a, b, c) -> 9a + 3b + c, the axis
-// being reduced has index stride S and offsets 0, 1, 2 (node = cell + offset
-// along it, cell coordinate `coord` in 0..3).  Afterwards, along that axis,
-// slot 0 holds node `coord` (all contributions: own offset 0, the cell below's
-// offset 1, the cell two below's offset 2), and slots 1, 2 hold nodes 4 and 5,
-// complete only on lanes with coord == 3.
-template <typename T, int S, int SH1, int SH2>
-__device__ __forceinline__ void merge_axis(T (&v)[27], int coord) {
+// After the merge along an axis, slot 0 of the axis holds node `coord` of the
+// lane's cell (all its contributions from this axis) and slots 1, 2 hold
+// nodes 4 and 5, complete only on lanes with coord == 3.
+__device__ __forceinline__ void merge_z(float (&v)[27], float m1, float m2) {
 #pragma unroll
-  for (int i = 0; i < 27; ++i) {
-    if ((i / S) % 3 != 0) continue;
-    const T w0 = v[i], w1 = v[i + S], w2 = v[i + 2 * S];
-    const T a1 = lane_shr<SH1>(w1), a2 = lane_shr<SH2>(w2), b2 = lane_shr<SH1>(w2);
-    v[i] = w0 + (coord >= 1 ? a1 : T(0)) + (coord >= 2 ? a2 : T(0));
-    v[i + S] = w1 + (coord >= 1 ? b2 : T(0));
-    v[i + 2 * S] = w2;
+  for (int i = 0; i < 27; i += 3) {
+    const float w0 = v[i], w1 = v[i + 1], w2 = v[i + 2];
+    v[i] = fmaf(row_shr<2>(w2), m2, fmaf(row_shr<1>(w1), m1, w0));
+    v[i + 1] = fmaf(row_shr<1>(w2), m1, w1);
   }
 }
-
-template <typename T>
-__device__ __forceinline__ void p2g_merge(T (&v)[27], int ci, int cj, int ck) {
-  merge_axis<T, 1, 1, 2>(v, ck);     // z: neighbouring lanes
-  merge_axis<T, 3, 4, 8>(v, cj);     // y: lanes 4 apart in the row
-  merge_axis<T, 9, 16, 32>(v, ci);   // x: lanes 16 apart
+__device__ __forceinline__ void merge_y(float (&v)[27]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int i = 9 * a + c;
+      const float w0 = v[i], w1 = v[i + 3], w2 = v[i + 6];
+      v[i] = (w0 + row_shr<4>(w1)) + row_shr<8>(w2);
+      v[i + 3] = w1 + row_shr<4>(w2);
+    }
 }
 
-// Tile node (0..5 per axis) held in merged slot (xs, ys, zs) by lane (ci, cj,
-// ck): slot 0 -> the cell coordinate, slots 1, 2 -> 4, 5 (owned by coord == 3).
-__device__ __forceinline__ bool slot_node(int xs, int ys, int zs, int ci, int cj, int ck, int& tx, int& ty, int& tz) {
-  tx = xs == 0 ? ci : 3 + xs;
-  ty = ys == 0 ? cj : 3 + ys;
-  tz = zs == 0 ? ck : 3 + zs;
-  return (xs == 0 || ci == 3) && (ys == 0 || cj == 3) && (zs == 0 || ck == 3);
+// Staging area of a channel, laid out by tile x: slot (xs, ys, zs) of the lane
+// with cell (ci, cj, ck) holds node x = ci + xs, and goes to word
+// ((9 xs + 3 ys + zs) * 6 + ci + xs) * 16 + 4 cj + ck.  In each slot's block of
+// six x rows, the two rows no lane writes (x < xs or x > xs + 3) stay zero, so
+// the read-out of tile node (tx, ty, tz) sums its three x sources, the words
+// B + 864 xs with B = ((3 ys + zs) * 6 + tx) * 16 + 4 oj + ok (oj = min(ty, 3),
+// ys = ty - oj, likewise for z), without masks: a source that does not exist
+// reads an exact zero.
+constexpr int P2G_STAGE_WORDS = 27 * 96;
+
+// Per-lane constants of the read-out of tile positions p = lane + 64 r (r =
+// 0..3, tile_pos order, p < 216): B of each position (two per word, 16 bits
+// each) and, per position, the bit of its node block offset (tx, ty, tz) >> 2.
+struct P2GReadout {
+  unsigned b01, b23;
+  unsigned bits;    // 8 bits per r: 1 << block offset index, 0 past 216
+};
+
+__device__ __forceinline__ P2GReadout p2g_readout(int lane) {
+  P2GReadout ro;
+  unsigned b[4];
+  ro.bits = 0u;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    int tx, ty, tz;
+    tile_node_at(min(lane + 64 * r, PT3 - 1), tx, ty, tz);
+    const int oj = min(ty, 3), ok = min(tz, 3);
+    b[r] = (unsigned)((((ty - oj) * 3 + (tz - ok)) * 6 + tx) * 16 + oj * 4 + ok);
+    if (lane + 64 * r < PT3) ro.bits |= (1u << ((((tx >> 2) * 2) + (ty >> 2)) * 2 + (tz >> 2))) << (8 * r);
+  }
+  ro.b01 = b[0] | (b[1] << 16);
+  ro.b23 = b[2] | (b[3] << 16);
+  return ro;
+}
+
+__device__ __forceinline__ int readout_base(const P2GReadout& ro, int r) {
+  const unsigned w = r < 2 ? ro.b01 : ro.b23;
+  return (int)((r & 1) ? w >> 16 : w & 0xFFFFu);
 }
 
 // Merged tile of block e: listed blocks below tile_cap store it (all 216
 // nodes, zeros included) for the grid kernel to gather; later blocks add it to
-// the grid accumulators with atomics.  A tile leaves through the wavefront's
-// LDS staging area: every lane writes its 27 merged slots to stage[n*64 + lane]
-// (no lane masks; a slot the lane does not own is never read), then each lane
-// reads tile nodes lane, lane + 64, ... from their owners' words (rd[], fixed
-// per lane) and the 216 nodes go out with four full-wavefront stores.
+// the grid accumulators with atomics.  Per channel, the lanes' z/y-merged
+// stencils go to the wavefront's LDS staging area (stage[slot*64 + lane]); each
+// lane then reads out tile positions lane, lane + 64, ..., summing their x
+// sources, and the 216 nodes leave with four full-wavefront stores.
 template <int G>
 __device__ __forceinline__ void p2g_group(const P2GItem& it, const DevPtrs& d, const SceneDev& sc, int e, int B0i,
-                                          int B0j, int B0k, int ci, int cj, int ck, unsigned& touched,
-                                          float* stage_f, const int (&rd)[4]) {
+                                          int B0j, int B0k, float mz1, float mz2, unsigned& touched,
+                                          float* stage_f, const P2GReadout& ro) {
   const bool tiled = e < d.tile_cap;
   const int lane = threadIdx.x;
   float acc[27][p2g_nch(G)];
@@ -1446,46 +1526,61 @@ __device__ __forceinline__ void p2g_group(const P2GItem& it, const DevPtrs& d, c
   for (int n = 0; n < 27; ++n)
 #pragma unroll
     for (int c = 0; c < p2g_nch(G); ++c) acc[n][c] = 0.0f;
-  p2g_accumulate<G>(it, acc);
-  {
-    // acc channel c goes to grid channel ch0 + c (group 0: latent 6, shear,
-    // mass = A_PIG + 6, A_SHR, A_MASS)
-    constexpr int ch0 = (G == 1 || G == 6) ? A_MOM : (G == 2 ? A_FRC : (G == 0 ? A_PIG + 6 : A_PIG + 3 * (G - 3)));
-    static_assert(A_SHR == A_PIG + 7 && A_MASS == A_SHR + 1, "group 0's channels are contiguous");
-    constexpr int nch = p2g_nch(G);
+  p2g_accumulate<G>(it, acc, stage_f);
+  // the ring's words become staging words again: zero them (rows that no lane
+  // writes in the staging layout must read zero)
 #pragma unroll
-    for (int c = 0; c < nch; ++c) {
-      float v[27];
+  for (int i = 0; i < 2 * P2G_RING_ROWS * 64 / 256; ++i)
+    *(float4*)(stage_f + 256 * i + 4 * lane) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  // acc channel c goes to grid channel ch0 + c (group 0: latent 6, shear,
+  // mass = A_PIG + 6, A_SHR, A_MASS)
+  constexpr int ch0 = (G == 1 || G == 6) ? A_MOM : (G == 2 ? A_FRC : (G == 0 ? A_PIG + 6 : A_PIG + 3 * (G - 3)));
+  static_assert(A_SHR == A_PIG + 7 && A_MASS == A_SHR + 1, "group 0's channels are contiguous");
+  constexpr int nch = p2g_nch(G);
+  // all channels are read out before any of them is stored: LDS accesses
+  // after a direct-to-LDS load wait for every outstanding vector memory
+  // operation, so a store issued between two channels' staging would be
+  // waited for
+  float out[nch][4];
 #pragma unroll
-      for (int n = 0; n < 27; ++n) v[n] = acc[n][c];
-      p2g_merge(v, ci, cj, ck);
-      if (G == 0 && c == 2) {
-        // node blocks receiving mass from this block
+  for (int c = 0; c < nch; ++c) {
+    float v[27];
 #pragma unroll
-        for (int n = 0; n < 27; ++n) {
+    for (int n = 0; n < 27; ++n) v[n] = acc[n][c];
+    merge_z(v, mz1, mz2);
+    merge_y(v);
+    // lane (ci, cj, ck) = 16 ci + 4 cj + ck writes slot n = (xs, ys, zs) at x
+    // row ci + xs of the slot's block
+#pragma unroll
+    for (int n = 0; n < 27; ++n) stage_f[n * 96 + (n / 9) * 16 + lane] = v[n];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int B = readout_base(ro, r);
+      out[c][r] = (stage_f[B] + stage_f[B + 864]) + stage_f[B + 1728];
+    }
+    __syncthreads();
+  }
+  if (G == 0) {
+    // node blocks receiving mass from this block (channel 2 is the mass)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) touched |= out[2][r] != 0.0f ? (ro.bits >> (8 * r)) & 0xFFu : 0u;
+  }
+#pragma unroll
+  for (int c = 0; c < nch; ++c) {
+    if (tiled) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (lane + 64 * r < PT3) d.tile_acc[((size_t)e * ACC_CH + ch0 + c) * PT3 + lane + 64 * r] = out[c][r];
+    } else {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (lane + 64 * r < PT3 && out[c][r] != 0.0f) {
           int tx, ty, tz;
-          if (slot_node(n / 9, (n / 3) % 3, n % 3, ci, cj, ck, tx, ty, tz) && v[n] != 0.0f)
-            touched |= 1u << ((((tx >> 2) * 2) + (ty >> 2)) * 2 + (tz >> 2));
-        }
-      }
-      if (tiled) {
-#pragma unroll
-        for (int n = 0; n < 27; ++n) stage_f[n * 64 + lane] = v[n];
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          if (lane + 64 * r < PT3) d.tile_acc[((size_t)e * ACC_CH + ch0 + c) * PT3 + lane + 64 * r] = stage_f[rd[r]];
-        __syncthreads();
-      } else {
-#pragma unroll
-        for (int n = 0; n < 27; ++n) {
-          int tx, ty, tz;
-          const bool own = slot_node(n / 9, (n / 3) % 3, n % 3, ci, cj, ck, tx, ty, tz);
-          if (own && v[n] != 0.0f) {
-            const int gi = B0i + tx, gj = B0j + ty, gk = B0k + tz;
-            atomicAdd(d.acc + ((size_t)node_block_of(sc, gi, gj, gk) * ACC_CH + ch0 + c) * BLK3 + node_local(gi, gj, gk),
-                      v[n]);
-          }
+          tile_node_at(lane + 64 * r, tx, ty, tz);
+          const int gi = B0i + tx, gj = B0j + ty, gk = B0k + tz;
+          atomicAdd(d.acc + ((size_t)node_block_of(sc, gi, gj, gk) * ACC_CH + ch0 + c) * BLK3 + node_local(gi, gj, gk),
+                    out[c][r]);
         }
       }
     }
@@ -1499,31 +1594,42 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                                             const int* __restrict__ perm, const int4* __restrict__ pdesc,
                                             const int* __restrict__ cell_num, const float* __restrict__ PB) {
   constexpr int NG = MUSL ? 1 : P2G_GROUPS;
-  // tile staging area of this workgroup's one wavefront: 27 slots x 64 lanes
-  __shared__ float stage[27 * 64];
+  // tile staging area of this workgroup's one wavefront (its rows that no lane
+  // writes are zeroed once)
+  __shared__ float stage[P2G_STAGE_WORDS];
   const int nblk = d.counters[CNT_PBLOCKS];
   const int parity = d.counters[CNT_GSTEP] & 1;
   const int lane = threadIdx.x;
   const int ci = lane >> 4, cj = (lane >> 2) & 3, ck = lane & 3;
-  // staging word of the owner of the tile node stored at position lane + 64 r
-  // (tile_pos order): the cell min(t, 3) per axis holds node t in merged slot
-  // t - min(t, 3)
-  int rd[4];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    int tx, ty, tz;
-    tile_node_at(min(lane + 64 * r, PT3 - 1), tx, ty, tz);
-    const int oi = min(tx, 3), oj = min(ty, 3), ok = min(tz, 3);
-    rd[r] = ((tx - oi) * 9 + (ty - oj) * 3 + (tz - ok)) * 64 + oi * 16 + oj * 4 + ok;
-  }
-  // items of this XCD: its contiguous segment of the block list x 6 groups
+  // read-out of the tile positions this lane stores, and the z-merge masks
+  // (a z shift into a lane with ck < 1 or ck < 2 has no source in its row)
+  const P2GReadout ro = p2g_readout(lane);
+  for (int i = lane; i < P2G_STAGE_WORDS; i += 64) stage[i] = 0.0f;
+  __syncthreads();
+  const float mz1 = ck >= 1 ? 1.0f : 0.0f, mz2 = ck >= 2 ? 1.0f : 0.0f;
+  // Items of this XCD: its contiguous segment of the block list x the groups.
+  // Items differ several-fold in cost (cell depth, group), so after
+  // P2G_STATIC static items (walker index w, w + W, ... for W walkers) a
+  // wavefront takes its items from the XCD's queue: claim c is item
+  // P2G_STATIC * W + c.  The claim for the item after next is made when an
+  // item ends, so it has a whole item to return, and the wavefronts do not all
+  // claim at once at the start.  The static items assume that every walker is
+  // resident from the start: the launch is sized to the kernel's occupancy.
+  // The last wavefront of the XCD to finish puts the queue back to zero for
+  // the next launch (the grid is a multiple of 8 workgroups; workgroup w
+  // belongs to walker w % 8).
   const int nx = 8;
   const int xcd = blockIdx.x % nx;
-  const int per = gridDim.x / nx;
+  const int walkers = gridDim.x / nx;
   const int seg = (nblk + nx - 1) / nx;
   const int e0 = xcd * seg;
   const int nitems = max(0, min(nblk, e0 + seg) - e0) * NG;
-  for (int item = blockIdx.x / nx; item < nitems; item += per) {
+  int* const q_next = d.wq + xcd * CQ_CNT_STRIDE;
+  int* const q_done = d.wq + (CQ_SEGS + xcd) * CQ_CNT_STRIDE;
+  int item = (int)(blockIdx.x / nx);
+  int claim = -1;   // lane 0: the claim made when the previous item ended
+  int nstarted = 1;  // items started so far
+  while (item < nitems) {
     const int e = e0 + item / NG;
     const int group = MUSL ? 6 : item % NG;
     // descriptor (key, first slot, count, scene) and this lane's cell, all by list position
@@ -1556,14 +1662,14 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     }
     unsigned touched = 0u;
     if constexpr (MUSL) {
-      p2g_group<6>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched, stage, rd);
+      p2g_group<6>(it, d, sc, e, B0i, B0j, B0k, mz1, mz2, touched, stage, ro);
     } else {
       switch (group) {
-        case 0: p2g_group<0>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched, stage, rd); break;
-        case 1: p2g_group<1>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched, stage, rd); break;
-        case 2: p2g_group<2>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched, stage, rd); break;
-        case 3: p2g_group<3>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched, stage, rd); break;
-        default: p2g_group<4>(it, d, sc, e, B0i, B0j, B0k, ci, cj, ck, touched, stage, rd); break;
+        case 0: p2g_group<0>(it, d, sc, e, B0i, B0j, B0k, mz1, mz2, touched, stage, ro); break;
+        case 1: p2g_group<1>(it, d, sc, e, B0i, B0j, B0k, mz1, mz2, touched, stage, ro); break;
+        case 2: p2g_group<2>(it, d, sc, e, B0i, B0j, B0k, mz1, mz2, touched, stage, ro); break;
+        case 3: p2g_group<3>(it, d, sc, e, B0i, B0j, B0k, mz1, mz2, touched, stage, ro); break;
+        default: p2g_group<4>(it, d, sc, e, B0i, B0j, B0k, mz1, mz2, touched, stage, ro); break;
       }
     }
     if (group == 0) {
@@ -1576,6 +1682,19 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
         activate_node_block(d, nb, parity);
       }
     }
+    // the next item: a static one, or the claim made when the previous item ended
+    const int prev = __builtin_amdgcn_readfirstlane(__shfl(claim, 0));
+    const int next = nstarted < P2G_STATIC ? item + walkers : P2G_STATIC * walkers + prev;
+    ++nstarted;
+    if (nstarted >= P2G_STATIC && lane == 0) claim = atomicAdd(q_next, 1);   // the item after next
+    item = next;
+  }
+  // the last claim has returned (its value is used) before this wavefront counts itself done
+  const int last = __shfl(claim, 0);
+  if (lane == 0 && atomicAdd(q_done, 1 + (last < -1 ? 1 : 0)) == walkers - 1) {
+    // every wavefront of this walker has drawn past the end: reset for the next launch
+    atomicExch(q_next, 0);
+    atomicExch(q_done, 0);
   }
 }
 
@@ -1887,6 +2006,12 @@ void launch_closure_eval(const SceneDev* scenes, const void* mlp_blob, const flo
   const int64_t blocks = std::min<int64_t>((n + 255) / 256, max_grid);
   hipLaunchKernelGGL(k_closure_eval, dim3((unsigned)blocks), dim3(256), 0, st, scenes, (const uint8_t*)mlp_blob,
                      trial4, n, learned4, exact4, accepted);
+}
+// Workgroups of k_p2g (one wavefront each) resident per CU: the P2G launch size.
+int p2g_resident_per_cu() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_p2g<false>, 64, 0) != hipSuccess || n <= 0) return 12;
+  return n;
 }
 void launch_p2g(const DevPtrs& d, int grid, int src_is_b, hipStream_t st) {
   hipLaunchKernelGGL(k_p2g<false>, dim3(grid), dim3(64), 0, st, d, (const SceneDev*)d.scenes,

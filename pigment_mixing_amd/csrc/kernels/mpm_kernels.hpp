@@ -77,6 +77,8 @@ struct DevPtrs {
   float4* cq;              // closure queue: CQ_REC float4 per deferred plastic candidate, or null
   int64_t cq_cap;          // records it holds (the particle capacity)
   int* cq_cnt;             // records appended per XCD segment this step (CQ_SEGS, CQ_CNT_STRIDE apart)
+  int* wq;                 // P2G item queues: per XCD the next item and the finished wavefronts
+                           // (2 * CQ_SEGS counters, CQ_CNT_STRIDE apart; zero between launches)
   int64_t cap;
   int n_scenes;
   int nb_list_cap;
@@ -145,6 +147,7 @@ void launch_closure(const DevPtrs& d, const void* mlp_blob, int grid, hipStream_
 void launch_closure_eval(const SceneDev* scenes, const void* mlp_blob, const float* trial4, int64_t n,
                          float* learned4, float* exact4, uint8_t* accepted, int max_grid, hipStream_t st);
 void launch_p2g(const DevPtrs& d, int grid, int src_is_b, hipStream_t st);
+int p2g_resident_per_cu();
 void launch_grid(const DevPtrs& d, int grid, hipStream_t st);
 // the last step's grid update again from its tiles with the current scene
 // constants (after a dt change; not after an atomic scatter)

@@ -80,7 +80,8 @@ def test_autotune_keeps_results():
     r = ha.autotune(2)
     log(f"[batch] autotune: p2g {r.wg_p2g}/CU ({r.ms_p2g:.4f} ms) g2p {r.wg_g2p}/CU ({r.ms_g2p:.4f} ms) "
         f"grid {r.wg_grid}/CU ({r.ms_grid:.4f} ms), {r.trials} trials, {r.steps} steps")
-    assert r.wg_p2g in (8, 12, 16, 20, 24, 32) and r.wg_g2p in (2, 3, 4, 6, 8, 12, 16) and r.wg_grid in (2, 4, 8)
+    # P2G is tuned up to its occupancy (at most 16 one-wavefront workgroups per CU here)
+    assert 1 <= r.wg_p2g <= 16 and r.wg_g2p in (2, 3, 4, 6, 8, 12, 16) and r.wg_grid in (2, 4, 8)
     assert r.steps == r.trials * 2
     ha.step(30)
     hb.step(30 + r.steps)
