@@ -120,6 +120,7 @@ struct mpmx_handle {
   // launch shapes (workgroups per CU), set by mpmx_autotune
   int wg_g2p = 4, wg_p2g = 12, wg_grid = 4;
   int p2g_resident = 12;   // k_p2g workgroups resident per CU (its occupancy)
+  int g2p_resident = 8;    // k_g2p workgroups resident per CU
 };
 
 namespace {
@@ -590,6 +591,7 @@ int mpmx_create_batch(const mpmx_config* cfgs, int n_scenes, mpmx_handle** out) 
   // P2G's first items per wavefront are static and assume every wavefront is
   // resident from the start: launch it at its occupancy
   h->wg_p2g = h->p2g_resident = p2g_resident_per_cu();
+  h->g2p_resident = g2p_resident_per_cu();
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
     return MPMX_ERR_HIP;
@@ -1015,7 +1017,7 @@ int mpmx_prepare(mpmx_handle* h) {
   ALLOC(d.nb_list, int, (size_t)2 * nb_total);
   ALLOC(d.nb_live, int, nb_total);
   ALLOC(d.counters, int, CNT_NCOUNTERS);
-  ALLOC(d.wq, int, 2 * CQ_SEGS * CQ_CNT_STRIDE);
+  ALLOC(d.wq, int, 4 * CQ_SEGS * CQ_CNT_STRIDE);
   ALLOC(d.stats, unsigned long long, ST_N);
   ALLOC(d.err, uint32_t, 4);
   ALLOC(h->d_step_zero, int, 1);
@@ -1708,7 +1710,10 @@ int mpmx_autotune(mpmx_handle* h, int steps_per_trial, mpmx_tune_report* rep) {
   std::vector<int> cand_p2g;
   for (int c : {h->p2g_resident / 2, (2 * h->p2g_resident) / 3, h->p2g_resident})
     if (c >= 1 && (cand_p2g.empty() || c != cand_p2g.back())) cand_p2g.push_back(c);
-  const std::vector<int> cand_g2p = {2, 3, 4, 6, 8, 12, 16};
+  // G2P: up to its occupancy as well (static first blocks)
+  std::vector<int> cand_g2p;
+  for (int c : {2, 3, 4, 6, 8, 12, 16})
+    if (c <= h->g2p_resident) cand_g2p.push_back(c);
   const std::vector<int> cand_grid = {2, 4, 8};
   int trials = 0;
   mpmx_kernel_times kt;

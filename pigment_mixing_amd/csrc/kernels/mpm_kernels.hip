@@ -459,6 +459,7 @@ __device__ __forceinline__ void copy_dead_share(const DevPtrs& d) {
 }
 
 // ------------------------------------------------------------------ k_g2p
+constexpr int G2P_STATIC = 4;   // blocks per workgroup assigned statically before the queue
 constexpr int GT = 6;              // gather tile edge: stencils of bins [B0, B0+4) span nodes [B0, B0+6)
 constexpr int GT3 = GT * GT * GT;  // 216 nodes
 // LDS row of one node: 16 floats padded to 5 float4 (80 B), so that the nodes
@@ -610,6 +611,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k
   __shared__ float4 gtile[GT3 * GT_ROW];
   __shared__ float sax[3][GT];       // node coordinates B0 .. B0+5 per axis
   __shared__ unsigned int plastic_wg, newton_wg;
+  __shared__ int next_blk;
   const int step = d.counters[CNT_STEP];
   const int nblk = d.counters[CNT_PBLOCKS];
   float* __restrict__ Bf = d.pb;
@@ -623,7 +625,21 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k
     cq_base = xr.first < nblk ? pdesc[xr.first].y : 0;
     cq_counter = d.cq_cnt + CQ_CNT_STRIDE * (blockIdx.x % CQ_SEGS);
   }
-  for (int e = xr.e; e < xr.end; e += xr.stride) {
+  // Blocks of this XCD's share: G2P_STATIC static ones per workgroup (walker
+  // index w, w + W, ... for W walkers), then claims from the XCD's queue
+  // (claim c is block G2P_STATIC * W + c of the share): block costs follow
+  // their particle counts.  When a block starts, thread 0 settles the next
+  // block and claims the one after it; the next block reaches the workgroup
+  // through LDS (ordered by the staging barrier).  The last workgroup of the
+  // XCD to finish resets the queue for the next launch.
+  const int walkers = xr.stride;
+  const int nshare = max(0, xr.end - xr.first);
+  int* const q_next = d.wq + (2 * CQ_SEGS + (int)(blockIdx.x % CQ_SEGS)) * CQ_CNT_STRIDE;
+  int* const q_done = d.wq + (3 * CQ_SEGS + (int)(blockIdx.x % CQ_SEGS)) * CQ_CNT_STRIDE;
+  int claim = -1;     // thread 0: the claim made when the previous block ended
+  int nstarted = 1;   // thread 0: blocks started so far
+  for (int k = xr.e - xr.first; k < nshare;) {
+    const int e = xr.first + k;
     const int4 ds = pdesc[e];       // key, first slot, count, scene
     const int key = ds.x;
     const int s = ds.w;
@@ -631,7 +647,14 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k
     int bi, bj, bk;
     block_coords(sc, key, bi, bj, bk);
     const int B0i = bi * 4, B0j = bj * 4, B0k = bk * 4;
-    __syncthreads();  // previous block done with the tile
+    __syncthreads();  // previous block done with the tile (and with next_blk)
+    // the next block: a static one, or the claim made when the previous block
+    // started; every thread reads it after this block's passes
+    if (threadIdx.x == 0) {
+      next_blk = nstarted < G2P_STATIC ? k + walkers : G2P_STATIC * walkers + claim;
+      ++nstarted;
+      if (nstarted >= G2P_STATIC) claim = atomicAdd(q_next, 1);   // the block after next
+    }
     // stage the gather grid of nodes [B0, B0+6)^3 (coalesced 16-byte rows)
     for (int q = threadIdx.x; q < GT3 * 4; q += blockDim.x) {
       const int t = q >> 2, part = q & 3;
@@ -963,6 +986,14 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k
       const uint64_t pm = __ballot(valid && plastic == 1);
       if (__lane_id() == 0 && pm) atomicAdd(&plastic_wg, (unsigned)__popcll(pm));
       if (newton) atomicAdd(&newton_wg, (unsigned)newton);
+    }
+    k = next_blk;   // written by thread 0 before the staging barrier of this block
+  }
+  if (threadIdx.x == 0) {
+    // the last claim has returned (its value is used) before this workgroup counts itself done
+    if (atomicAdd(q_done, 1 + (claim < -1 ? 1 : 0)) == walkers - 1) {
+      atomicExch(q_next, 0);
+      atomicExch(q_done, 0);
     }
   }
   if constexpr (PHASE != 1) copy_dead_share(d);
@@ -2006,6 +2037,13 @@ void launch_closure_eval(const SceneDev* scenes, const void* mlp_blob, const flo
   const int64_t blocks = std::min<int64_t>((n + 255) / 256, max_grid);
   hipLaunchKernelGGL(k_closure_eval, dim3((unsigned)blocks), dim3(256), 0, st, scenes, (const uint8_t*)mlp_blob,
                      trial4, n, learned4, exact4, accepted);
+}
+// Workgroups of k_g2p (two wavefronts each) resident per CU.
+int g2p_resident_per_cu() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_g2p<false, 0, false>, 128, 0) != hipSuccess || n <= 0)
+    return 8;
+  return n;
 }
 // Workgroups of k_p2g (one wavefront each) resident per CU: the P2G launch size.
 int p2g_resident_per_cu() {

@@ -77,8 +77,9 @@ struct DevPtrs {
   float4* cq;              // closure queue: CQ_REC float4 per deferred plastic candidate, or null
   int64_t cq_cap;          // records it holds (the particle capacity)
   int* cq_cnt;             // records appended per XCD segment this step (CQ_SEGS, CQ_CNT_STRIDE apart)
-  int* wq;                 // P2G item queues: per XCD the next item and the finished wavefronts
-                           // (2 * CQ_SEGS counters, CQ_CNT_STRIDE apart; zero between launches)
+  int* wq;                 // work queues of P2G (items) and G2P (blocks): per XCD the next claim
+                           // and the finished walkers (4 * CQ_SEGS counters, CQ_CNT_STRIDE apart;
+                           // zero between launches)
   int64_t cap;
   int n_scenes;
   int nb_list_cap;
@@ -148,6 +149,7 @@ void launch_closure_eval(const SceneDev* scenes, const void* mlp_blob, const flo
                          float* learned4, float* exact4, uint8_t* accepted, int max_grid, hipStream_t st);
 void launch_p2g(const DevPtrs& d, int grid, int src_is_b, hipStream_t st);
 int p2g_resident_per_cu();
+int g2p_resident_per_cu();
 void launch_grid(const DevPtrs& d, int grid, hipStream_t st);
 // the last step's grid update again from its tiles with the current scene
 // constants (after a dt change; not after an atomic scatter)
