@@ -1178,7 +1178,9 @@ struct P2GItem {
   const int* perm;     // permutation
   int start;           // the block's first slot
   int cnt, kmax;       // particles of this lane's cell, deepest cell of the wave
-  float X[3][3];       // node coordinates of the lane's stencil per axis
+  const float* xs;     // node coordinates of the scene (x axis, then y, then z)
+  int n0, n1, n2;      // lattice nodes per axis
+  int ix, iy, iz;      // first node of the lane's stencil per axis
   float inv_dx, apicDinv;
 };
 
@@ -1295,6 +1297,15 @@ __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[2
   int j_next = it.cnt > 1 ? it.start + kb + (int)__popcll(m & below) : it.start;   // rank k + 1
   kb += (int)__popcll(m);
   p2g_fetch<G>(P, it.PB, cap, it.perm, ld_slot(it.perm, j0), j0, j_next, ring);
+  // node coordinates of this cell's stencil (clamped for empty edge cells),
+  // loaded here so that they hold no registers outside the loop
+  float X[3][3];
+#pragma unroll
+  for (int o = 0; o < 3; ++o) {
+    X[0][o] = it.xs[min(it.ix + o, it.n0 - 1)];
+    X[1][o] = it.xs[it.n0 + min(it.iy + o, it.n1 - 1)];
+    X[2][o] = it.xs[it.n0 + it.n1 + min(it.iz + o, it.n2 - 1)];
+  }
   for (int k = 0; k < it.kmax; ++k) {
     const bool act = k < it.cnt;
     // rank k (fetched during the previous step) and the permutation entry of rank k + 1
@@ -1309,15 +1320,15 @@ __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[2
       p2g_fetch<G>(P, it.PB, cap, it.perm, src_next, j_next, j_after, ring + ((k + 1) & 1) * (P2G_RING_ROWS * 64));
       j_next = j_after;
     }
-    const float x0 = act ? cur.x[0] : it.X[0][1];
-    const float x1 = act ? cur.x[1] : it.X[1][1];
-    const float x2 = act ? cur.x[2] : it.X[2][1];
+    const float x0 = act ? cur.x[0] : X[0][1];
+    const float x1 = act ? cur.x[1] : X[1][1];
+    const float x2 = act ? cur.x[2] : X[2][1];
     float u[9], N[9];
 #pragma unroll
     for (int o = 0; o < 3; ++o) {
-      u[o] = spline_u(x0, it.X[0][o], inv_dx);
-      u[3 + o] = spline_u(x1, it.X[1][o], inv_dx);
-      u[6 + o] = spline_u(x2, it.X[2][o], inv_dx);
+      u[o] = spline_u(x0, X[0][o], inv_dx);
+      u[3 + o] = spline_u(x1, X[1][o], inv_dx);
+      u[6 + o] = spline_u(x2, X[2][o], inv_dx);
     }
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -1375,7 +1386,7 @@ __device__ __forceinline__ void p2g_accumulate(const P2GItem& it, float (&acc)[2
         float ax[3][3], ay[3][3], az[3][3];   // [offset][component]
 #pragma unroll
         for (int o = 0; o < 3; ++o) {
-          const float px = it.X[0][o] - x0, py = it.X[1][o] - x1, pz = it.X[2][o] - x2;
+          const float px = X[0][o] - x0, py = X[1][o] - x1, pz = X[2][o] - x2;
 #pragma unroll
           for (int r = 0; r < 3; ++r) {
             const float v = act ? cur.q[r] : 0.0f;
@@ -1548,8 +1559,8 @@ __device__ __forceinline__ int readout_base(const P2GReadout& ro, int r) {
 // sources, and the 216 nodes leave with four full-wavefront stores.
 template <int G>
 __device__ __forceinline__ void p2g_group(const P2GItem& it, const DevPtrs& d, const SceneDev& sc, int e, int B0i,
-                                          int B0j, int B0k, float mz1, float mz2, unsigned& touched,
-                                          float* stage_f, const P2GReadout& ro) {
+                                          int B0j, int B0k, unsigned& touched, float* stage_f,
+                                          const P2GReadout* ro_tab) {
   const bool tiled = e < d.tile_cap;
   const int lane = threadIdx.x;
   float acc[27][p2g_nch(G)];
@@ -1568,6 +1579,9 @@ __device__ __forceinline__ void p2g_group(const P2GItem& it, const DevPtrs& d, c
   constexpr int ch0 = (G == 1 || G == 6) ? A_MOM : (G == 2 ? A_FRC : (G == 0 ? A_PIG + 6 : A_PIG + 3 * (G - 3)));
   static_assert(A_SHR == A_PIG + 7 && A_MASS == A_SHR + 1, "group 0's channels are contiguous");
   constexpr int nch = p2g_nch(G);
+  const P2GReadout ro = ro_tab[lane];
+  // z-merge masks: a z shift into a lane with ck < 1 or ck < 2 has no source in its row
+  const float mz1 = (lane & 3) >= 1 ? 1.0f : 0.0f, mz2 = (lane & 3) >= 2 ? 1.0f : 0.0f;
   // all channels are read out before any of them is stored: LDS accesses
   // after a direct-to-LDS load wait for every outstanding vector memory
   // operation, so a store issued between two channels' staging would be
@@ -1632,12 +1646,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
   const int parity = d.counters[CNT_GSTEP] & 1;
   const int lane = threadIdx.x;
   const int ci = lane >> 4, cj = (lane >> 2) & 3, ck = lane & 3;
-  // read-out of the tile positions this lane stores, and the z-merge masks
-  // (a z shift into a lane with ck < 1 or ck < 2 has no source in its row)
-  const P2GReadout ro = p2g_readout(lane);
+  // read-out of the tile positions each lane stores, kept in LDS (no
+  // registers held across the items)
+  __shared__ P2GReadout ro_tab[64];
+  ro_tab[lane] = p2g_readout(lane);
   for (int i = lane; i < P2G_STAGE_WORDS; i += 64) stage[i] = 0.0f;
   __syncthreads();
-  const float mz1 = ck >= 1 ? 1.0f : 0.0f, mz2 = ck >= 2 ? 1.0f : 0.0f;
   // Items of this XCD: its contiguous segment of the block list x the groups.
   // Items differ several-fold in cost (cell depth, group), so after
   // P2G_STATIC static items (walker index w, w + W, ... for W walkers) a
@@ -1683,24 +1697,23 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) kmax = max(kmax, __shfl_xor(kmax, off));
     it.kmax = __builtin_amdgcn_readfirstlane(kmax);
-    // node coordinates of this cell's stencil (clamped for empty edge cells)
-    const float* xs = axis + sc.axis_off;
-#pragma unroll
-    for (int o = 0; o < 3; ++o) {
-      it.X[0][o] = xs[min(B0i + ci + o, sc.dims[0] - 1)];
-      it.X[1][o] = xs[sc.dims[0] + min(B0j + cj + o, sc.dims[1] - 1)];
-      it.X[2][o] = xs[sc.dims[0] + sc.dims[1] + min(B0k + ck + o, sc.dims[2] - 1)];
-    }
+    it.xs = axis + sc.axis_off;
+    it.n0 = sc.dims[0];
+    it.n1 = sc.dims[1];
+    it.n2 = sc.dims[2];
+    it.ix = B0i + ci;
+    it.iy = B0j + cj;
+    it.iz = B0k + ck;
     unsigned touched = 0u;
     if constexpr (MUSL) {
-      p2g_group<6>(it, d, sc, e, B0i, B0j, B0k, mz1, mz2, touched, stage, ro);
+      p2g_group<6>(it, d, sc, e, B0i, B0j, B0k, touched, stage, ro_tab);
     } else {
       switch (group) {
-        case 0: p2g_group<0>(it, d, sc, e, B0i, B0j, B0k, mz1, mz2, touched, stage, ro); break;
-        case 1: p2g_group<1>(it, d, sc, e, B0i, B0j, B0k, mz1, mz2, touched, stage, ro); break;
-        case 2: p2g_group<2>(it, d, sc, e, B0i, B0j, B0k, mz1, mz2, touched, stage, ro); break;
-        case 3: p2g_group<3>(it, d, sc, e, B0i, B0j, B0k, mz1, mz2, touched, stage, ro); break;
-        default: p2g_group<4>(it, d, sc, e, B0i, B0j, B0k, mz1, mz2, touched, stage, ro); break;
+        case 0: p2g_group<0>(it, d, sc, e, B0i, B0j, B0k, touched, stage, ro_tab); break;
+        case 1: p2g_group<1>(it, d, sc, e, B0i, B0j, B0k, touched, stage, ro_tab); break;
+        case 2: p2g_group<2>(it, d, sc, e, B0i, B0j, B0k, touched, stage, ro_tab); break;
+        case 3: p2g_group<3>(it, d, sc, e, B0i, B0j, B0k, touched, stage, ro_tab); break;
+        default: p2g_group<4>(it, d, sc, e, B0i, B0j, B0k, touched, stage, ro_tab); break;
       }
     }
     if (group == 0) {
