@@ -89,6 +89,33 @@ def test_autotune_keeps_results():
     assert_identical(ha.download(), hb.download(), "autotuned vs default shapes")
 
 
+@pytest.mark.parametrize("musl", [0, 1])
+def test_launch_shapes_keep_results(musl):
+    """P2G items and G2P blocks go to wavefronts first statically (four each)
+    and then through per-XCD queues; which wavefront processes what depends on
+    the launch shape and on timing, the results must not.  A 590k-particle
+    scene (~1,660 particle blocks) at one workgroup per CU (32 walkers per
+    XCD: ~6 G2P blocks and ~26 P2G items each) runs mostly from the queues;
+    shapes beyond the kernels' occupancy start some static shares late.
+    Every shape, set with mpmx_set_launch_shape, must give the default shape's
+    state bit for bit over steps that reset and reuse the queues in every
+    launch (USL, and MUSL with its second scatter)."""
+    sc = scenes.paint_4blob(cells=(48, 8, 48), dims=(192, 64, 192))
+    sc["cfg"].use_musl = musl
+    ref = scenes.make_handle(sc)
+    ref.step(12)
+    want = ref.download()
+    for shape in ((1, 1, 1), (24, 16, 8), (5, 3, 2)):
+        h = scenes.make_handle(sc)
+        h.set_launch_shape(*shape)
+        h.step(12)
+        assert h.stats().device_error == 0
+        assert_identical(h.download(), want, f"launch shape {shape} (musl={musl})")
+        h.close()
+    with pytest.raises(RuntimeError):
+        ref.set_launch_shape(0, 4, 4)
+
+
 def test_batch_mixing_drucker_prager_and_von_mises_scenes():
     """A batch holding the paint app's Drucker-Prager scene (volume rows,
     Pradhana accumulation) next to a von Mises scene (both MUSL, as a batch
