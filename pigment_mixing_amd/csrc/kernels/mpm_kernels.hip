@@ -32,6 +32,21 @@
 namespace mpmx {
 
 // ------------------------------------------------------------------ helpers
+// Direct-to-LDS loads (global_load_lds): lane l's dword lands at word l of
+// the 64-word LDS line given (a wave-uniform address), without registers.
+typedef __attribute__((address_space(1))) const void* GlobalPtr;
+typedef __attribute__((address_space(3))) void* LdsPtr;
+
+__device__ __forceinline__ void dma_dword(const void* row_base, int i, float* lds_line) {
+  const char* p = (const char*)row_base + (uint64_t)((uint32_t)i * 4u);
+  __builtin_amdgcn_global_load_lds((GlobalPtr)p, (LdsPtr)lds_line, 4, 0, 0);
+}
+
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maximum): every vector
+// memory operation of this wavefront has completed (direct-to-LDS loads have
+// landed, atomics have been performed).
+__device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
   return (__lane_id() == 0) ? 0ull : ((~0ull) >> (64 - __lane_id()));
 }
@@ -989,9 +1004,12 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k
     }
     k = next_blk;   // written by thread 0 before the staging barrier of this block
   }
+  // the last (unused) claim has completed (vmcnt(0)) before this workgroup
+  // counts itself done: a claim landing after the reset would skip a block of
+  // the next launch
+  wait_vm_all();
   if (threadIdx.x == 0) {
-    // the last claim has returned (its value is used) before this workgroup counts itself done
-    if (atomicAdd(q_done, 1 + (claim < -1 ? 1 : 0)) == walkers - 1) {
+    if (atomicAdd(q_done, 1) == walkers - 1) {
       atomicExch(q_next, 0);
       atomicExch(q_done, 0);
     }
@@ -1214,18 +1232,6 @@ __device__ __forceinline__ int ld_slot(const int* __restrict__ A, int i) {
 // the permutation entry of the rank after), lives in the first words of the
 // wavefront's staging area, which is zeroed again once the item's loop is done.
 constexpr int P2G_RING_ROWS = 16;
-typedef __attribute__((address_space(1))) const void* GlobalPtr;
-typedef __attribute__((address_space(3))) void* LdsPtr;
-
-__device__ __forceinline__ void dma_dword(const void* row_base, int i, float* lds_line) {
-  const char* p = (const char*)row_base + (uint64_t)((uint32_t)i * 4u);
-  __builtin_amdgcn_global_load_lds((GlobalPtr)p, (LdsPtr)lds_line, 4, 0, 0);
-}
-
-// s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maximum): every
-// direct-to-LDS load of this wavefront has landed.
-__device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
-
 // Lines of a buffer: 0..2 x, 3..3+NQ-1 payload, 15 the permutation entry.
 template <int G>
 __device__ __forceinline__ void p2g_fetch(const float* __restrict__ P, const float* __restrict__ PB, int64_t cap,
@@ -1733,9 +1739,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     if (nstarted >= P2G_STATIC && lane == 0) claim = atomicAdd(q_next, 1);   // the item after next
     item = next;
   }
-  // the last claim has returned (its value is used) before this wavefront counts itself done
-  const int last = __shfl(claim, 0);
-  if (lane == 0 && atomicAdd(q_done, 1 + (last < -1 ? 1 : 0)) == walkers - 1) {
+  // this wavefront's last (unused) claim has completed (vmcnt(0)) before it
+  // counts itself done: a claim landing after the reset would skip an item of
+  // the next launch
+  wait_vm_all();
+  if (lane == 0 && atomicAdd(q_done, 1) == walkers - 1) {
     // every wavefront of this walker has drawn past the end: reset for the next launch
     atomicExch(q_next, 0);
     atomicExch(q_done, 0);
