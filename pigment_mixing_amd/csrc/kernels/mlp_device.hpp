@@ -111,6 +111,29 @@ __device__ __forceinline__ mlp_f32x4 mfma_k64(const uint16_t* a_row, const uint1
   return c;
 }
 
+// The network blob (MLP_BLOB_BYTES; src and dst 16-byte aligned) into LDS by
+// the NT threads of a workgroup: 16-byte loads, eight in flight per thread
+// before their stores (a load-store loop waits on every load in turn).  No
+// branches: the indices past the blob are clamped to its last word, which the
+// threads concerned copy again with the same value (a branch would let the
+// compiler sink each load into its store's block, one wait per load again).
+template <int NT>
+__device__ __forceinline__ void mlp_blob_to_lds(const uint8_t* __restrict__ src, uint8_t* dst) {
+  static_assert(MLP_BLOB_BYTES % 16 == 0, "blob in whole 16-byte words");
+  constexpr int NV = MLP_BLOB_BYTES / 16;
+  constexpr int PER = (NV + NT - 1) / NT;
+  constexpr int CH = PER < 8 ? PER : 8;
+  const uint4* s = reinterpret_cast<const uint4*>(src);
+  uint4* t = reinterpret_cast<uint4*>(dst);
+  for (int c0 = 0; c0 < PER; c0 += CH) {
+    uint4 v[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) v[u] = s[min((int)threadIdx.x + NT * (c0 + u), NV - 1)];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) t[min((int)threadIdx.x + NT * (c0 + u), NV - 1)] = v[u];
+  }
+}
+
 struct MlpNet {
   const uint16_t* W[5];
   const float* b[5];

@@ -66,8 +66,7 @@ __global__ void __launch_bounds__(64) k_mlp_forward(const uint8_t* __restrict__ 
                                                     float* __restrict__ y4, int64_t n) {
   __shared__ __align__(16) uint8_t net_lds[MLP_BLOB_BYTES];
   __shared__ uint4 stage[64];
-  for (int i = threadIdx.x; i < MLP_BLOB_BYTES / 4; i += 64)
-    reinterpret_cast<uint32_t*>(net_lds)[i] = reinterpret_cast<const uint32_t*>(blob)[i];
+  mlp_blob_to_lds<64>(blob, net_lds);
   const MlpNet net = mlp_net(net_lds);
   const int lane = threadIdx.x;
   const int64_t row = (int64_t)blockIdx.x * 64 + lane;

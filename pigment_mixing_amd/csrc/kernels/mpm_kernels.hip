@@ -1047,8 +1047,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
   const int n = pre[CQ_SEGS];
   if ((int64_t)blockIdx.x * blockDim.x >= n) return;   // whole workgroup: nothing queued for it
   if (threadIdx.x == 0) { plastic_wg = 0u; fallback_wg = 0u; learned_wg = 0u; newton_wg = 0u; }
-  for (int i = threadIdx.x; i < MLP_BLOB_BYTES / 4; i += blockDim.x)
-    reinterpret_cast<uint32_t*>(lrn.net)[i] = reinterpret_cast<const uint32_t*>(mlp_blob)[i];
+  mlp_blob_to_lds<256>(mlp_blob, lrn.net);
   __syncthreads();
   float* __restrict__ Bf = d.pb;
   const int64_t cap = d.cap;
@@ -1131,8 +1130,7 @@ __global__ void __launch_bounds__(256) k_closure_eval(const SceneDev* __restrict
                                                       float* __restrict__ learned4, float* __restrict__ exact4,
                                                       uint8_t* __restrict__ accepted) {
   __shared__ __align__(16) G2PLearnedLds lrn;
-  for (int i = threadIdx.x; i < MLP_BLOB_BYTES / 4; i += blockDim.x)
-    reinterpret_cast<uint32_t*>(lrn.net)[i] = reinterpret_cast<const uint32_t*>(mlp_blob)[i];
+  mlp_blob_to_lds<256>(mlp_blob, lrn.net);
   __syncthreads();
   const SceneDev& sc = scenes[0];
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
