@@ -1001,7 +1001,7 @@ int mpmx_prepare(mpmx_handle* h) {
   ALLOC(d.list_pos, int, h->nkeys);
   if (learned(h)) {
     // closure queue: every live particle can be a candidate in one step
-    ALLOC(d.cq, float4, (size_t)CQ_REC * n);
+    ALLOC(d.cq, float4, (size_t)CQ_REC * ((n + CQ_GROUP - 1) / CQ_GROUP * CQ_GROUP));
     d.cq_cap = n;
     ALLOC(d.cq_cnt, int, CQ_SEGS * CQ_CNT_STRIDE);
   }

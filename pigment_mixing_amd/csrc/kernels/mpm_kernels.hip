@@ -901,13 +901,13 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k
             const int64_t slot = cq_base + base + __popcll(dm & ((1ull << __lane_id()) - 1ull));
             deferred = deferred && slot < d.cq_cap;   // a full queue maps in place
             if (deferred) {
-              float4* r = d.cq + CQ_REC * slot;
+              float4* r = d.cq + cq_word(slot, 0);
               r[0] = make_float4(__int_as_float((int)j), __int_as_float(s), eps_in, lam[0]);
-              r[1] = make_float4(lam[1], lam[2], Ft[0], Ft[1]);
-              r[2] = make_float4(Ft[2], Ft[3], Ft[4], Ft[5]);
-              r[3] = make_float4(Ft[6], Ft[7], Ft[8], V[0]);
-              r[4] = make_float4(V[1], V[2], V[3], V[4]);
-              r[5] = make_float4(V[5], V[6], V[7], V[8]);
+              r[CQ_GROUP] = make_float4(lam[1], lam[2], Ft[0], Ft[1]);
+              r[2 * CQ_GROUP] = make_float4(Ft[2], Ft[3], Ft[4], Ft[5]);
+              r[3 * CQ_GROUP] = make_float4(Ft[6], Ft[7], Ft[8], V[0]);
+              r[4 * CQ_GROUP] = make_float4(V[1], V[2], V[3], V[4]);
+              r[5 * CQ_GROUP] = make_float4(V[5], V[6], V[7], V[8]);
             }
           }
         }
@@ -1059,9 +1059,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     for (int k = 1; k < CQ_SEGS; ++k)
       if (q >= pre[k]) slot = start[k] + (q - pre[k]);
     const bool valid = q0 < n && slot < d.cq_cap;      // G2P mapped in place past the capacity
-    const float4* r = d.cq + CQ_REC * min((int64_t)slot, d.cq_cap - 1);
+    const float4* r = d.cq + cq_word(min((int64_t)slot, d.cq_cap - 1), 0);
     const float4 r0 = r[0];
-    const float2 r1 = *reinterpret_cast<const float2*>(r + 1);
+    const float2 r1 = *reinterpret_cast<const float2*>(r + CQ_GROUP);
     const int64_t j = __float_as_int(r0.x);
     const SceneDev& sc = scenes[__float_as_int(r0.y)];
     const float eps_in = r0.z;
@@ -1075,7 +1075,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     const bool ok = learned_return_map(sc, lrn, h, eps, dgr, valid, cand);
     // the trial F and V only after the network (its LDS fences keep these loads
     // below it), so that they do not hold registers across it
-    const float4 r1b = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5];
+    const float4 r1b = r[CQ_GROUP], r2 = r[2 * CQ_GROUP], r3 = r[3 * CQ_GROUP], r4 = r[4 * CQ_GROUP],
+                 r5 = r[5 * CQ_GROUP];
     const float Ft[9] = {r1b.z, r1b.w, r2.x, r2.y, r2.z, r2.w, r3.x, r3.y, r3.z};
     const float V[9] = {r3.w, r4.x, r4.y, r4.z, r4.w, r5.x, r5.y, r5.z, r5.w};
     float U[9];

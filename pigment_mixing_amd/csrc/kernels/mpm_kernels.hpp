@@ -74,7 +74,7 @@ struct DevPtrs {
   uint32_t* err;
   float* harvest;          // (trial, corrected) records of 12 floats, or null
   int64_t harvest_cap;
-  float4* cq;              // closure queue: CQ_REC float4 per deferred plastic candidate, or null
+  float4* cq;              // closure queue: CQ_REC float4 per deferred plastic candidate (cq_word), or null
   int64_t cq_cap;          // records it holds (the particle capacity)
   int* cq_cnt;             // records appended per XCD segment this step (CQ_SEGS, CQ_CNT_STRIDE apart)
   int* wq;                 // work queues of P2G (items) and G2P (blocks): per XCD the next claim
@@ -92,6 +92,15 @@ struct DevPtrs {
 // F again: {slot, scene, eps, lam0} {lam1, lam2, F0, F1} {F2..F5} {F6, F7, F8, V0}
 // {V1..V4} {V5..V8}
 constexpr int CQ_REC = 6;
+// The records are stored in groups of 64 slots, each float4 of the group's
+// records contiguous: word f of slot s at ((s / 64) * CQ_REC + f) * 64 + s % 64.
+// A wavefront's appends (consecutive slots) and the closure's reads (one slot
+// per lane) then move whole runs of 16-byte words per instruction.
+constexpr int CQ_GROUP = 64;
+__host__ __device__ __forceinline__ int64_t cq_word(int64_t slot, int f) {   // slot >= 0
+  const uint64_t u = (uint64_t)slot;
+  return (int64_t)(((u / CQ_GROUP) * CQ_REC + (uint64_t)f) * CQ_GROUP + u % CQ_GROUP);
+}
 // The queue is split into one segment per XCD walker: segment x starts at the
 // first particle slot of XCD x's contiguous share of the block list and can
 // hold every particle of that share, so it never overflows.
