@@ -184,7 +184,10 @@ typedef struct {
   int64_t active_nodes;      /* nodes with mass > 0, mean over the last mpmx_step call */
   int64_t active_blocks;     /* 4x4x4 node blocks touched, mean over the last call    */
   int64_t stray_count;       /* particles scattered outside their LDS tile, last call */
-  uint32_t device_error;     /* bit mask; nonzero <=> reference exit = 1              */
+  uint32_t device_error;     /* bit mask: 1 dgamma < 0, 2 Newton, 4 unknown boundary  *
+                              * condition (the reference's exit = 1 causes); internal *
+                              * checks: 8 a particle moved > 1 cell in a step, 16 a   *
+                              * closure-queue segment overflowed                      */
   float   ms_last_step;      /* wall ms of the last mpmx_step call / its steps        */
   int64_t learned_count;     /* learned closure: predictions accepted in the last step */
   int64_t newton_iters;      /* exact map: Newton iterations (plasticity.cpp:196-229), sum over the last call */

@@ -45,6 +45,7 @@ enum : uint32_t {
   ERR_NEWTON = 2u,           // plasticity.cpp:200-203, 219-222
   ERR_BAD_BC = 4u,           // boundary_collision.cpp:84-88
   ERR_TILE_OVERFLOW = 8u,    // internal: a particle moved > 1 cell in one step
+  ERR_QUEUE_OVERFLOW = 16u,  // internal: a closure-queue segment ran past its particle range
 };
 
 struct PlateDev {

@@ -889,27 +889,16 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k
         uint32_t err = 0;
         const float h_trial[3] = {h[0], h[1], h[2]};
         bool deferred = false;
+        uint64_t dm = 0;
+        int cq_first = 0;   // lane 0: the wavefront's first slot in the queue segment
         if constexpr (LEARNED) {
           // append the wavefront's candidates to this XCD's queue segment (one
-          // atomic per wavefront; slots in lane order)
+          // atomic per wavefront; slots in lane order).  The claim is issued
+          // here and read when the records are written, after the stress and
+          // the advection, which hide its round trip.
           deferred = valid && trial_state(sc, h, eps_in).cand;
-          const uint64_t dm = __ballot(deferred);
-          if (dm) {
-            int base = 0;
-            if (__lane_id() == 0) base = atomicAdd(cq_counter, __popcll(dm));
-            base = __shfl(base, 0);
-            const int64_t slot = cq_base + base + __popcll(dm & ((1ull << __lane_id()) - 1ull));
-            deferred = deferred && slot < d.cq_cap;   // a full queue maps in place
-            if (deferred) {
-              float4* r = d.cq + cq_word(slot, 0);
-              r[0] = make_float4(__int_as_float((int)j), __int_as_float(s), eps_in, lam[0]);
-              r[CQ_GROUP] = make_float4(lam[1], lam[2], Ft[0], Ft[1]);
-              r[2 * CQ_GROUP] = make_float4(Ft[2], Ft[3], Ft[4], Ft[5]);
-              r[3 * CQ_GROUP] = make_float4(Ft[6], Ft[7], Ft[8], V[0]);
-              r[4 * CQ_GROUP] = make_float4(V[1], V[2], V[3], V[4]);
-              r[5 * CQ_GROUP] = make_float4(V[5], V[6], V[7], V[8]);
-            }
-          }
+          dm = __ballot(deferred);
+          if (dm && __lane_id() == 0) cq_first = atomicAdd(cq_counter, __popcll(dm));
         }
         // VOL: the eps_pl_vol / eps_pl_vol_pradhana rows, read and written for
         // the scenes whose model tracks them (DPVisc, capped von Mises)
@@ -972,6 +961,26 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k
           if (!alive) { vn[0] = vn[1] = vn[2] = 0.0f; }
         }
         if (!alive || !valid) { newkey = d.dead_key; newcell = -1; }
+        if constexpr (LEARNED) {
+          // the candidates' queue records.  A segment starts at the first
+          // particle slot of its XCD's share of the block list and receives
+          // at most one record per particle of that share, so it cannot run
+          // into the next one or past the capacity; the check only guards.
+          if (dm) {
+            const int64_t slot = cq_base + __shfl(cq_first, 0) + __popcll(dm & ((1ull << __lane_id()) - 1ull));
+            if (deferred && slot < d.cq_cap) {
+              float4* r = d.cq + cq_word(slot, 0);
+              r[0] = make_float4(__int_as_float((int)j), __int_as_float(s), eps_in, lam[0]);
+              r[CQ_GROUP] = make_float4(lam[1], lam[2], Ft[0], Ft[1]);
+              r[2 * CQ_GROUP] = make_float4(Ft[2], Ft[3], Ft[4], Ft[5]);
+              r[3 * CQ_GROUP] = make_float4(Ft[6], Ft[7], Ft[8], V[0]);
+              r[4 * CQ_GROUP] = make_float4(V[1], V[2], V[3], V[4]);
+              r[5 * CQ_GROUP] = make_float4(V[5], V[6], V[7], V[8]);
+            } else if (deferred) {
+              atomicOr(&d.err[0], ERR_QUEUE_OVERFLOW);
+            }
+          }
+        }
         // ---------------- write the new state in slot j
         if (valid) {
 #pragma unroll
