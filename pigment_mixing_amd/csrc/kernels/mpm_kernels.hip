@@ -1067,7 +1067,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
 #pragma unroll
     for (int k = 1; k < CQ_SEGS; ++k)
       if (q >= pre[k]) slot = start[k] + (q - pre[k]);
-    const bool valid = q0 < n && slot < d.cq_cap;      // G2P mapped in place past the capacity
+    const bool valid = q0 < n && slot < d.cq_cap;      // (the capacity check only guards: G2P stays inside it)
     const float4* r = d.cq + cq_word(min((int64_t)slot, d.cq_cap - 1), 0);
     const float4 r0 = r[0];
     const float2 r1 = *reinterpret_cast<const float2*>(r + CQ_GROUP);
