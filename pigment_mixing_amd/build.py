@@ -21,6 +21,7 @@ LIB = os.path.join(LIBDIR, "libmpmx.so")
 DEMO = os.path.join(OBJ, "simulation_api_demo")
 ADAPTER_DEMO = os.path.join(OBJ, "adapter_style_demo")
 COPY_PROBE = os.path.join(OBJ, "hbm_copy")   # HBM-counter calibration (plain copy)
+PHASE_PROBE = os.path.join(OBJ, "phase_barrier")   # grid barrier vs graph kernel boundary
 
 SOURCES = [
     "kernels/mpm_kernels.hip",
@@ -100,11 +101,12 @@ def build(verbose=False):
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"demo build failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    probe_src = os.path.join(CSRC, "bench", "hbm_copy.hip")
-    cmd = [hipcc(), "-x", "hip", *COMMON, probe_src, "-o", COPY_PROBE]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"copy probe build failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    for name, exe in (("hbm_copy.hip", COPY_PROBE), ("phase_barrier.hip", PHASE_PROBE)):
+        probe_src = os.path.join(CSRC, "bench", name)
+        cmd = [hipcc(), "-x", "hip", *COMMON, probe_src, "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"probe build failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     if verbose:
         print("built", LIB)
     return LIB
