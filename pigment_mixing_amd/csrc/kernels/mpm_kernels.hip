@@ -658,6 +658,10 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k
     const int4 ds = pdesc[e];       // key, first slot, count, scene
     const int key = ds.x;
     const int s = ds.w;
+    // the first pass's source slots, loaded while the gather tile is staged
+    // (each pass then loads the next pass's: the particle rows of a pass wait
+    // on one round trip, not on two dependent ones)
+    int src_next = perm[ds.y + min((int)threadIdx.x, max(ds.z - 1, 0))];
     const SceneDev& sc = scenes[s];
     int bi, bj, bk;
     block_coords(sc, key, bi, bj, bk);
@@ -708,11 +712,12 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k
       // Every lane computes; a lane past the block's count recomputes the block's
       // last particle and stores nothing.
       const int64_t j = start + (valid ? t : cnt - 1);
+      const int src = src_next;
+      if (it + 1 < iters) src_next = perm[start + min(t + (int)blockDim.x, cnt - 1)];
       uint32_t okey = 0u;
       int newkey = d.dead_key, newcell = -1, plastic = 0;
       int newton = 0;
       {
-        const int src = perm[j];
         float x[3], v[3], F[9], pig[NPIG], picB[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) x[c] = A[(P_X + c) * cap + src];
