@@ -333,7 +333,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=5)
-    ap.add_argument("--harvest-steps", type=int, default=60)
+    # 80 steps of the developed flow give ~44M training pairs (~555k plastic
+    # particles per step at 1M particles), below the buffer's capacity: a full
+    # buffer would drop the last steps' pairs, the ones nearest the timed runs
+    ap.add_argument("--harvest-steps", type=int, default=80)
     ap.add_argument("--harvest-cap", type=int, default=50_000_000)
     ap.add_argument("--preroll", type=int, default=300,
                     help="untimed plain steps before both timed runs (plastic flow developed)")
