@@ -65,6 +65,10 @@ struct mpmx_handle {
   int device = 0;
   int n_cu = 256;
   hipStream_t stream = nullptr;
+  // the learned closure's stream: k_closure runs beside the sort (both follow
+  // G2P, the scatter follows both); default priority, like the main stream
+  hipStream_t stream_closure = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::vector<Scene> scenes;
   // host particle staging (upload order)
   int64_t np = 0;
@@ -383,6 +387,7 @@ DevPtrs swapped(const DevPtrs& d) {
   DevPtrs e = d;
   std::swap(e.pa, e.pb);
   std::swap(e.pa_id, e.pb_id);
+  std::swap(e.cq_cnt, e.cq_cnt_prev);
   return e;
 }
 
@@ -402,7 +407,10 @@ int grid_closure(mpmx_handle* h) { return round8(h->n_cu * 3); }
 bool learned(const mpmx_handle* h) { return h->scenes[0].cfg.closure != MPMX_CLOSURE_EXACT; }
 bool musl(const mpmx_handle* h) { return h->scenes[0].cfg.use_musl != 0; }
 // USL: one fused G2P.  MUSL (musl.cpp:6-98): first half, second scatter of
-// momentum + grid velocity, second half.
+// momentum + grid velocity, second half.  The learned closure is forked onto
+// its own stream (it reads only G2P's queue, and the sort reads nothing it
+// writes) and joined before the scatter (enqueue_join_closure), unless
+// before_closure asks for the kernels in sequence (for timing each).
 void enqueue_g2p(mpmx_handle* h, const DevPtrs& d, hipEvent_t before_closure = nullptr) {
   const int g = grid_fused(h);
   const int lrn = learned(h) ? 1 : 0;
@@ -413,8 +421,19 @@ void enqueue_g2p(mpmx_handle* h, const DevPtrs& d, hipEvent_t before_closure = n
   } else {
     launch_g2p(d, g, lrn, 0, h->stream);
   }
-  if (before_closure) (void)hipEventRecord(before_closure, h->stream);
-  if (lrn) launch_closure(d, h->closure.d_blob, grid_closure(h), h->stream);
+  if (before_closure) {
+    (void)hipEventRecord(before_closure, h->stream);
+    if (lrn) launch_closure(d, h->closure.d_blob, grid_closure(h), h->stream);
+  } else if (lrn) {
+    (void)hipEventRecord(h->ev_fork, h->stream);
+    (void)hipStreamWaitEvent(h->stream_closure, h->ev_fork, 0);
+    launch_closure(d, h->closure.d_blob, grid_closure(h), h->stream_closure);
+    (void)hipEventRecord(h->ev_join, h->stream_closure);
+  }
+}
+// The scatter reads the closure's output (F, stress): wait for its stream.
+void enqueue_join_closure(mpmx_handle* h) {
+  if (learned(h)) (void)hipStreamWaitEvent(h->stream, h->ev_join, 0);
 }
 
 // One step: G2P (A -> B, counts for the new sort), sort, P2G from B, grid update.
@@ -422,6 +441,7 @@ void enqueue_step(mpmx_handle* h, const DevPtrs& d) {
   enqueue_g2p(h, d);
   launch_scan(d, h->nkeys, 1, h->stream);
   launch_perm(d, h->np, grid_grid(h), 1, h->stream);
+  enqueue_join_closure(h);
   launch_p2g(d, grid_p2g(h), 1, h->stream);
   launch_grid(d, grid_grid(h), h->stream);
 }
@@ -592,8 +612,11 @@ int mpmx_create_batch(const mpmx_config* cfgs, int n_scenes, mpmx_handle** out) 
   // resident from the start: launch it at its occupancy
   h->wg_p2g = h->p2g_resident = p2g_resident_per_cu();
   h->g2p_resident = g2p_resident_per_cu();
-  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
-    delete h;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&h->stream_closure, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess) {
+    mpmx_destroy(h);
     return MPMX_ERR_HIP;
   }
   h->scenes.resize(n_scenes);
@@ -612,6 +635,9 @@ void mpmx_destroy(mpmx_handle* h) {
   if (h->h_counters) (void)hipHostFree(h->h_counters);
   if (h->h_stats) (void)hipHostFree(h->h_stats);
   if (h->h_err) (void)hipHostFree(h->h_err);
+  if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+  if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+  if (h->stream_closure) (void)hipStreamDestroy(h->stream_closure);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
@@ -1004,6 +1030,8 @@ int mpmx_prepare(mpmx_handle* h) {
     ALLOC(d.cq, float4, (size_t)CQ_REC * ((n + CQ_GROUP - 1) / CQ_GROUP * CQ_GROUP));
     d.cq_cap = n;
     ALLOC(d.cq_cnt, int, CQ_SEGS * CQ_CNT_STRIDE);
+    ALLOC(d.cq_cnt_prev, int, CQ_SEGS * CQ_CNT_STRIDE);
+    ALLOC(d.cq_start, int, CQ_SEGS);
   }
   ALLOC(d.count_next, int, h->nkeys);
   ALLOC(d.count_cur, int, h->nkeys);

@@ -278,28 +278,15 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_down(DevPtrs d, int nkeys
   if (blockIdx.x == nchunks - 1 && threadIdx.x == SCAN_THREADS - 1) {
     d.counters[CNT_PBLOCKS] = (int)(run & 0xffffffffull);
     d.counters[CNT_LIVE] = d.offsets[nkeys - 1];
-    // the closure queue of the step that just ended is consumed
-    if (d.cq_cnt) {
-      unsigned long long q = 0;
-      for (int x = 0; x < CQ_SEGS; ++x) {
-        q += (unsigned long long)d.cq_cnt[x * CQ_CNT_STRIDE];
-        d.cq_cnt[x * CQ_CNT_STRIDE] = 0;
-      }
-      d.stats[ST_QUEUED] += q;
+    // the closure may still be reading this step's queue counts (it runs
+    // beside the sort); the previous step's are consumed: clear them for the
+    // step after this one, which appends to that buffer again
+    if (d.cq_cnt_prev) {
+      for (int x = 0; x < CQ_SEGS; ++x) d.cq_cnt_prev[x * CQ_CNT_STRIDE] = 0;
     }
     if (advance) {
       d.counters[CNT_STEP] += 1;
       d.counters[CNT_GSTEP] += 1;
-      // per-step counters of the step that just ended
-      d.stats[ST_PLASTIC_LAST] = d.stats[ST_PLASTIC_CUR];
-      d.stats[ST_PLASTIC] += d.stats[ST_PLASTIC_CUR];
-      d.stats[ST_PLASTIC_CUR] = 0;
-      d.stats[ST_FALLBACK_LAST] = d.stats[ST_FALLBACK_CUR];
-      d.stats[ST_FALLBACK] += d.stats[ST_FALLBACK_CUR];
-      d.stats[ST_FALLBACK_CUR] = 0;
-      d.stats[ST_LEARNED_LAST] = d.stats[ST_LEARNED_CUR];
-      d.stats[ST_LEARNED] += d.stats[ST_LEARNED_CUR];
-      d.stats[ST_LEARNED_CUR] = 0;
     }
   }
 }
@@ -639,6 +626,9 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k
   if constexpr (LEARNED) {
     cq_base = xr.first < nblk ? pdesc[xr.first].y : 0;
     cq_counter = d.cq_cnt + CQ_CNT_STRIDE * (blockIdx.x % CQ_SEGS);
+    // the segment's start for the closure, which runs beside the next sort
+    // and so cannot read the block list itself (the sort rewrites it)
+    if (blockIdx.x < CQ_SEGS && threadIdx.x == 0) d.cq_start[blockIdx.x] = (int)cq_base;
   }
   // Blocks of this XCD's share: G2P_STATIC static ones per workgroup (walker
   // index w, w + W, ... for W walkers), then claims from the XCD's queue
@@ -1047,18 +1037,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
   __shared__ __align__(16) G2PLearnedLds lrn;
   __shared__ unsigned int plastic_wg, fallback_wg, learned_wg, newton_wg;
   // the XCD segments, concatenated: record q of the whole queue lies in segment
-  // x with pre[x] <= q < pre[x + 1], at slot start[x] + q - pre[x]
-  const int nblk = d.counters[CNT_PBLOCKS];
-  const int segb = (nblk + CQ_SEGS - 1) / CQ_SEGS;
+  // x with pre[x] <= q < pre[x + 1], at slot start[x] + q - pre[x].  Starts and
+  // counts come from G2P (the block list is being rebuilt by the sort, which
+  // runs beside this kernel).
   int pre[CQ_SEGS + 1], start[CQ_SEGS];   // slots and counts are below the particle capacity (int)
   pre[0] = 0;
 #pragma unroll
   for (int x = 0; x < CQ_SEGS; ++x) {
-    const bool any = x * segb < nblk;
-    start[x] = any ? d.pdesc[x * segb].y : 0;
-    pre[x + 1] = pre[x] + (any ? d.cq_cnt[x * CQ_CNT_STRIDE] : 0);
+    start[x] = d.cq_start[x];
+    pre[x + 1] = pre[x] + d.cq_cnt[x * CQ_CNT_STRIDE];
   }
   const int n = pre[CQ_SEGS];
+  if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0) atomicAdd(&d.stats[ST_QUEUED], (unsigned long long)n);
   if ((int64_t)blockIdx.x * blockDim.x >= n) return;   // whole workgroup: nothing queued for it
   if (threadIdx.x == 0) { plastic_wg = 0u; fallback_wg = 0u; learned_wg = 0u; newton_wg = 0u; }
   mlp_blob_to_lds<256>(mlp_blob, lrn.net);
@@ -1777,7 +1767,20 @@ __global__ void __launch_bounds__(256) k_grid(DevPtrs d) {
   __syncthreads();
   const int step = d.counters[CNT_STEP];
   const int parity = d.counters[CNT_GSTEP] & 1;
-  if (!REDO && blockIdx.x == 0 && threadIdx.x == 0) d.counters[CNT_NBLOCKS0 + (parity ^ 1)] = 0;
+  if (!REDO && blockIdx.x == 0 && threadIdx.x == 0) {
+    d.counters[CNT_NBLOCKS0 + (parity ^ 1)] = 0;
+    // per-step counters of the step's G2P and closure (both done: the grid
+    // update follows the scatter, which follows both)
+    d.stats[ST_PLASTIC_LAST] = d.stats[ST_PLASTIC_CUR];
+    d.stats[ST_PLASTIC] += d.stats[ST_PLASTIC_CUR];
+    d.stats[ST_PLASTIC_CUR] = 0;
+    d.stats[ST_FALLBACK_LAST] = d.stats[ST_FALLBACK_CUR];
+    d.stats[ST_FALLBACK] += d.stats[ST_FALLBACK_CUR];
+    d.stats[ST_FALLBACK_CUR] = 0;
+    d.stats[ST_LEARNED_LAST] = d.stats[ST_LEARNED_CUR];
+    d.stats[ST_LEARNED] += d.stats[ST_LEARNED_CUR];
+    d.stats[ST_LEARNED_CUR] = 0;
+  }
   const int nblk = d.counters[CNT_NBLOCKS0 + parity];
   const bool scattered = d.counters[CNT_PBLOCKS] > d.tile_cap;   // P2G used its atomic path
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;

@@ -77,6 +77,9 @@ struct DevPtrs {
   float4* cq;              // closure queue: CQ_REC float4 per deferred plastic candidate (cq_word), or null
   int64_t cq_cap;          // records it holds (the particle capacity)
   int* cq_cnt;             // records appended per XCD segment this step (CQ_SEGS, CQ_CNT_STRIDE apart)
+  int* cq_cnt_prev;        // the same counts of the step before (the other of two buffers, which
+                           // alternate with the particle buffers): the sort clears them
+  int* cq_start;           // this step's first queue slot per XCD segment (CQ_SEGS), written by G2P
   int* wq;                 // work queues of P2G (items) and G2P (blocks): per XCD the next claim
                            // and the finished walkers (4 * CQ_SEGS counters, CQ_CNT_STRIDE apart;
                            // zero between launches)
