@@ -40,7 +40,8 @@
 //     U diag(e^h_new) V^T.
 //
 // n_threads == 1 gives the deterministic sequential order used for parity
-// (the reference itself is only reproducible single-threaded, survey §4).
+// (the reference itself is only reproducible single-threaded: its threads'
+// grid replicas are merged, and its force atomics land, in timing order).
 // n_threads > 1 keeps the reference's OpenMP cost structure (per-thread full
 // grid replicas merged under a critical section, atomic force scatter) for the
 // CPU baseline.
@@ -76,7 +77,7 @@ struct KState {
 
 // ---------------------------------------------------------------- splines
 // tools.hpp:153-177.  The constants are double literals in the reference, so
-// the polynomial is evaluated in double and rounded to float (survey quirk 5).
+// the polynomial is evaluated in double and rounded to float.
 inline float spline_N(float u) {
   float uabs = std::fabs(u);
   if (uabs < 0.5f) return (float)(0.75 - (double)(uabs * uabs));

@@ -3,7 +3,7 @@
 // pose at t_n, velocity (p(t_n) - p(t_{n-1}))/dt, angular velocity from the
 // rotation between the two poses divided by dt.  The reference sets these at
 // the end of the previous step; step 0 uses the same difference rule (the
-// reference leaves it uninitialised, survey quirk 1).
+// reference leaves it uninitialised).
 #pragma once
 #include <vector>
 #include <stdint.h>
