@@ -217,6 +217,33 @@ def test_learned_closure_in_step(mode):
     assert ex <= 1e-3 and eF <= 1e-3
 
 
+def test_closure_beside_the_sort_changes_nothing():
+    """The learned step runs k_closure on a stream of its own, beside the sort,
+    inside the step graph (one graph launch runs two steps); the event-timed
+    path (mpmx_profile_steps) runs the same kernels one after another.  From
+    the same state, 24 steps each way download bit-identical state and report
+    the same closure counts for the last step (the per-step statistics roll
+    over after both kernels)."""
+    s = plastic_scene()
+    params, norm, st, _ = trained_closure(s, train_steps=800)
+    ha = handle_from_state(s, st, closure=2, params=params, norm=norm)
+    hb = handle_from_state(s, st, closure=2, params=params, norm=norm)
+    ha.step(24)
+    hb.profile(24)
+    sa, sb = ha.stats(), hb.stats()
+    log(f"[closure] beside the sort: accepted {sa.learned_count} fallback {sa.fallback_count}; "
+        f"in sequence: accepted {sb.learned_count} fallback {sb.fallback_count}")
+    assert sa.device_error == 0 and sb.device_error == 0
+    assert sa.step == sb.step == 24
+    assert sa.learned_count > 0
+    assert (sa.learned_count, sa.fallback_count, sa.plastic_count) == (sb.learned_count, sb.fallback_count,
+                                                                       sb.plastic_count)
+    a, b = ha.download(), hb.download()
+    for k in ("x", "v", "F", "C", "pig", "diff", "eps", "dgamma", "active"):
+        diff = np.count_nonzero(a[k] != b[k])
+        assert diff == 0, f"{diff} values of {k} differ"
+
+
 def test_learned_closure_gain_on_newton_material():
     """With a rate exponent of 2 the exact map iterates Newton (plasticity.cpp:196-229).
     The refined closure starts from the network's increment and accepts after one
