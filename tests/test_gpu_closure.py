@@ -217,21 +217,24 @@ def test_learned_closure_in_step(mode):
     assert ex <= 1e-3 and eF <= 1e-3
 
 
-def test_closure_beside_the_sort_changes_nothing():
+@pytest.mark.parametrize("musl", [0, 1])
+def test_closure_beside_the_sort_changes_nothing(musl):
     """The learned step runs k_closure on a stream of its own, beside the sort,
     inside the step graph (one graph launch runs two steps); the event-timed
     path (mpmx_profile_steps) runs the same kernels one after another.  From
     the same state, 24 steps each way download bit-identical state and report
     the same closure counts for the last step (the per-step statistics roll
-    over after both kernels)."""
+    over after both kernels).  USL, and MUSL, whose second half-step queues
+    the candidates."""
     s = plastic_scene()
+    s["cfg"].use_musl = musl
     params, norm, st, _ = trained_closure(s, train_steps=800)
     ha = handle_from_state(s, st, closure=2, params=params, norm=norm)
     hb = handle_from_state(s, st, closure=2, params=params, norm=norm)
     ha.step(24)
     hb.profile(24)
     sa, sb = ha.stats(), hb.stats()
-    log(f"[closure] beside the sort: accepted {sa.learned_count} fallback {sa.fallback_count}; "
+    log(f"[closure] musl {musl} beside the sort: accepted {sa.learned_count} fallback {sa.fallback_count}; "
         f"in sequence: accepted {sb.learned_count} fallback {sb.fallback_count}")
     assert sa.device_error == 0 and sb.device_error == 0
     assert sa.step == sb.step == 24
