@@ -19,7 +19,9 @@
  *       scalars   : n floats
  *       active    : n bytes (0/1)
  *   - Device memory is owned by the handle.  One handle per host thread; calls on
- *     one handle are serialised on the handle's HIP stream.  Not reentrant, like
+ *     one handle are serialised on the handle's HIP stream (a learned-closure
+ *     step forks its closure onto a second stream of the handle and joins it
+ *     within the step).  Not reentrant, like
  *     the reference's single GL-thread use (raytracing/src/main.cpp:270-285).
  *   - FP32 throughout (matter/src/tools.hpp:19, `typedef float T`), 3-D, quadratic
  *     B-spline (tools.hpp:20-21).
